@@ -1,0 +1,333 @@
+"""GPU collision backend: a ``BaseProblem`` whose collision checks run in libmrcc (gfx950).
+
+Per-call methods keep the reference signatures (planning_env.py:1724-1881) so planners call them
+unchanged; each call is one kernel launch on inputs staged through pinned memory.  Batch methods
+(``batch_is_collision_free``, ``batch_is_edge_collision_free``) and ``is_path_collision_free`` issue
+one launch for many configurations / edges.  PyTorch provides device buffers and the stream.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import EdgeBatch, Tune, check, dptr, iptr, lib
+from .configuration import Configuration, NpConfiguration, config_dist
+from .planning_env import BaseProblem, Mode, State, generate_binary_search_indices
+from .scenes import SceneDesc
+
+ALL_ROBOTS = 0xFFFFFFFF
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("libmrcc batch calls need a HIP device (torch.cuda.is_available() is False)")
+    return torch
+
+
+def _stream_handle():
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Scene:
+    """Owner of one ``mrcc_scene`` handle built from a ``SceneDesc``."""
+
+    def __init__(self, desc: SceneDesc, device: int = 0):
+        L = lib()
+        check(L.mrcc_init(device))
+        self.desc = desc
+        self.device = device
+        h = ctypes.c_void_p()
+        f64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        if desc.kind == "disks":
+            rr, sc, sr = f64(desc.robot_radius), f64(desc.sphere_center), f64(desc.sphere_radius)
+            rc, rs = f64(desc.rect_center), f64(desc.rect_size)
+            keep = [rr, sc, sr, rc, rs]
+            check(L.mrcc_scene_create_disks(len(rr), desc.robot_dim, dptr(rr), len(sr), dptr(sc), dptr(sr),
+                                            len(rc), dptr(rc), dptr(rs), ctypes.byref(h)))
+        elif desc.kind == "planar":
+            rh, op, oh = f64(desc.robot_half), f64(desc.obstacle_pose), f64(desc.obstacle_half)
+            keep = [rh, op, oh]
+            check(L.mrcc_scene_create_planar_boxes(len(rh), dptr(rh), len(op), dptr(op), dptr(oh),
+                                                   ctypes.byref(h)))
+        elif desc.kind == "arms":
+            b, dh, sl, bx = f64(desc.base), f64(desc.dh), f64(desc.sphere_local), f64(desc.boxes)
+            cnt = np.ascontiguousarray(desc.link_sphere_count, dtype=np.int32)
+            keep = [b, dh, sl, bx, cnt]
+            check(L.mrcc_scene_create_arms(desc.n_arms, desc.dof, dptr(b), dptr(dh), iptr(cnt), dptr(sl),
+                                           len(bx), dptr(bx), ctypes.byref(h)))
+        else:
+            raise ValueError(f"unknown scene kind {desc.kind}")
+        del keep
+        self.handle = h
+        if desc.kind == "arms" and desc.modes is not None:
+            self.set_modes(desc.modes)
+        info = np.zeros(4, dtype=np.int32)
+        check(L.mrcc_scene_info(self.handle, iptr(info)))
+        self.kind, self.D, self.n_robots, self.n_modes = (int(v) for v in info)
+
+    def set_modes(self, modes: dict) -> None:
+        na = np.ascontiguousarray(modes["n_active"], dtype=np.int32)
+        g = np.ascontiguousarray(modes["has_grasp"], dtype=np.uint8)
+        rel = np.ascontiguousarray(modes["grasp_rel"], dtype=np.float64)
+        half = np.ascontiguousarray(modes["grasp_half"], dtype=np.float64)
+        check(lib().mrcc_scene_set_modes(self.handle, len(na), iptr(na), g.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                         dptr(rel), dptr(half)))
+        self.n_modes = len(na)
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                lib().mrcc_scene_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+    # -------------------------------------------------------------------------------------------
+    # device-tensor entry points (inputs already resident)
+    def check_configs_device(self, q, out, modes=None, robot_mask: int = ALL_ROBOTS, stream=None):
+        """q: float64 device tensor [n, D] (rows contiguous) or any 2-D strided view."""
+        n = q.shape[0]
+        check(lib().mrcc_check_configs(self.handle, ctypes.c_void_p(q.data_ptr()), n, q.stride(0), q.stride(1),
+                                       ctypes.c_void_p(modes.data_ptr()) if modes is not None else None,
+                                       robot_mask, ctypes.c_void_p(out.data_ptr()),
+                                       stream if stream is not None else _stream_handle()))
+
+    def edge_batch(self, q1, q2, out, resolution, N=None, N_start=None, N_max=None,
+                   include_endpoints=False, modes=None, robot_mask=ALL_ROBOTS, idx1=None, idx2=None):
+        """Builds the C batch descriptor over device tensors (kept alive by the caller)."""
+        assert q1.stride() == q2.stride(), "q1 and q2 must share the layout"
+        p = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
+        b = EdgeBatch()
+        b.q1, b.q2 = p(q1), p(q2)
+        b.row_stride, b.dim_stride = q1.stride(0), q1.stride(1)
+        b.idx1, b.idx2 = p(idx1), p(idx2)
+        b.n_edges = out.shape[0]
+        b.resolution = float(resolution)
+        b.N, b.N_start, b.N_max = p(N), p(N_start), p(N_max)
+        b.include_endpoints = int(bool(include_endpoints))
+        b.mode_ids = p(modes)
+        b.robot_mask = robot_mask
+        b.out = p(out)
+        return b
+
+    def check_edges_device(self, batch: EdgeBatch, tune: Optional[Tune] = None, stream=None):
+        check(lib().mrcc_check_edges(self.handle, ctypes.byref(batch), ctypes.byref(tune) if tune else None,
+                                     stream if stream is not None else _stream_handle()))
+
+    def check_edges_per_call_device(self, batch: EdgeBatch, stream=None):
+        check(lib().mrcc_check_edges_per_call(self.handle, ctypes.byref(batch),
+                                              stream if stream is not None else _stream_handle()))
+
+
+class EdgePlan:
+    """A planned batched edge check bound to device tensors (graph-captured by default)."""
+
+    def __init__(self, scene: Scene, batch: EdgeBatch, tune: Optional[Tune] = None, keep=()):
+        self.scene = scene
+        self.batch = batch
+        self._keep = keep
+        self.handle = ctypes.c_void_p()
+        check(lib().mrcc_edge_plan_create(scene.handle, ctypes.byref(batch),
+                                          ctypes.byref(tune) if tune is not None else None,
+                                          _stream_handle(), ctypes.byref(self.handle)))
+
+    def run(self, stream=None):
+        check(lib().mrcc_edge_plan_run(self.handle, stream if stream is not None else _stream_handle()))
+
+    def stats(self):
+        s = (ctypes.c_int64 * 4)()
+        check(lib().mrcc_edge_plan_stats(self.handle, s))
+        return {"states": s[0], "rechecks": s[1], "dead_skips": s[2], "slots": s[3]}
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                lib().mrcc_edge_plan_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+
+def make_tune(slots_per_pass=0, slots_per_ticket=0, waves_per_cu=0, sorted=1, persistent=1,
+              use_graph=1, collect_stats=0) -> Tune:
+    t = Tune()
+    t.slots_per_pass, t.slots_per_ticket, t.waves_per_cu = slots_per_pass, slots_per_ticket, waves_per_cu
+    t.sorted, t.persistent, t.use_graph, t.collect_stats = sorted, persistent, use_graph, collect_stats
+    return t
+
+
+class MrccEnvironment(BaseProblem):
+    """``BaseProblem`` backend on the GPU engine.
+
+    ``tolerance`` arguments are accepted and unused, as in AbstractEnvironment
+    (abstract_env.py:316-317): the predicates are exact booleans.
+    """
+
+    def __init__(self, desc: SceneDesc, robots: Optional[List[str]] = None, start_pos=None,
+                 collision_resolution: float = 0.01, collision_tolerance: float = 0.01,
+                 device: int = 0):
+        self.desc = desc
+        self.scene = Scene(desc, device)
+        self.limits = desc.limits
+        n = desc.n_robots
+        self.robots = robots if robots is not None else [f"a{i + 1}" for i in range(n)]
+        sl = desc.robot_slices()
+        self.robot_idx = {r: list(range(s, e)) for r, (s, e) in zip(self.robots, sl)}
+        self.robot_dims = {r: e - s for r, (s, e) in zip(self.robots, sl)}
+        if start_pos is None:
+            start_pos = NpConfiguration(np.zeros(desc.dim), sl)
+        self.start_pos = start_pos
+        self.start_mode = Mode([0] * n, start_pos)
+        self.collision_resolution = collision_resolution
+        self.collision_tolerance = collision_tolerance
+        self._slices = sl
+        self._current_mode = self.start_mode
+        BaseProblem.__init__(self)
+
+    # ---- modes ----------------------------------------------------------------------------
+    @staticmethod
+    def mode_id(mode) -> int:
+        if mode is None:
+            return 0
+        if isinstance(mode, (int, np.integer)):
+            return int(mode)
+        return int(getattr(mode, "mrcc_mode_id", 0))
+
+    def set_to_mode(self, mode: Mode) -> None:
+        self._current_mode = mode
+
+    def get_scenegraph_info_for_mode(self, mode: Mode, is_start_mode: bool = False) -> Dict:
+        return {"mrcc_mode_id": self.mode_id(mode)}
+
+    def robot_mask(self, r) -> int:
+        if isinstance(r, str):
+            r = [r]
+        m = 0
+        for name in r:
+            m |= 1 << self.robots.index(name)
+        return m
+
+    # ---- per call --------------------------------------------------------------------------
+    @staticmethod
+    def _flat(q) -> np.ndarray:
+        v = q.state() if isinstance(q, Configuration) else q
+        return np.ascontiguousarray(v, dtype=np.float64)
+
+    def is_collision_free(self, q: Optional[Configuration], mode: Optional[Mode]) -> bool:
+        """abstract_env.py:255-276 contract; ValueError when q is None (:256-257)."""
+        if q is None:
+            raise ValueError
+        v = self._flat(q)
+        res = ctypes.c_int32()
+        check(lib().mrcc_check_config_host(self.scene.handle, dptr(v), self.mode_id(mode), ALL_ROBOTS,
+                                           ctypes.byref(res)))
+        return bool(res.value)
+
+    def is_collision_free_for_robot(self, r, q, m=None, collision_tolerance=None, set_mode=True) -> bool:
+        """Collisions involving robots r and no robot outside r (rai_base_env.py:515-615)."""
+        v = self._flat(q)
+        res = ctypes.c_int32()
+        check(lib().mrcc_check_config_host(self.scene.handle, dptr(v), self.mode_id(m), self.robot_mask(r),
+                                           ctypes.byref(res)))
+        return bool(res.value)
+
+    def is_edge_collision_free(self, q1, q2, mode, resolution=None, tolerance=None,
+                               include_endpoints=False, N_start=0, N_max=None, N=None) -> bool:
+        """abstract_env.py:301-354: N states at `resolution`, BFS order, window [N_start, N_max)."""
+        if resolution is None:
+            resolution = self.collision_resolution
+        a, b = self._flat(q1), self._flat(q2)
+        if N is None:
+            N = max(2, int(float(np.max(np.abs(a - b))) / resolution) + 1)
+        if N_start > N:
+            assert False
+        if N_start < 0:
+            raise ValueError("N_start < 0 is not supported")
+        res = ctypes.c_int32()
+        check(lib().mrcc_check_edge_host(self.scene.handle, dptr(a), dptr(b), float(resolution), int(N),
+                                         int(N_start), -1 if N_max is None else int(N_max),
+                                         int(bool(include_endpoints)), self.mode_id(mode), ALL_ROBOTS,
+                                         ctypes.byref(res)))
+        return bool(res.value)
+
+    # ---- batched ---------------------------------------------------------------------------
+    def batch_is_collision_free(self, qs, modes=None, robots=None) -> np.ndarray:
+        torch = _torch()
+        Q = np.ascontiguousarray(np.stack([self._flat(q) for q in qs]) if not isinstance(qs, np.ndarray) else qs,
+                                 dtype=np.float64)
+        n = Q.shape[0]
+        if n == 0:
+            return np.zeros(0, dtype=bool)
+        qd = torch.from_numpy(Q).cuda()
+        out = torch.empty(n, dtype=torch.uint8, device="cuda")
+        md = None
+        if modes is not None:
+            md = torch.from_numpy(np.asarray([self.mode_id(m) for m in modes], dtype=np.int32)).cuda()
+        mask = ALL_ROBOTS if robots is None else self.robot_mask(robots)
+        self.scene.check_configs_device(qd, out, md, mask)
+        return out.cpu().numpy().astype(bool)
+
+    def batch_is_edge_collision_free(self, q1s, q2s, modes=None, resolution=None, include_endpoints=False,
+                                     N_start=None, N_max=None, N=None, robots=None,
+                                     tune: Optional[Tune] = None) -> np.ndarray:
+        torch = _torch()
+        if resolution is None:
+            resolution = self.collision_resolution
+        A = np.ascontiguousarray(q1s if isinstance(q1s, np.ndarray) else np.stack([self._flat(q) for q in q1s]),
+                                 dtype=np.float64)
+        B = np.ascontiguousarray(q2s if isinstance(q2s, np.ndarray) else np.stack([self._flat(q) for q in q2s]),
+                                 dtype=np.float64)
+        n = A.shape[0]
+        if n == 0:
+            return np.zeros(0, dtype=bool)
+        iv = lambda v: None if v is None else torch.from_numpy(np.ascontiguousarray(v, dtype=np.int32)).cuda()
+        if N_start is not None:
+            Nv = np.asarray(N) if N is not None else np.maximum(
+                2, (np.max(np.abs(A - B), axis=1) / resolution).astype(np.int64) + 1)
+            if np.any(np.asarray(N_start) > Nv):
+                assert False, "N_start > N"
+            if np.any(np.asarray(N_start) < 0):
+                raise ValueError("N_start < 0 is not supported")
+        a, b = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+        out = torch.empty(n, dtype=torch.uint8, device="cuda")
+        Nt, St, Mt = iv(N), iv(N_start), iv(N_max)  # N_max entries < 0 mean None
+        md = iv(None if modes is None else [self.mode_id(m) for m in modes])
+        mask = ALL_ROBOTS if robots is None else self.robot_mask(robots)
+        batch = self.scene.edge_batch(a, b, out, resolution, Nt, St, Mt, include_endpoints, md, mask)
+        self.scene.check_edges_device(batch, tune)
+        return out.cpu().numpy().astype(bool)
+
+    def is_path_collision_free(self, path: List[State], binary_order: bool = True,
+                               resolution: Optional[float] = None, tolerance: Optional[float] = None,
+                               check_edges_in_order: bool = False, check_start_and_end: bool = True) -> bool:
+        """Same boolean as planning_env.py:1765-1881, from one batched launch of the vertices and
+        one of the edges: both schedules check every interior state of every edge; the vertex set
+        differs (the in-order schedule always checks the last vertex)."""
+        if resolution is None:
+            resolution = self.collision_resolution
+        K = len(path)
+        if K < 2:
+            raise IndexError("a path needs at least two states")
+        verts = [i for i in range(K - 1) if not (i == 0 and not check_start_and_end)]
+        if check_edges_in_order or check_start_and_end:
+            verts.append(K - 1)
+        V = np.stack([self._flat(s.q) for s in path])
+        vm = [path[i].mode for i in verts]
+        ok_v = self.batch_is_collision_free(V[verts], modes=vm)
+        if not ok_v.all():
+            return False
+        em = [path[i].mode for i in range(K - 1)]
+        ok_e = self.batch_is_edge_collision_free(V[:-1], V[1:], modes=em, resolution=resolution)
+        return bool(ok_e.all())
+
+    # ---- misc ------------------------------------------------------------------------------
+    def config_cost(self, start: Configuration, end: Configuration) -> float:
+        return config_dist(start, end, "euclidean")

@@ -1,0 +1,900 @@
+// api.hip — C ABI of libmrcc.so (declared in include/mrcc.h).
+//
+// Host side of the engine: scene construction (exact binary64 thresholds, binary32 copies and
+// margins), edge plans (workspace + hipGraph), the per-call entry points and the error contract
+// (status codes, thread-local message, injectable failures).
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mrcc.h"
+#include "engine.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+thread_local int g_inject_code = 0;
+thread_local int g_inject_count = 0;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int injected() {
+    if (g_inject_count > 0) {
+        --g_inject_count;
+        return fail(g_inject_code ? g_inject_code : MRCC_ERR_INJECTED, "injected failure");
+    }
+    return MRCC_OK;
+}
+
+#define HIP_OK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(MRCC_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));      \
+    } while (0)
+
+#define CHECK_INJECT()                 \
+    do {                               \
+        int r_ = injected();           \
+        if (r_ != MRCC_OK) return r_;  \
+    } while (0)
+
+int device_cus() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        return 256;
+    return cus;
+}
+
+// smallest binary64 x >= 0 with sqrt(x) >= R (so sqrt(d2) < R <=> d2 < T)
+double sqrt_threshold(double R) {
+    if (!(R > 0.0)) return 0.0;  // sqrt(d2) < R never holds for d2 >= 0
+    double x = R * R;
+    while (x > 0.0 && sqrt(x) >= R) x = nextafter(x, 0.0);
+    while (sqrt(x) < R) x = nextafter(x, INFINITY);
+    return x;
+}
+
+double (*volatile pow_fn)(double, double) = pow;  // a libm call, as Python's r**2
+
+// packs host arrays into one device allocation
+struct Packer {
+    std::vector<unsigned char> buf;
+    template <class T>
+    size_t add(const T* p, size_t n) {
+        size_t off = (buf.size() + 255) & ~size_t(255);
+        buf.resize(off + sizeof(T) * n + 1);
+        if (n) memcpy(buf.data() + off, p, sizeof(T) * n);
+        return off;
+    }
+    template <class T>
+    size_t add(const std::vector<T>& v) { return add(v.data(), v.size()); }
+};
+
+template <class T>
+const T* at(void* base, size_t off) { return (const T*)((unsigned char*)base + off); }
+
+int upload(SceneRec* sc, Packer& P) {
+    HIP_OK(hipMalloc(&sc->dev_mem, P.buf.size()));
+    HIP_OK(hipMemcpy(sc->dev_mem, P.buf.data(), P.buf.size(), hipMemcpyHostToDevice));
+    sc->dev_bytes = P.buf.size();
+    return MRCC_OK;
+}
+
+// host description of an arm scene, kept so modes can be replaced
+struct ArmHost {
+    int na, dof;
+    std::vector<double> base, dh;  // dh: [L*3] d a alpha
+    std::vector<int> counts;
+    std::vector<double> sph;       // [S*4]
+    std::vector<double> boxes;     // [nb*6] centre half
+    int nm;
+    std::vector<int> m_active;
+    std::vector<unsigned char> m_grasp;
+    std::vector<double> m_rel, m_half;
+};
+
+int build_arm_device(SceneRec* sc, const ArmHost& H) {
+    const int na = H.na, dof = H.dof, L = na * dof;
+    std::vector<int> link_first(L + 1, 0);
+    for (int l = 0; l < L; ++l) link_first[l + 1] = link_first[l] + H.counts[l];
+    const int S = link_first[L];
+    if (S > MRCC_MAX_SPHERES) return fail(MRCC_ERR_INVALID, "at most %d spheres", MRCC_MAX_SPHERES);
+    std::vector<int> sph_link(S), sph_lo(S);
+    for (int l = 0; l < L; ++l)
+        for (int t = link_first[l]; t < link_first[l + 1]; ++t) {
+            sph_link[t] = l;
+            const int a = l / dof, k = l % dof;
+            sph_lo[t] = (k + 2 < dof) ? link_first[a * dof + k + 2] : link_first[(a + 1) * dof];
+        }
+    std::vector<double> dh4(4 * (size_t)L);
+    double B = 0.0;
+    for (int a = 0; a < na; ++a) {
+        double reach = sqrt(H.base[12 * a + 9] * H.base[12 * a + 9] + H.base[12 * a + 10] * H.base[12 * a + 10] +
+                            H.base[12 * a + 11] * H.base[12 * a + 11]);
+        for (int k = 0; k < dof; ++k) {
+            const int l = a * dof + k;
+            mrcc::SinCos t = mrcc::sincos_f64(H.dh[3 * l + 2]);
+            dh4[4 * l] = H.dh[3 * l];
+            dh4[4 * l + 1] = H.dh[3 * l + 1];
+            dh4[4 * l + 2] = t.c;
+            dh4[4 * l + 3] = t.s;
+            reach += fabs(H.dh[3 * l]) + fabs(H.dh[3 * l + 1]);
+        }
+        B = std::max(B, reach);
+    }
+    double rmax = 0.0, lmax = 0.0;
+    std::vector<float> sph_r32(S);
+    for (int t = 0; t < S; ++t) {
+        sph_r32[t] = (float)H.sph[4 * t + 3];
+        rmax = std::max(rmax, H.sph[4 * t + 3]);
+        lmax = std::max(lmax, fabs(H.sph[4 * t]) + fabs(H.sph[4 * t + 1]) + fabs(H.sph[4 * t + 2]));
+    }
+    B += lmax + rmax;
+    const int nb = (int)(H.boxes.size() / 6);
+    std::vector<double> box64(6 * (size_t)nb);
+    std::vector<float> box32(8 * (size_t)nb), boxch32(6 * (size_t)nb);
+    double hmax = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        for (int k = 0; k < 3; ++k) {
+            const double c = H.boxes[6 * b + k], h = H.boxes[6 * b + 3 + k];
+            box64[6 * b + k] = c - h;
+            box64[6 * b + 3 + k] = c + h;
+            box32[8 * b + k] = (float)(c - h);
+            box32[8 * b + 4 + k] = (float)(c + h);
+            B = std::max(B, fabs(c) + fabs(h));
+            hmax = std::max(hmax, fabs(h));
+        }
+        for (int k = 0; k < 6; ++k) boxch32[6 * b + k] = (float)H.boxes[6 * b + k];
+    }
+    // held boxes: reach of the end-effector + grasp offset + box extent
+    double grasp_reach = 0.0;
+    for (size_t i = 0; i < H.m_half.size(); ++i) hmax = std::max(hmax, fabs(H.m_half[i]));
+    for (size_t i = 0; i < H.m_rel.size(); i += 12)
+        grasp_reach = std::max(grasp_reach, fabs(H.m_rel[i + 9]) + fabs(H.m_rel[i + 10]) + fabs(H.m_rel[i + 11]));
+    B += grasp_reach + 2.0 * hmax;
+    Packer P;
+    size_t o_base = P.add(H.base), o_dh = P.add(dh4), o_lf = P.add(link_first), o_sl = P.add(sph_link),
+           o_slo = P.add(sph_lo), o_s64 = P.add(H.sph), o_sr = P.add(sph_r32), o_b32 = P.add(box32),
+           o_b64 = P.add(box64), o_bch64 = P.add(H.boxes), o_bch32 = P.add(boxch32),
+           o_ma = P.add(H.m_active), o_mg = P.add(H.m_grasp), o_mr = P.add(H.m_rel),
+           o_mh = P.add(H.m_half);
+    void* old = sc->dev_mem;
+    int rc = upload(sc, P);
+    if (rc != MRCC_OK) return rc;
+    if (old) (void)hipFree(old);
+    void* d = sc->dev_mem;
+    ArmView& V = sc->arm;
+    V.na = na; V.dof = dof; V.D = na * dof; V.L = L; V.S = S; V.nb = nb; V.nm = H.nm;
+    // binary32 margins: squared distances near contact are off by <= ~2^-21 sqrt(3) R B; box SAT
+    // axis values by <= ~2^-21 (B + 3 H)
+    V.Msph = (float)(ldexp(1.0, -18) * (1.0 + B) * (1.0 + 2.0 * rmax));
+    V.Mobb = (float)(ldexp(1.0, -17) * (1.0 + B) * (1.0 + 3.0 * hmax));
+    V.base = at<double>(d, o_base);
+    V.dh = at<double>(d, o_dh);
+    V.link_first = at<int>(d, o_lf);
+    V.sph_link = at<int>(d, o_sl);
+    V.sph_lo = at<int>(d, o_slo);
+    V.sph64 = at<double>(d, o_s64);
+    V.sph_r32 = at<float>(d, o_sr);
+    V.box32 = at<float>(d, o_b32);
+    V.box64 = at<double>(d, o_b64);
+    V.boxch64 = at<double>(d, o_bch64);
+    V.boxch32 = at<float>(d, o_bch32);
+    V.m_active = at<int>(d, o_ma);
+    V.m_grasp = at<unsigned char>(d, o_mg);
+    V.m_rel = at<double>(d, o_mr);
+    V.m_half = at<double>(d, o_mh);
+    sc->D = V.D;
+    sc->n_robots = na;
+    sc->arm_lds = arm_lds_bytes(V.D, L, na, S);
+    sc->arm_na = na;
+    sc->arm_dof = dof;
+    sc->arm_S = S;
+    sc->arm_nm = H.nm;
+    return MRCC_OK;
+}
+
+struct SceneBox {
+    SceneRec rec;
+    ArmHost arm_host;
+};
+
+SceneRec* rec_of(const mrcc_scene* s) { return &((SceneBox*)s)->rec; }
+
+EdgeBatchView view_of(const mrcc_edge_batch* b) {
+    EdgeBatchView v;
+    v.q1 = b->q1;
+    v.q2 = b->q2;
+    v.rs = b->row_stride;
+    v.ds = b->dim_stride;
+    v.idx1 = b->idx1;
+    v.idx2 = b->idx2;
+    v.n = b->n_edges;
+    v.res = b->resolution;
+    v.N = b->N;
+    v.N_start = b->N_start;
+    v.N_max = b->N_max;
+    v.incl = b->include_endpoints;
+    v.modes = b->mode_ids;
+    v.mask = b->robot_mask;
+    v.out = b->out;
+    return v;
+}
+
+int check_batch(const SceneRec* sc, const mrcc_edge_batch* b) {
+    if (!b || !b->q1 || !b->q2 || !b->out) return fail(MRCC_ERR_INVALID, "null edge batch buffer");
+    if (b->n_edges < 0) return fail(MRCC_ERR_INVALID, "n_edges < 0");
+    if (!b->N && !(b->resolution > 0.0)) return fail(MRCC_ERR_INVALID, "resolution must be > 0");
+    if (sc->D > MRCC_MAX_D) return fail(MRCC_ERR_INVALID, "D > %d", MRCC_MAX_D);
+    return MRCC_OK;
+}
+
+}  // namespace
+
+struct mrcc_edge_plan {
+    SceneRec* sc;
+    EdgeBatchView B;
+    mrcc_tune tune;
+    int lgG;
+    int grid;
+    void* work_mem;
+    PlanWork W;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    hipStream_t cap_stream;
+};
+
+static void plan_launch_sequence(mrcc_edge_plan* p, hipStream_t st) {
+    if (p->tune.persistent) {
+        launch_plan_build(*p->sc, p->B, p->W, p->lgG, st);
+        launch_plan_run(*p->sc, p->B, p->W, p->lgG, p->tune.slots_per_ticket, p->grid, st);
+        // edges with more passes than the queue holds: one workgroup per edge
+        launch_direct(*p->sc, p->B, p->W.long_list, 0, p->W.meta + 3, 1, 0, 4 * device_cus(),
+                      p->W.stats, st);
+    } else {
+        launch_direct(*p->sc, p->B, nullptr, p->B.n, nullptr, 1, 1, 1 << 20, p->W.stats, st);
+    }
+}
+
+extern "C" {
+
+int mrcc_abi_version(void) { return MRCC_ABI_VERSION; }
+
+const char* mrcc_last_error(void) { return g_err.c_str(); }
+
+int mrcc_init(int device) {
+    CHECK_INJECT();
+    int n = 0;
+    HIP_OK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(MRCC_ERR_INVALID, "device %d of %d", device, n);
+    HIP_OK(hipSetDevice(device));
+    return MRCC_OK;
+}
+
+int mrcc_debug_inject_failure(int code, int count) {
+    g_inject_code = code;
+    g_inject_count = count;
+    return MRCC_OK;
+}
+
+int mrcc_scene_create_disks(int n_robots, int robot_dim, const double* robot_radius,
+                            int n_spheres, const double* sphere_center, const double* sphere_radius,
+                            int n_rects, const double* rect_center, const double* rect_size,
+                            mrcc_scene** out) {
+    CHECK_INJECT();
+    if (!out) return fail(MRCC_ERR_INVALID, "out is null");
+    if (n_robots < 1 || n_robots > MRCC_DISK_MAX_ROBOTS || robot_dim < 1 || robot_dim > MRCC_DISK_MAX_RD)
+        return fail(MRCC_ERR_INVALID, "disks: need 1..%d robots of dimension 1..%d", MRCC_DISK_MAX_ROBOTS,
+                    MRCC_DISK_MAX_RD);
+    if (n_robots * robot_dim > MRCC_MAX_D) return fail(MRCC_ERR_INVALID, "D > %d", MRCC_MAX_D);
+    if (n_spheres < 0 || n_rects < 0) return fail(MRCC_ERR_INVALID, "negative obstacle count");
+    const int nr = n_robots, rd = robot_dim, ns = n_spheres, nq = n_rects;
+    std::vector<float> r32(nr), rrT32(nr * nr), sc32(ns * rd + 1), srT32(nr * ns + 1),
+        lo32(nq * rd + 1), hi32(nq * rd + 1), rp2_32(nr);
+    std::vector<double> rrT64(nr * nr), sc64(ns * rd + 1), srT64(nr * ns + 1), lo64(nq * rd + 1),
+        hi64(nq * rd + 1), rp2_64(nr);
+    double B = 0.0;
+    for (int i = 0; i < nr; ++i) {
+        r32[i] = (float)robot_radius[i];
+        rp2_64[i] = pow_fn(robot_radius[i], 2.0);
+        rp2_32[i] = (float)rp2_64[i];
+        for (int j = 0; j < nr; ++j) {
+            rrT64[i * nr + j] = sqrt_threshold(robot_radius[i] + robot_radius[j]);
+            rrT32[i * nr + j] = (float)rrT64[i * nr + j];
+        }
+        B = std::max(B, 2.0 * fabs(robot_radius[i]));
+    }
+    for (int o = 0; o < ns; ++o) {
+        for (int k = 0; k < rd; ++k) {
+            sc64[o * rd + k] = sphere_center[o * rd + k];
+            sc32[o * rd + k] = (float)sphere_center[o * rd + k];
+            B = std::max(B, fabs(sphere_center[o * rd + k]) + fabs(sphere_radius[o]));
+        }
+        for (int i = 0; i < nr; ++i) {
+            srT64[i * ns + o] = sqrt_threshold(sphere_radius[o] + robot_radius[i]);
+            srT32[i * ns + o] = (float)srT64[i * ns + o];
+        }
+    }
+    for (int o = 0; o < nq; ++o)
+        for (int k = 0; k < rd; ++k) {
+            const double c = rect_center[o * rd + k], h = rect_size[o * rd + k] / 2;
+            lo64[o * rd + k] = c - h;  // abstract_env.py:66
+            hi64[o * rd + k] = c + h;  // abstract_env.py:67
+            lo32[o * rd + k] = (float)lo64[o * rd + k];
+            hi32[o * rd + k] = (float)hi64[o * rd + k];
+            B = std::max(B, std::max(fabs(lo64[o * rd + k]), fabs(hi64[o * rd + k])));
+        }
+    SceneBox* box = new SceneBox();
+    SceneRec* sc = &box->rec;
+    memset(sc, 0, sizeof(SceneRec));
+    sc->kind = KIND_DISKS;
+    sc->D = nr * rd;
+    sc->n_robots = nr;
+    hipGetDevice(&sc->device);
+    Packer P;
+    size_t o_r = P.add(r32), o_rrT32 = P.add(rrT32), o_rrT64 = P.add(rrT64), o_sc32 = P.add(sc32),
+           o_sc64 = P.add(sc64), o_srT32 = P.add(srT32), o_srT64 = P.add(srT64), o_lo32 = P.add(lo32),
+           o_hi32 = P.add(hi32), o_lo64 = P.add(lo64), o_hi64 = P.add(hi64), o_rp32 = P.add(rp2_32),
+           o_rp64 = P.add(rp2_64);
+    int rc = upload(sc, P);
+    if (rc != MRCC_OK) {
+        delete box;
+        return rc;
+    }
+    void* d = sc->dev_mem;
+    DiskView& V = sc->disk;
+    V.nr = nr; V.rd = rd; V.D = nr * rd; V.ns = ns; V.nrect = nq;
+    V.Bscene = (float)B;
+    V.r32 = at<float>(d, o_r);
+    V.rrT32 = at<float>(d, o_rrT32);
+    V.rrT64 = at<double>(d, o_rrT64);
+    V.sc32 = at<float>(d, o_sc32);
+    V.sc64 = at<double>(d, o_sc64);
+    V.srT32 = at<float>(d, o_srT32);
+    V.srT64 = at<double>(d, o_srT64);
+    V.lo32 = at<float>(d, o_lo32);
+    V.hi32 = at<float>(d, o_hi32);
+    V.lo64 = at<double>(d, o_lo64);
+    V.hi64 = at<double>(d, o_hi64);
+    V.rp2_32 = at<float>(d, o_rp32);
+    V.rp2_64 = at<double>(d, o_rp64);
+    *out = (mrcc_scene*)box;
+    return MRCC_OK;
+}
+
+int mrcc_scene_create_planar_boxes(int n_robots, const double* robot_half_extents, int n_obstacles,
+                                   const double* obstacle_pose, const double* obstacle_half_extents,
+                                   mrcc_scene** out) {
+    CHECK_INJECT();
+    if (!out) return fail(MRCC_ERR_INVALID, "out is null");
+    if (n_robots < 1 || n_robots > MRCC_PLANAR_MAX_ROBOTS)
+        return fail(MRCC_ERR_INVALID, "planar: 1..%d robots", MRCC_PLANAR_MAX_ROBOTS);
+    if (n_obstacles < 0) return fail(MRCC_ERR_INVALID, "negative obstacle count");
+    const int nr = n_robots, no = n_obstacles;
+    std::vector<double> rh64(2 * nr), ob64(8 * no + 1);
+    std::vector<float> rh32(2 * nr), ob32(8 * no + 1);
+    double B = 0.0, H = 0.0;
+    for (int i = 0; i < 2 * nr; ++i) {
+        rh64[i] = robot_half_extents[i];
+        rh32[i] = (float)rh64[i];
+        H = std::max(H, fabs(rh64[i]));
+    }
+    for (int o = 0; o < no; ++o) {
+        mrcc::SinCos t = mrcc::sincos_f64(obstacle_pose[3 * o + 2]);
+        double* b = ob64.data() + 8 * o;
+        b[0] = obstacle_pose[3 * o];
+        b[1] = obstacle_pose[3 * o + 1];
+        b[2] = t.c; b[3] = t.s;
+        b[4] = -t.s; b[5] = t.c;
+        b[6] = obstacle_half_extents[2 * o];
+        b[7] = obstacle_half_extents[2 * o + 1];
+        for (int k = 0; k < 8; ++k) ob32[8 * o + k] = (float)b[k];
+        B = std::max(B, std::max(fabs(b[0]), fabs(b[1])));
+        H = std::max(H, std::max(fabs(b[6]), fabs(b[7])));
+    }
+    SceneBox* box = new SceneBox();
+    SceneRec* sc = &box->rec;
+    memset(sc, 0, sizeof(SceneRec));
+    sc->kind = KIND_PLANAR;
+    sc->D = 3 * nr;
+    sc->n_robots = nr;
+    hipGetDevice(&sc->device);
+    Packer P;
+    size_t o_rh32 = P.add(rh32), o_rh64 = P.add(rh64), o_ob32 = P.add(ob32), o_ob64 = P.add(ob64);
+    int rc = upload(sc, P);
+    if (rc != MRCC_OK) {
+        delete box;
+        return rc;
+    }
+    void* d = sc->dev_mem;
+    PlanarView& V = sc->planar;
+    V.nr = nr; V.no = no; V.D = 3 * nr;
+    V.Bscene = (float)(B + 4.0 * H);
+    V.rh32 = at<float>(d, o_rh32);
+    V.rh64 = at<double>(d, o_rh64);
+    V.ob32 = at<float>(d, o_ob32);
+    V.ob64 = at<double>(d, o_ob64);
+    *out = (mrcc_scene*)box;
+    return MRCC_OK;
+}
+
+int mrcc_scene_create_arms(int n_arms, int dof, const double* base, const double* dh,
+                           const int32_t* link_sphere_count, const double* sphere_local,
+                           int n_boxes, const double* boxes, mrcc_scene** out) {
+    CHECK_INJECT();
+    if (!out) return fail(MRCC_ERR_INVALID, "out is null");
+    if (n_arms < 1 || n_arms > MRCC_MAX_ARMS || dof < 1 || dof > MRCC_MAX_DOF)
+        return fail(MRCC_ERR_INVALID, "arms: 1..%d arms of 1..%d joints", MRCC_MAX_ARMS, MRCC_MAX_DOF);
+    if (n_boxes < 0) return fail(MRCC_ERR_INVALID, "negative box count");
+    SceneBox* box = new SceneBox();
+    SceneRec* sc = &box->rec;
+    memset(sc, 0, sizeof(SceneRec));
+    sc->kind = KIND_ARMS;
+    hipGetDevice(&sc->device);
+    ArmHost& H = box->arm_host;
+    const int L = n_arms * dof;
+    H.na = n_arms;
+    H.dof = dof;
+    H.base.assign(base, base + 12 * n_arms);
+    H.dh.assign(dh, dh + 3 * L);
+    H.counts.assign(link_sphere_count, link_sphere_count + L);
+    int S = 0;
+    for (int l = 0; l < L; ++l) {
+        if (H.counts[l] < 0) {
+            delete box;
+            return fail(MRCC_ERR_INVALID, "negative sphere count");
+        }
+        S += H.counts[l];
+    }
+    H.sph.assign(sphere_local, sphere_local + 4 * S);
+    H.boxes.assign(boxes, boxes + 6 * n_boxes);
+    H.nm = 1;
+    H.m_active.assign(1, n_arms);
+    H.m_grasp.assign(n_arms, 0);
+    H.m_rel.assign(12 * n_arms, 0.0);
+    H.m_half.assign(3 * n_arms, 0.0);
+    int rc = build_arm_device(sc, H);
+    if (rc != MRCC_OK) {
+        if (sc->dev_mem) (void)hipFree(sc->dev_mem);
+        delete box;
+        return rc;
+    }
+    *out = (mrcc_scene*)box;
+    return MRCC_OK;
+}
+
+int mrcc_scene_set_modes(mrcc_scene* scene, int n_modes, const int32_t* n_active,
+                         const uint8_t* has_grasp, const double* grasp_rel,
+                         const double* grasp_half) {
+    CHECK_INJECT();
+    if (!scene) return fail(MRCC_ERR_INVALID, "scene is null");
+    SceneBox* box = (SceneBox*)scene;
+    if (box->rec.kind != KIND_ARMS) return fail(MRCC_ERR_UNSUPPORTED, "modes need an arm scene");
+    if (n_modes < 1) return fail(MRCC_ERR_INVALID, "n_modes < 1");
+    ArmHost& H = box->arm_host;
+    const size_t nma = (size_t)n_modes * H.na;
+    for (int m = 0; m < n_modes; ++m)
+        if (n_active[m] < 1 || n_active[m] > H.na)
+            return fail(MRCC_ERR_INVALID, "mode %d: n_active %d not in 1..%d", m, n_active[m], H.na);
+    H.nm = n_modes;
+    H.m_active.assign(n_active, n_active + n_modes);
+    H.m_grasp.assign(has_grasp, has_grasp + nma);
+    H.m_rel.assign(grasp_rel, grasp_rel + 12 * nma);
+    H.m_half.assign(grasp_half, grasp_half + 3 * nma);
+    HIP_OK(hipDeviceSynchronize());  // kernels may still read the old arrays
+    return build_arm_device(&box->rec, H);
+}
+
+int mrcc_scene_destroy(mrcc_scene* scene) {
+    if (!scene) return MRCC_OK;
+    SceneBox* box = (SceneBox*)scene;
+    SceneRec* sc = &box->rec;
+    (void)hipDeviceSynchronize();
+    if (sc->dev_mem) (void)hipFree(sc->dev_mem);
+    if (sc->stage_dev) (void)hipFree(sc->stage_dev);
+    if (sc->stage_host) (void)hipHostFree(sc->stage_host);
+    delete box;
+    return MRCC_OK;
+}
+
+int mrcc_scene_info(const mrcc_scene* scene, int32_t* out4) {
+    if (!scene || !out4) return fail(MRCC_ERR_INVALID, "null argument");
+    const SceneRec* sc = rec_of(scene);
+    out4[0] = sc->kind;
+    out4[1] = sc->D;
+    out4[2] = sc->n_robots;
+    out4[3] = sc->kind == KIND_ARMS ? sc->arm.nm : 1;
+    return MRCC_OK;
+}
+
+int mrcc_check_configs(const mrcc_scene* scene, const double* q, int64_t n, int64_t row_stride,
+                       int64_t dim_stride, const int32_t* mode_ids, uint32_t robot_mask,
+                       uint8_t* out, void* stream) {
+    CHECK_INJECT();
+    if (!scene || (!q && n) || (!out && n)) return fail(MRCC_ERR_INVALID, "null argument");
+    if (n < 0) return fail(MRCC_ERR_INVALID, "n < 0");
+    if (n == 0) return MRCC_OK;
+    const SceneRec* sc = rec_of(scene);
+    launch_configs(*sc, q, n, row_stride, dim_stride, mode_ids, robot_mask, out, 64 * device_cus(),
+                   (hipStream_t)stream);
+    HIP_OK(hipGetLastError());
+    return MRCC_OK;
+}
+
+int mrcc_edge_plan_create(const mrcc_scene* scene, const mrcc_edge_batch* batch,
+                          const mrcc_tune* tune, void* stream, mrcc_edge_plan** out) {
+    CHECK_INJECT();
+    if (!scene || !out) return fail(MRCC_ERR_INVALID, "null argument");
+    SceneRec* sc = rec_of(scene);
+    int rc = check_batch(sc, batch);
+    if (rc != MRCC_OK) return rc;
+    mrcc_tune t;
+    memset(&t, 0, sizeof(t));
+    if (tune) t = *tune;
+    const bool arms = sc->kind == KIND_ARMS;
+    if (t.slots_per_pass <= 0) t.slots_per_pass = arms ? 4 : 8;
+    if (t.slots_per_ticket <= 0) t.slots_per_ticket = 1;
+    if (t.waves_per_cu <= 0) t.waves_per_cu = arms ? 16 : 16;
+    if (!tune) {
+        t.sorted = 1;
+        t.persistent = 1;
+        t.use_graph = 1;
+    }
+    if ((t.slots_per_pass & (t.slots_per_pass - 1)) || t.slots_per_pass > 64)
+        return fail(MRCC_ERR_INVALID, "slots_per_pass must be a power of two <= 64");
+    mrcc_edge_plan* p = new mrcc_edge_plan();
+    memset(p, 0, sizeof(*p));
+    p->sc = sc;
+    p->B = view_of(batch);
+    p->tune = t;
+    p->lgG = __builtin_ctz((unsigned)t.slots_per_pass);
+    const int cus = device_cus();
+    p->grid = arms ? t.waves_per_cu * cus : std::max(1, t.waves_per_cu * cus / 4);
+    // workspace
+    const long long n = std::max<long long>(batch->n_edges, 1);
+    const int D = sc->D;
+    Packer P;  // sizes only
+    auto sz = [&](size_t bytes) { size_t off = (P.buf.size() + 255) & ~size_t(255); P.buf.resize(off + bytes); return off; };
+    size_t o_eN = sz(4 * n), o_eM = sz(4 * n), o_eS = sz(4 * n), o_w0 = sz(4 * n), o_full = sz(n),
+           o_order = sz(4 * n), o_hist = sz(4 * (MRCC_MAX_PASSES + 1)), o_cur = sz(4 * (MRCC_MAX_PASSES + 1)),
+           o_ps = sz(8 * (MRCC_MAX_PASSES + 1)), o_meta = sz(8 * 8), o_long = sz(4 * n), o_tick = sz(4),
+           o_stats = sz(8 * 4), o_dir = arms ? 0 : sz(8 * (size_t)n * D);
+    if (hipMalloc(&p->work_mem, P.buf.size()) != hipSuccess) {
+        delete p;
+        return fail(MRCC_ERR_NOMEM, "workspace of %zu bytes", P.buf.size());
+    }
+    unsigned char* w = (unsigned char*)p->work_mem;
+    p->W.eN = (int*)(w + o_eN);
+    p->W.eM = (int*)(w + o_eM);
+    p->W.eS = (int*)(w + o_eS);
+    p->W.ew0 = (int*)(w + o_w0);
+    p->W.efull = w + o_full;
+    p->W.order = (int*)(w + o_order);
+    p->W.hist = (int*)(w + o_hist);
+    p->W.cursor = (int*)(w + o_cur);
+    p->W.pass_start = (long long*)(w + o_ps);
+    p->W.meta = (long long*)(w + o_meta);
+    p->W.long_list = (int*)(w + o_long);
+    p->W.ticket = (unsigned*)(w + o_tick);
+    p->W.stats = t.collect_stats ? (unsigned long long*)(w + o_stats) : nullptr;
+    p->W.dir = arms ? nullptr : (double*)(w + o_dir);
+    if (t.collect_stats) HIP_OK(hipMemset(w + o_stats, 0, 32));
+    HIP_OK(hipMemset(w + o_meta, 0, 64));
+    if (t.use_graph && batch->n_edges > 0) {
+        HIP_OK(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+        HIP_OK(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+        plan_launch_sequence(p, p->cap_stream);
+        hipError_t e1 = hipStreamEndCapture(p->cap_stream, &p->graph);
+        if (e1 != hipSuccess) {
+            mrcc_edge_plan_destroy(p);
+            return fail(MRCC_ERR_HIP, "graph capture: %s", hipGetErrorString(e1));
+        }
+        hipError_t e2 = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+        if (e2 != hipSuccess) {
+            mrcc_edge_plan_destroy(p);
+            return fail(MRCC_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e2));
+        }
+    }
+    HIP_OK(hipStreamSynchronize((hipStream_t)stream));
+    *out = p;
+    return MRCC_OK;
+}
+
+int mrcc_edge_plan_run(mrcc_edge_plan* p, void* stream) {
+    CHECK_INJECT();
+    if (!p) return fail(MRCC_ERR_INVALID, "plan is null");
+    if (p->B.n == 0) return MRCC_OK;
+    hipStream_t st = (hipStream_t)stream;
+    if (p->exec) {
+        HIP_OK(hipGraphLaunch(p->exec, st));
+    } else {
+        plan_launch_sequence(p, st);
+        HIP_OK(hipGetLastError());
+    }
+    return MRCC_OK;
+}
+
+int mrcc_edge_plan_stats(mrcc_edge_plan* p, int64_t* out4) {
+    if (!p || !out4) return fail(MRCC_ERR_INVALID, "null argument");
+    HIP_OK(hipDeviceSynchronize());
+    unsigned long long s[4] = {0, 0, 0, 0};
+    if (p->W.stats) HIP_OK(hipMemcpy(s, p->W.stats, sizeof(s), hipMemcpyDeviceToHost));
+    for (int i = 0; i < 4; ++i) out4[i] = (int64_t)s[i];
+    return MRCC_OK;
+}
+
+int mrcc_edge_plan_destroy(mrcc_edge_plan* p) {
+    if (!p) return MRCC_OK;
+    (void)hipDeviceSynchronize();
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+    if (p->work_mem) (void)hipFree(p->work_mem);
+    delete p;
+    return MRCC_OK;
+}
+
+int mrcc_check_edges(const mrcc_scene* scene, const mrcc_edge_batch* batch, const mrcc_tune* tune,
+                     void* stream) {
+    CHECK_INJECT();
+    if (!scene) return fail(MRCC_ERR_INVALID, "scene is null");
+    int rc = check_batch(rec_of(scene), batch);
+    if (rc != MRCC_OK) return rc;
+    if (batch->n_edges == 0) return MRCC_OK;
+    mrcc_tune t;
+    memset(&t, 0, sizeof(t));
+    if (tune) t = *tune;
+    else {
+        t.sorted = 1;
+        t.persistent = 1;
+    }
+    t.use_graph = 0;
+    mrcc_edge_plan* p = nullptr;
+    rc = mrcc_edge_plan_create(scene, batch, &t, stream, &p);
+    if (rc != MRCC_OK) return rc;
+    rc = mrcc_edge_plan_run(p, stream);
+    long long meta[8] = {0};
+    if (rc == MRCC_OK) {
+        hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+        if (e == hipSuccess) e = hipMemcpy(meta, p->W.meta, sizeof(meta), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(MRCC_ERR_HIP, "edge batch: %s", hipGetErrorString(e));
+    }
+    mrcc_edge_plan_destroy(p);
+    if (rc == MRCC_OK && meta[2] > 0)
+        return fail(MRCC_ERR_INVALID, "%lld edges have N_start > N (reference asserts)", meta[2]);
+    return rc;
+}
+
+static int ensure_stage(SceneRec* sc) {
+    if (sc->stage_dev) return MRCC_OK;
+    const size_t bytes = 8 * (2 * MRCC_MAX_D + 4);
+    HIP_OK(hipMalloc(&sc->stage_dev, bytes));
+    HIP_OK(hipHostMalloc(&sc->stage_host, bytes));
+    sc->stage_out_dev = (unsigned char*)(sc->stage_dev + 2 * MRCC_MAX_D + 2);
+    sc->stage_out_host = (unsigned char*)(sc->stage_host + 2 * MRCC_MAX_D + 2);
+    return MRCC_OK;
+}
+
+// Stages (optionally) q1 / q2 and the scalar arguments of one edge; returns the 1-edge view.
+// Layout of the staging block: q1[MAX_D] q2[MAX_D] int32{N, N_start, N_max, mode} out.
+static int stage_edge(SceneRec* sc, const double* q1h, const double* q2h, int32_t N, int32_t N_start,
+                      int32_t N_max, int32_t mode_id, hipStream_t st, EdgeBatchView* v) {
+    int rc = ensure_stage(sc);
+    if (rc != MRCC_OK) return rc;
+    HIP_OK(hipStreamSynchronize(st));  // the previous call may still read the block
+    double* h = sc->stage_host;
+    size_t bytes = 8 * (2 * MRCC_MAX_D) + 16;
+    if (q1h) memcpy(h, q1h, 8 * (size_t)sc->D);
+    if (q2h) memcpy(h + MRCC_MAX_D, q2h, 8 * (size_t)sc->D);
+    int32_t* ih = (int32_t*)(h + 2 * MRCC_MAX_D);
+    ih[0] = N;
+    ih[1] = N_start;
+    ih[2] = N_max;
+    ih[3] = mode_id;
+    sc->stage_out_host[0] = 1;
+    bytes += 8;
+    HIP_OK(hipMemcpyAsync(sc->stage_dev, h, bytes, hipMemcpyHostToDevice, st));
+    const int32_t* id = (const int32_t*)(sc->stage_dev + 2 * MRCC_MAX_D);
+    memset(v, 0, sizeof(*v));
+    v->q1 = sc->stage_dev;
+    v->q2 = sc->stage_dev + MRCC_MAX_D;
+    v->rs = sc->D;
+    v->ds = 1;
+    v->n = 1;
+    v->N = N > 0 ? id : nullptr;
+    v->N_start = id + 1;
+    v->N_max = id + 2;
+    v->modes = id + 3;
+    v->out = sc->stage_out_dev;
+    return MRCC_OK;
+}
+
+int mrcc_check_edge(const mrcc_scene* scene, const double* q1, const double* q2, double resolution,
+                    int32_t N, int32_t N_start, int32_t N_max, int32_t include_endpoints,
+                    int32_t mode_id, uint32_t robot_mask, uint8_t* out, void* stream) {
+    CHECK_INJECT();
+    if (!scene || !q1 || !q2 || !out) return fail(MRCC_ERR_INVALID, "null argument");
+    if (N <= 0 && !(resolution > 0.0)) return fail(MRCC_ERR_INVALID, "resolution must be > 0");
+    if (N_start < 0) return fail(MRCC_ERR_INVALID, "N_start < 0");
+    SceneRec* sc = (SceneRec*)rec_of(scene);
+    hipStream_t st = (hipStream_t)stream;
+    EdgeBatchView v;
+    memset(&v, 0, sizeof(v));
+    if (N > 0 || N_start != 0 || N_max >= 0 || mode_id != 0) {
+        // scalar arguments travel through the scene's staging block
+        int rc = stage_edge(sc, nullptr, nullptr, N, N_start, N_max, mode_id, st, &v);
+        if (rc != MRCC_OK) return rc;
+    }
+    v.q1 = q1;
+    v.q2 = q2;
+    v.rs = sc->D;
+    v.ds = 1;
+    v.n = 1;
+    v.res = resolution;
+    v.incl = include_endpoints;
+    v.mask = robot_mask;
+    v.out = out;
+    launch_direct(*sc, v, nullptr, 1, nullptr, 1, 1, 1, nullptr, st);
+    HIP_OK(hipGetLastError());
+    return MRCC_OK;
+}
+
+int mrcc_check_edge_host(const mrcc_scene* scene, const double* q1, const double* q2,
+                         double resolution, int32_t N, int32_t N_start, int32_t N_max,
+                         int32_t include_endpoints, int32_t mode_id, uint32_t robot_mask,
+                         int32_t* result) {
+    CHECK_INJECT();
+    if (!scene || !q1 || !q2 || !result) return fail(MRCC_ERR_INVALID, "null argument");
+    if (N <= 0 && !(resolution > 0.0)) return fail(MRCC_ERR_INVALID, "resolution must be > 0");
+    if (N_start < 0) return fail(MRCC_ERR_INVALID, "N_start < 0");
+    SceneRec* sc = (SceneRec*)rec_of(scene);
+    hipStream_t st = nullptr;
+    EdgeBatchView v;
+    int rc = stage_edge(sc, q1, q2, N, N_start, N_max, mode_id, st, &v);
+    if (rc != MRCC_OK) return rc;
+    v.res = resolution;
+    v.incl = include_endpoints;
+    v.mask = robot_mask;
+    launch_direct(*sc, v, nullptr, 1, nullptr, 1, 0, 1, nullptr, st);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(sc->stage_out_host, sc->stage_out_dev, 1, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    *result = sc->stage_out_host[0] ? 1 : 0;
+    return MRCC_OK;
+}
+
+int mrcc_check_config_host(const mrcc_scene* scene, const double* q, int32_t mode_id,
+                           uint32_t robot_mask, int32_t* result) {
+    CHECK_INJECT();
+    if (!scene || !q || !result) return fail(MRCC_ERR_INVALID, "null argument");
+    SceneRec* sc = (SceneRec*)rec_of(scene);
+    hipStream_t st = nullptr;
+    EdgeBatchView v;
+    int rc = stage_edge(sc, q, nullptr, 0, 0, -1, mode_id, st, &v);
+    if (rc != MRCC_OK) return rc;
+    launch_configs(*sc, v.q1, 1, sc->D, 1, v.modes, robot_mask, v.out, 1, st);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(sc->stage_out_host, sc->stage_out_dev, 1, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    *result = sc->stage_out_host[0] ? 1 : 0;
+    return MRCC_OK;
+}
+
+int mrcc_check_edges_per_call(const mrcc_scene* scene, const mrcc_edge_batch* batch, void* stream) {
+    CHECK_INJECT();
+    if (!scene) return fail(MRCC_ERR_INVALID, "scene is null");
+    const SceneRec* sc = rec_of(scene);
+    int rc = check_batch(sc, batch);
+    if (rc != MRCC_OK) return rc;
+    EdgeBatchView v = view_of(batch);
+    hipStream_t st = (hipStream_t)stream;
+    for (long long e = 0; e < batch->n_edges; ++e) {
+        // one launch per edge: the batch view offset to edge e
+        EdgeBatchView u = v;
+        const long long r1 = e, r2 = e;
+        if (!v.idx1) u.q1 = v.q1 + r1 * v.rs;
+        else u.idx1 = v.idx1 + e;
+        if (!v.idx2) u.q2 = v.q2 + r2 * v.rs;
+        else u.idx2 = v.idx2 + e;
+        if (v.N) u.N = v.N + e;
+        if (v.N_start) u.N_start = v.N_start + e;
+        if (v.N_max) u.N_max = v.N_max + e;
+        if (v.modes) u.modes = v.modes + e;
+        u.out = v.out + e;
+        u.n = 1;
+        launch_direct(*sc, u, nullptr, 1, nullptr, 1, 1, 1, nullptr, st);
+    }
+    HIP_OK(hipGetLastError());
+    return MRCC_OK;
+}
+
+int mrcc_bfs_indices(int32_t N, int32_t* out) {
+    if (N < 1 || !out) return fail(MRCC_ERR_INVALID, "N < 1");
+    std::deque<std::pair<int, int>> q;
+    q.emplace_back(0, N - 1);
+    int idx = 0;
+    while (!q.empty()) {
+        auto [s, e] = q.front();
+        q.pop_front();
+        const int mid = (s + e) / 2;
+        out[idx++] = mid;
+        if (s <= mid - 1) q.emplace_back(s, mid - 1);
+        if (mid + 1 <= e) q.emplace_back(mid + 1, e);
+    }
+    return MRCC_OK;
+}
+
+int64_t mrcc_edge_num_states(int32_t N, int32_t N_start, int32_t N_max, int32_t include_endpoints) {
+    if (N < 1 || N_start < 0 || N_start > N) return -1;
+    mrcc::EdgeWin w = mrcc::edge_window(N, N_start, N_max, include_endpoints);
+    if (w.full) return w.M;
+    int64_t c = 0;
+    for (int k = 0; k < w.M; ++k)
+        if (mrcc::edge_state_index(w, k, include_endpoints) >= 0) ++c;
+    return c;
+}
+
+int mrcc_arm_fk(const mrcc_scene* scene, const double* q, int64_t n, int64_t row_stride,
+                int64_t dim_stride, double* out, void* stream) {
+    CHECK_INJECT();
+    if (!scene || !q || !out) return fail(MRCC_ERR_INVALID, "null argument");
+    const SceneRec* sc = rec_of(scene);
+    if (sc->kind != KIND_ARMS) return fail(MRCC_ERR_UNSUPPORTED, "FK needs an arm scene");
+    if (n <= 0) return MRCC_OK;
+    launch_arm_fk(*sc, q, n, row_stride, dim_stride, out, (hipStream_t)stream);
+    HIP_OK(hipGetLastError());
+    return MRCC_OK;
+}
+
+int mrcc_sincos(const double* x, int64_t n, double* s, double* c, void* stream) {
+    CHECK_INJECT();
+    if (n <= 0) return MRCC_OK;
+    if (!x || !s || !c) return fail(MRCC_ERR_INVALID, "null argument");
+    launch_sincos(x, n, s, c, (hipStream_t)stream);
+    HIP_OK(hipGetLastError());
+    return MRCC_OK;
+}
+
+int mrcc_clearance_forward(const mrcc_scene* scene, const float* q, int64_t n, float margin,
+                           float* cost, void* stream) {
+    CHECK_INJECT();
+    if (!scene || (!q && n) || (!cost && n)) return fail(MRCC_ERR_INVALID, "null argument");
+    const SceneRec* sc = rec_of(scene);
+    if (sc->kind != KIND_ARMS) return fail(MRCC_ERR_UNSUPPORTED, "clearance needs an arm scene");
+    if (n <= 0) return MRCC_OK;
+    launch_clearance(*sc, q, n, margin, nullptr, cost, nullptr, (hipStream_t)stream);
+    HIP_OK(hipGetLastError());
+    return MRCC_OK;
+}
+
+int mrcc_clearance_backward(const mrcc_scene* scene, const float* q, int64_t n, float margin,
+                            const float* grad_cost, float* grad_q, void* stream) {
+    CHECK_INJECT();
+    if (!scene || (!q && n) || (!grad_cost && n) || (!grad_q && n))
+        return fail(MRCC_ERR_INVALID, "null argument");
+    const SceneRec* sc = rec_of(scene);
+    if (sc->kind != KIND_ARMS) return fail(MRCC_ERR_UNSUPPORTED, "clearance needs an arm scene");
+    if (n <= 0) return MRCC_OK;
+    launch_clearance(*sc, q, n, margin, grad_cost, nullptr, grad_q, (hipStream_t)stream);
+    HIP_OK(hipGetLastError());
+    return MRCC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,203 @@
+// clearance.hip — differentiable clearance cost of arm scenes (binary32).
+//
+//   cost(q) = sum over the checked pair set of max(0, m + r_i + r_j - |c_i - c_j|)^2
+//           + sum over spheres x boxes of max(0, m + r_i - dist(c_i, box))^2
+// with the pair set of the boolean check (links >= 2 apart in one arm, all links of different
+// arms).  One waypoint per wavefront.  The backward recomputes the forward kinematics, gathers the
+// force on every sphere centre (d cost / d c) in LDS and maps it to the joints with
+// d c / d q_k = z_{k-1} x (c - o_{k-1}) for spheres on links >= k of the same arm.
+#include "engine.hpp"
+
+namespace {
+
+struct ClrLds {
+    float* cs;     // [D]
+    float* sn;     // [D]
+    float* X;      // [L*12]
+    float4* sph;   // [S]
+    float* F;      // [S*3] forces (backward)
+    float* red;    // [64]
+};
+
+__device__ inline ClrLds clr_lds(void* base, const ArmView& V) {
+    ClrLds W;
+    float* f = (float*)base;
+    W.cs = f;
+    W.sn = f + V.D;
+    W.X = f + 2 * V.D;
+    W.sph = (float4*)(((uintptr_t)(W.X + 12 * V.L) + 15) & ~uintptr_t(15));
+    W.F = (float*)(W.sph + V.S);
+    W.red = W.F + 3 * V.S;
+    return W;
+}
+
+// forward kinematics in binary32 and sphere centres into LDS
+__device__ inline void clr_fk(const ArmView& V, const ClrLds& W, const float* q) {
+    const int lane = threadIdx.x;
+    if (lane < V.D) {
+        float s, c;
+        sincosf(q[lane], &s, &c);
+        W.cs[lane] = c;
+        W.sn[lane] = s;
+    }
+    __syncthreads();
+    if (lane < V.na) {
+        float X[12];
+        for (int k = 0; k < 12; ++k) X[k] = (float)V.base[12 * lane + k];
+        for (int k = 0; k < V.dof; ++k) {
+            const int l = lane * V.dof + k;
+            const double* p = V.dh + 4 * l;
+            const float d = (float)p[0], a = (float)p[1], ca = (float)p[2], sa = (float)p[3];
+            const float cs = W.cs[l], sn = W.sn[l];
+            const float A[3][4] = {{cs, -sn * ca, sn * sa, a * cs}, {sn, cs * ca, -cs * sa, a * sn}, {0.f, sa, ca, d}};
+            float Y[12];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    Y[3 * r + c] = X[3 * r] * A[0][c] + X[3 * r + 1] * A[1][c] + X[3 * r + 2] * A[2][c];
+                Y[9 + r] = X[3 * r] * A[0][3] + X[3 * r + 1] * A[1][3] + X[3 * r + 2] * A[2][3] + X[9 + r];
+            }
+#pragma unroll
+            for (int t = 0; t < 12; ++t) {
+                X[t] = Y[t];
+                W.X[12 * l + t] = Y[t];
+            }
+        }
+    }
+    __syncthreads();
+    for (int s = lane; s < V.S; s += 64) {
+        const float* X = W.X + 12 * V.sph_link[s];
+        const double* l = V.sph64 + 4 * s;
+        const float l0 = (float)l[0], l1 = (float)l[1], l2 = (float)l[2];
+        W.sph[s] = make_float4(X[0] * l0 + X[1] * l1 + X[2] * l2 + X[9], X[3] * l0 + X[4] * l1 + X[5] * l2 + X[10],
+                               X[6] * l0 + X[7] * l1 + X[8] * l2 + X[11], V.sph_r32[s]);
+    }
+    __syncthreads();
+}
+
+// Accumulates the cost of the pair set; when `grad`, also the forces in W.F (zeroed here).
+__device__ inline float clr_pairs(const ArmView& V, const ClrLds& W, float m, bool grad) {
+    const int lane = threadIdx.x;
+    const int S = V.S;
+    if (grad) {
+        for (int t = lane; t < 3 * S; t += 64) W.F[t] = 0.f;
+        __syncthreads();
+    }
+    float cost = 0.f;
+    for (int base = 0; base < S; base += 64) {
+        const int i = base + lane;
+        const bool act = i < S;
+        const float4 ci = W.sph[act ? i : 0];
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        // spheres vs boxes
+        for (int b = 0; b < V.nb; ++b) {
+            const float* bx = V.box32 + 8 * b;
+            const float ex = ci.x - fminf(fmaxf(ci.x, bx[0]), bx[4]);
+            const float ey = ci.y - fminf(fmaxf(ci.y, bx[1]), bx[5]);
+            const float ez = ci.z - fminf(fmaxf(ci.z, bx[2]), bx[6]);
+            const float d = sqrtf(ex * ex + ey * ey + ez * ez);
+            const float pen = m + ci.w - d;
+            if (act && pen > 0.f) {
+                cost += pen * pen;
+                if (grad && d > 0.f) {
+                    const float g = -2.f * pen / d;
+                    fx += g * ex;
+                    fy += g * ey;
+                    fz += g * ez;
+                }
+            }
+        }
+        // sphere pairs
+        const int lo = act ? V.sph_lo[i] : 0x7fffffff;
+        for (int j = V.sph_lo[base]; j < S; ++j) {
+            const float4 cj = W.sph[j];
+            const float ex = ci.x - cj.x, ey = ci.y - cj.y, ez = ci.z - cj.z;
+            const float d = sqrtf(ex * ex + ey * ey + ez * ez);
+            const float pen = m + ci.w + cj.w - d;
+            if (j >= lo && pen > 0.f) {
+                cost += pen * pen;
+                if (grad && d > 0.f) {
+                    const float g = -2.f * pen / d;
+                    fx += g * ex;
+                    fy += g * ey;
+                    fz += g * ez;
+                    atomicAdd(&W.F[3 * j], -g * ex);
+                    atomicAdd(&W.F[3 * j + 1], -g * ey);
+                    atomicAdd(&W.F[3 * j + 2], -g * ez);
+                }
+            }
+        }
+        if (grad && act) {
+            atomicAdd(&W.F[3 * i], fx);
+            atomicAdd(&W.F[3 * i + 1], fy);
+            atomicAdd(&W.F[3 * i + 2], fz);
+        }
+    }
+    // wave sum
+    W.red[lane] = cost;
+    __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) {
+        if (lane < off) W.red[lane] += W.red[lane + off];
+        __syncthreads();
+    }
+    const float total = W.red[0];
+    __syncthreads();
+    return total;
+}
+
+__global__ void __launch_bounds__(64) k_clearance(ArmView V, const float* q, long long n, float m,
+                                                  const float* grad_cost, float* cost, float* grad_q) {
+    extern __shared__ float lds_clr[];
+    ClrLds W = clr_lds(lds_clr, V);
+    const bool grad = grad_q != nullptr;
+    for (long long r = blockIdx.x; r < n; r += gridDim.x) {
+        const float* qr = q + r * V.D;
+        clr_fk(V, W, qr);
+        const float c = clr_pairs(V, W, m, grad);
+        if (!grad) {
+            if (threadIdx.x == 0) cost[r] = c;
+        } else {
+            __syncthreads();
+            // lane (arm a, joint k): g = z_{k-1} . sum_{s on links >= k} (c_s - o_{k-1}) x f_s
+            const int lane = threadIdx.x;
+            if (lane < V.D) {
+                const int a = lane / V.dof, k = lane % V.dof;
+                float z0, z1, z2, o0, o1, o2;
+                if (k == 0) {
+                    const double* P = V.base + 12 * a;
+                    z0 = (float)P[2]; z1 = (float)P[5]; z2 = (float)P[8];
+                    o0 = (float)P[9]; o1 = (float)P[10]; o2 = (float)P[11];
+                } else {
+                    const float* P = W.X + 12 * (a * V.dof + k - 1);
+                    z0 = P[2]; z1 = P[5]; z2 = P[8];
+                    o0 = P[9]; o1 = P[10]; o2 = P[11];
+                }
+                float g = 0.f;
+                const int s0 = V.link_first[a * V.dof + k], s1 = V.link_first[(a + 1) * V.dof];
+                for (int s = s0; s < s1; ++s) {
+                    const float4 cs = W.sph[s];
+                    const float w0 = cs.x - o0, w1 = cs.y - o1, w2 = cs.z - o2;
+                    const float f0 = W.F[3 * s], f1 = W.F[3 * s + 1], f2 = W.F[3 * s + 2];
+                    // z . (w x f)
+                    g += z0 * (w1 * f2 - w2 * f1) + z1 * (w2 * f0 - w0 * f2) + z2 * (w0 * f1 - w1 * f0);
+                }
+                grad_q[r * V.D + lane] = grad_cost[r] * g;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+size_t clearance_lds_bytes(const ArmView& V) {
+    return sizeof(float) * (2 * V.D + 12 * V.L) + 16 + sizeof(float4) * V.S + sizeof(float) * 3 * V.S +
+           sizeof(float) * 64;
+}
+
+void launch_clearance(const SceneRec& sc, const float* q, long long n, float margin,
+                      const float* grad_cost, float* cost, float* grad_q, hipStream_t st) {
+    long long g = n < 65536 ? n : 65536;
+    k_clearance<<<(int)g, 64, clearance_lds_bytes(sc.arm), st>>>(sc.arm, q, n, margin, grad_cost, cost, grad_q);
+}

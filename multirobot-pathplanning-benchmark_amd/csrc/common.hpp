@@ -1,0 +1,385 @@
+// common.hpp — shared device/host definitions of the collision engine (gfx950).
+//
+// Arithmetic contract
+//   * Everything that decides a boolean near contact is binary64 with contraction disabled
+//     (`#pragma clang fp contract(off)` inside the *_f64 functions): each product and sum rounds
+//     separately, exactly like the CPU definition of the predicates.
+//   * The bulk narrow phase runs in binary32 on coordinates rounded from the binary64 kinematics.
+//     Every binary32 test returns one of three answers: sure-free, sure-hit, or "band" when the
+//     value lies within a margin M of the contact threshold.  M bounds the binary32 error (see
+//     margin_* below), so sure answers agree with binary64; band pairs are re-evaluated in binary64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MRCC_MAX_ARMS 8
+#define MRCC_MAX_DOF 8
+#define MRCC_MAX_LINKS (MRCC_MAX_ARMS * MRCC_MAX_DOF)
+#define MRCC_MAX_SPHERES 512
+#define MRCC_MAX_D 64
+#define MRCC_DISK_MAX_RD 12
+#define MRCC_DISK_MAX_ROBOTS 16
+#define MRCC_PLANAR_MAX_ROBOTS 16
+#define MRCC_MAX_PASSES 16384
+
+enum { KIND_DISKS = 1, KIND_PLANAR = 2, KIND_ARMS = 3 };
+
+// --------------------------------------------------------------------------------------------
+// binary64 sincos (Cody-Waite reduction by pi/2 in three parts, Taylor polynomials on
+// [-pi/4, pi/4], Horner, no FMA).  The same sequence of operations on host and device gives the
+// same bits on both.
+namespace mrcc {
+
+struct SinCos {
+    double s, c;
+};
+
+__host__ __device__ inline SinCos sincos_f64(double x) {
+#pragma clang fp contract(off)
+    const double TWO_OVER_PI = 0x1.45f306dc9c883p-1;
+    const double PIO2_1 = 0x1.921fb54400000p+0;
+    const double PIO2_2 = 0x1.0b4611a600000p-34;
+    const double PIO2_3 = 0x1.3198a2e037073p-69;
+    double kd = __builtin_rint(x * TWO_OVER_PI);
+    double r = ((x - kd * PIO2_1) - kd * PIO2_2) - kd * PIO2_3;
+    double r2 = r * r;
+    double ps = 0x1.952c77030ad4ap-49;
+    ps = -0x1.ae7f3e733b81fp-41 + r2 * ps;
+    ps = 0x1.6124613a86d09p-33 + r2 * ps;
+    ps = -0x1.ae64567f544e4p-26 + r2 * ps;
+    ps = 0x1.71de3a556c734p-19 + r2 * ps;
+    ps = -0x1.a01a01a01a01ap-13 + r2 * ps;
+    ps = 0x1.1111111111111p-7 + r2 * ps;
+    ps = -0x1.5555555555555p-3 + r2 * ps;
+    double sn = r + (r * r2) * ps;
+    double pc = -0x1.6827863b97d97p-53;
+    pc = 0x1.ae7f3e733b81fp-45 + r2 * pc;
+    pc = -0x1.93974a8c07c9dp-37 + r2 * pc;
+    pc = 0x1.1eed8eff8d898p-29 + r2 * pc;
+    pc = -0x1.27e4fb7789f5cp-22 + r2 * pc;
+    pc = 0x1.a01a01a01a01ap-16 + r2 * pc;
+    pc = -0x1.6c16c16c16c17p-10 + r2 * pc;
+    pc = 0x1.5555555555555p-5 + r2 * pc;
+    double cs = (1.0 - 0.5 * r2) + (r2 * r2) * pc;
+    double qd = kd - 4.0 * __builtin_floor(kd * 0.25);
+    int q = (int)qd & 3;
+    SinCos o;
+    if (q == 0) { o.s = sn; o.c = cs; }
+    else if (q == 1) { o.s = cs; o.c = -sn; }
+    else if (q == 2) { o.s = -sn; o.c = -cs; }
+    else { o.s = -cs; o.c = sn; }
+    return o;
+}
+
+// X = [R row-major | t]; X <- X * A(q) for one DH joint (d, a, cos alpha, sin alpha)
+__host__ __device__ inline void dh_step_f64(double* X, double d, double a, double ca, double sa,
+                                            SinCos j) {
+#pragma clang fp contract(off)
+    const double sn = j.s, cs = j.c;
+    double A[3][4];
+    A[0][0] = cs; A[0][1] = -(sn * ca); A[0][2] = sn * sa; A[0][3] = a * cs;
+    A[1][0] = sn; A[1][1] = cs * ca;   A[1][2] = -(cs * sa); A[1][3] = a * sn;
+    A[2][0] = 0.0; A[2][1] = sa;       A[2][2] = ca;         A[2][3] = d;
+    double Y[12];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const double R0 = X[3 * r], R1 = X[3 * r + 1], R2 = X[3 * r + 2];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Y[3 * r + c] = (R0 * A[0][c] + R1 * A[1][c]) + R2 * A[2][c];
+        Y[9 + r] = ((R0 * A[0][3] + R1 * A[1][3]) + R2 * A[2][3]) + X[9 + r];
+    }
+#pragma unroll
+    for (int k = 0; k < 12; ++k) X[k] = Y[k];
+}
+
+__host__ __device__ inline void xform_point_f64(const double* X, double l0, double l1, double l2,
+                                                double* c) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) c[r] = ((X[3 * r] * l0 + X[3 * r + 1] * l1) + X[3 * r + 2] * l2) + X[9 + r];
+}
+
+__host__ __device__ inline void compose_f64(const double* X, const double* Rel, double* G) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            G[3 * r + c] = (X[3 * r] * Rel[c] + X[3 * r + 1] * Rel[3 + c]) + X[3 * r + 2] * Rel[6 + c];
+        G[9 + r] = ((X[3 * r] * Rel[9] + X[3 * r + 1] * Rel[10]) + X[3 * r + 2] * Rel[11]) + X[9 + r];
+    }
+}
+
+// closed sphere vs axis-aligned box [lo, hi]
+__host__ __device__ inline bool sph_box_f64(const double* c, double r, const double* lo,
+                                            const double* hi) {
+#pragma clang fp contract(off)
+    double e[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double m = c[k] >= lo[k] ? c[k] : lo[k];
+        m = m <= hi[k] ? m : hi[k];
+        e[k] = m - c[k];
+    }
+    return ((e[0] * e[0] + e[1] * e[1]) + e[2] * e[2]) <= r * r;
+}
+
+__host__ __device__ inline bool sph_sph_f64(const double* a, double ra, const double* b, double rb) {
+#pragma clang fp contract(off)
+    double d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+    double R = ra + rb;
+    return ((d0 * d0 + d1 * d1) + d2 * d2) <= R * R;
+}
+
+// closed sphere vs oriented box G = [R | t] with half extents h
+__host__ __device__ inline bool sph_obb_f64(const double* c, double r, const double* G,
+                                            const double* h) {
+#pragma clang fp contract(off)
+    double w0 = c[0] - G[9], w1 = c[1] - G[10], w2 = c[2] - G[11];
+    double e[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double l = (G[k] * w0 + G[3 + k] * w1) + G[6 + k] * w2;
+        double m = l >= -h[k] ? l : -h[k];
+        m = m <= h[k] ? m : h[k];
+        e[k] = m - l;
+    }
+    return ((e[0] * e[0] + e[1] * e[1]) + e[2] * e[2]) <= r * r;
+}
+
+// closed 15-axis separating-axis overlap of boxes A = [R | t] (half a) and B (half b)
+__host__ __device__ inline bool obb3_overlap_f64(const double* A, const double* a, const double* B,
+                                                 const double* b) {
+#pragma clang fp contract(off)
+    double Rm[3][3], AR[3][3], t[3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            Rm[i][j] = (A[i] * B[j] + A[3 + i] * B[3 + j]) + A[6 + i] * B[6 + j];
+            AR[i][j] = __builtin_fabs(Rm[i][j]);
+        }
+    double D0 = B[9] - A[9], D1 = B[10] - A[10], D2 = B[11] - A[11];
+    for (int i = 0; i < 3; ++i) t[i] = (A[i] * D0 + A[3 + i] * D1) + A[6 + i] * D2;
+    for (int i = 0; i < 3; ++i) {
+        double rb = (b[0] * AR[i][0] + b[1] * AR[i][1]) + b[2] * AR[i][2];
+        if (__builtin_fabs(t[i]) - (a[i] + rb) > 0.0) return false;
+    }
+    for (int j = 0; j < 3; ++j) {
+        double ra = (a[0] * AR[0][j] + a[1] * AR[1][j]) + a[2] * AR[2][j];
+        double tp = (t[0] * Rm[0][j] + t[1] * Rm[1][j]) + t[2] * Rm[2][j];
+        if (__builtin_fabs(tp) - (ra + b[j]) > 0.0) return false;
+    }
+    for (int i = 0; i < 3; ++i) {
+        int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+        for (int j = 0; j < 3; ++j) {
+            int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            double tp = t[i2] * Rm[i1][j] - t[i1] * Rm[i2][j];
+            double rr = ((a[i1] * AR[i2][j] + a[i2] * AR[i1][j]) + b[j1] * AR[i][j2]) + b[j2] * AR[i][j1];
+            if (__builtin_fabs(tp) - rr > 0.0) return false;
+        }
+    }
+    return true;
+}
+
+// binary32 15-axis SAT: returns 1 sure-overlap, 0 sure-separated, 2 band (|value| <= M on an
+// axis, no axis surely separating)
+__device__ inline int obb3_overlap_f32(const float* A, const float* a, const float* B,
+                                       const float* b, float M) {
+    float Rm[3][3], AR[3][3], t[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            Rm[i][j] = A[i] * B[j] + A[3 + i] * B[3 + j] + A[6 + i] * B[6 + j];
+            AR[i][j] = fabsf(Rm[i][j]);
+        }
+    float D0 = B[9] - A[9], D1 = B[10] - A[10], D2 = B[11] - A[11];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) t[i] = A[i] * D0 + A[3 + i] * D1 + A[6 + i] * D2;
+    float mx = -3.0e38f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        mx = fmaxf(mx, fabsf(t[i]) - (a[i] + b[0] * AR[i][0] + b[1] * AR[i][1] + b[2] * AR[i][2]));
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+        mx = fmaxf(mx, fabsf(t[0] * Rm[0][j] + t[1] * Rm[1][j] + t[2] * Rm[2][j]) -
+                           (a[0] * AR[0][j] + a[1] * AR[1][j] + a[2] * AR[2][j] + b[j]));
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            mx = fmaxf(mx, fabsf(t[i2] * Rm[i1][j] - t[i1] * Rm[i2][j]) -
+                               (a[i1] * AR[i2][j] + a[i2] * AR[i1][j] + b[j1] * AR[i][j2] + b[j2] * AR[i][j1]));
+        }
+    }
+    if (mx > M) return 0;
+    if (mx < -M) return 1;
+    return 2;
+}
+
+// --------------------------------------------------------------------------------------------
+// breadth-first midpoint order of planning_env.py:34-51 without tables: the k-th element.
+// Levels of the interval tree fill completely except the last; the last level holds the
+// length-1 intervals, found by descending with the count of non-empty nodes per subtree.
+__host__ __device__ inline int ilog2_floor(unsigned v) { return 31 - __builtin_clz(v); }
+
+// number of non-empty nodes at depth dd in the tree of an interval of length n
+__host__ __device__ inline int bfs_level_count(int n, int dd) {
+    if (n <= 0) return 0;
+    int D = ilog2_floor((unsigned)n + 1u);  // levels 0..D-1 are full
+    if (dd < D) return 1 << dd;
+    if (dd == D) return n - ((1 << D) - 1);
+    return 0;
+}
+
+__host__ __device__ inline int bfs_index(int N, int k) {
+    int D = ilog2_floor((unsigned)N + 1u);
+    int s = 0, e = N - 1;
+    if (k < (1 << D) - 1) {
+        int L = ilog2_floor((unsigned)k + 1u);
+        int o = k - ((1 << L) - 1);
+        for (int b = L - 1; b >= 0; --b) {
+            int mid = (s + e) >> 1;
+            if ((o >> b) & 1) s = mid + 1;
+            else e = mid - 1;
+        }
+        return (s + e) >> 1;
+    }
+    int o = k - ((1 << D) - 1);
+    for (int dd = D; dd > 0; --dd) {
+        int mid = (s + e) >> 1;
+        int nl = bfs_level_count(mid - s, dd - 1);
+        if (o < nl) e = mid - 1;
+        else { o -= nl; s = mid + 1; }
+    }
+    return (s + e) >> 1;
+}
+
+// --------------------------------------------------------------------------------------------
+// edge window bookkeeping (abstract_env.py:321-352)
+struct EdgeWin {
+    int N;        // state count
+    int w0;       // first BFS position of the window
+    int M;        // slots (BFS positions or interior states) to visit
+    int full;     // 1: whole edge (visiting order is free)
+    int first;    // full edges: first state index (0 with endpoints, else 1)
+    int err;      // 1: N_start > N
+};
+
+__host__ __device__ inline int edge_N_from_max(double m, double res) {
+#pragma clang fp contract(off)
+    double t = m / res;
+    if (!(t < 1.0e9)) return 1000000001;  // saturate (rejected by the planner of the batch)
+    int N = (int)t + 1;
+    return N < 2 ? 2 : N;
+}
+
+__host__ __device__ inline EdgeWin edge_window(int N, int N_start, int N_max, int incl) {
+    EdgeWin w;
+    w.N = N;
+    w.err = (N_start < 0 || N_start > N) ? 1 : 0;
+    int hi = (N_max < 0 || N_max > N) ? N : N_max;
+    w.full = (N_start == 0 && hi == N) ? 1 : 0;
+    w.first = incl ? 0 : 1;
+    if (w.err) { w.M = 0; w.w0 = 0; return w; }
+    if (w.full) {
+        w.w0 = 0;
+        w.M = incl ? N : N - 2;
+        if (w.M < 0) w.M = 0;
+    } else {
+        w.w0 = N_start;
+        w.M = hi > N_start ? hi - N_start : 0;
+    }
+    return w;
+}
+
+// state index of slot `local` (0 <= local < M) of an edge, or -1 when the slot is an endpoint
+// that the call excludes
+__host__ __device__ inline int edge_state_index(const EdgeWin& w, int local, int incl) {
+    if (w.full) return w.first + local;
+    int i = bfs_index(w.N, w.w0 + local);
+    if (!incl && (i == 0 || i == w.N - 1)) return -1;
+    return i;
+}
+
+}  // namespace mrcc
+
+// --------------------------------------------------------------------------------------------
+// device views of the scenes (plain pointers into one library-owned device allocation)
+struct DiskView {
+    int nr, rd, D, ns, nrect;
+    float Bscene;
+    const float* r32;                       // [nr]
+    const float* rrT32; const double* rrT64;  // [nr*nr] thresholds of sqrt(d2) < r_i + r_j
+    const float* sc32; const double* sc64;    // [ns*rd]
+    const float* srT32; const double* srT64;  // [nr*ns] thresholds of sqrt(d2) < r_o + r_i
+    const float* lo32; const float* hi32;     // [nrect*rd]
+    const double* lo64; const double* hi64;
+    const float* rp2_32; const double* rp2_64;  // [nr] r**2
+};
+
+struct PlanarView {
+    int nr, no, D;
+    float Bscene;
+    const float* rh32; const double* rh64;  // [nr*2]
+    const float* ob32; const double* ob64;  // [no*8] x y ux uy vx vy h0 h1
+};
+
+struct ArmView {
+    int na, dof, D, L, S, nb, nm;
+    float Msph;   // binary32 margin of sphere tests (squared-distance units)
+    float Mobb;   // binary32 margin of box SAT axis values
+    const double* base;      // [na*12]
+    const double* dh;        // [L*4] d a cos(alpha) sin(alpha)
+    const int* link_first;   // [L+1]
+    const int* sph_link;     // [S] global link index
+    const int* sph_lo;       // [S] first allowed partner sphere (same arm, links >= link+2)
+    const double* sph64;     // [S*4] local xyz r
+    const float* sph_r32;    // [S]
+    const float* box32;      // [nb*8] lo xyz, pad, hi xyz, pad
+    const double* box64;     // [nb*6] lo xyz hi xyz
+    const double* boxch64;   // [nb*6] centre xyz half xyz
+    const float* boxch32;    // [nb*6]
+    const int* m_active;     // [nm]
+    const unsigned char* m_grasp;  // [nm*na]
+    const double* m_rel;     // [nm*na*12]
+    const double* m_half;    // [nm*na*3]
+};
+
+// batch of edges as seen by kernels
+struct EdgeBatchView {
+    const double* q1;
+    const double* q2;
+    long long rs, ds;  // row / dim strides
+    const int* idx1;
+    const int* idx2;
+    long long n;
+    double res;
+    const int* N;
+    const int* N_start;
+    const int* N_max;
+    int incl;
+    const int* modes;
+    unsigned mask;
+    unsigned char* out;
+};
+
+// workspace of a batched plan
+struct PlanWork {
+    int* eN;           // [n] state count
+    int* eM;           // [n] slots
+    int* eS;           // [n] passes
+    int* ew0;          // [n] window start
+    unsigned char* efull;  // [n]
+    int* order;        // [n] edges longest-first
+    int* hist;         // [MAX_PASSES+1]
+    int* cursor;       // [MAX_PASSES+1]
+    long long* pass_start;  // [MAX_PASSES+1] slot offset of each pass (G * sum E)
+    long long* meta;   // [8]: 0 total slots, 1 n_pass, 2 error count, 3 long edges, 4 tickets
+    int* long_list;    // [n] edges with more passes than MAX_PASSES
+    unsigned* ticket;  // [1]
+    unsigned long long* stats;  // [4]
+    double* dir;       // [n*D] (lane-per-state kinds) or null
+};

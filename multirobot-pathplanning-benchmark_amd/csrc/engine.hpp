@@ -1,0 +1,42 @@
+// engine.hpp — host-side records of the collision engine and the kernel launchers.
+#pragma once
+
+#include "evaluators.hpp"
+
+// A scene as held by the library: the kind, its device view and the device allocation behind it.
+struct SceneRec {
+    int kind;
+    int D;
+    int n_robots;
+    int device;
+    DiskView disk;
+    PlanarView planar;
+    ArmView arm;
+    size_t arm_lds;       // dynamic LDS bytes of the arm kernels
+    void* dev_mem;        // one allocation holding every array the views point to
+    size_t dev_bytes;
+    // host copies needed by host code
+    int arm_na, arm_dof, arm_S, arm_nm;
+    // staging buffers of the host-pointer per-call entry points (lazily allocated)
+    double* stage_dev;
+    double* stage_host;
+    unsigned char* stage_out_dev;
+    unsigned char* stage_out_host;
+};
+
+void launch_plan_build(const SceneRec& sc, const EdgeBatchView& B, const PlanWork& W, int lgG,
+                       hipStream_t st);
+void launch_plan_run(const SceneRec& sc, const EdgeBatchView& B, const PlanWork& W, int lgG,
+                     int spt, int grid, hipStream_t st);
+void launch_direct(const SceneRec& sc, const EdgeBatchView& B, const int* list, long long n_list,
+                   const long long* n_dev, int wpe, int init_out, int max_grid,
+                   unsigned long long* stats, hipStream_t st);
+void launch_fill_u8(unsigned char* p, long long n, unsigned char v, hipStream_t st);
+void launch_configs(const SceneRec& sc, const double* q, long long n, long long rs, long long ds,
+                    const int* modes, unsigned mask, unsigned char* out, int max_grid,
+                    hipStream_t st);
+void launch_arm_fk(const SceneRec& sc, const double* q, long long n, long long rs, long long ds,
+                   double* out, hipStream_t st);
+void launch_sincos(const double* x, long long n, double* s, double* c, hipStream_t st);
+void launch_clearance(const SceneRec& sc, const float* q, long long n, float margin,
+                      const float* grad_cost, float* cost, float* grad_q, hipStream_t st);

@@ -1,0 +1,551 @@
+// evaluators.hpp — per-scene state evaluators of the collision engine (device code).
+//
+// Disk scenes (the reference's AbstractEnvironment, abstract_env.py:41-109, :255-276) and planar box
+// scenes are evaluated one state per lane; arm scenes one state per wavefront (64 lanes share the
+// FK and split the pair set).  Every evaluator returns true when the state is in collision.
+#pragma once
+
+#include "common.hpp"
+
+using namespace mrcc;
+
+// --------------------------------------------------------------------------------------------
+// binary64 predicates with the reference's numpy rounding sequence
+__device__ inline double np_sqnorm_f64(const double* v, int n) {
+    // np.dot(v, v) (OpenBLAS scalar tail loop): sequential fused chain
+    double s = 0.0;
+    for (int k = 0; k < n; ++k) s = __builtin_fma(v[k], v[k], s);
+    return s;
+}
+
+__device__ inline double np_sum_f64(const double* a, int n) {
+#pragma clang fp contract(off)
+    // np.sum: identity 0 + numpy pairwise_sum (n <= 128)
+    double res;
+    if (n < 8) {
+        res = -0.0;
+        for (int i = 0; i < n; ++i) res += a[i];
+    } else {
+        double r[8];
+        int i;
+        for (int j = 0; j < 8; ++j) r[j] = a[j];
+        for (i = 8; i < n - (n % 8); i += 8)
+            for (int j = 0; j < 8; ++j) r[j] += a[i + j];
+        res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+        for (; i < n; ++i) res += a[i];
+    }
+    return 0.0 + res;
+}
+
+__device__ inline double np_clip_f64(double x, double lo, double hi) {
+    // NaN-propagating _NPY_MIN(_NPY_MAX(x, lo), hi)
+    double m = __builtin_isnan(x) ? x : (x > lo ? x : lo);
+    return __builtin_isnan(m) ? m : (m < hi ? m : hi);
+}
+
+// robot pair: np.linalg.norm(q[i] - q[j]) < r_i + r_j  <=>  sqnorm < T(r_i + r_j)
+__device__ __attribute__((noinline)) bool disk_rr_f64(const DiskView& S, const double* q, int i, int j) {
+#pragma clang fp contract(off)
+    double d[MRCC_DISK_MAX_RD];
+    for (int k = 0; k < S.rd; ++k) d[k] = q[i * S.rd + k] - q[j * S.rd + k];
+    return np_sqnorm_f64(d, S.rd) < S.rrT64[i * S.nr + j];
+}
+
+// Sphere.collides_with_sphere: np.linalg.norm(pos - center) < r_o + r_i
+__device__ __attribute__((noinline)) bool disk_rs_f64(const DiskView& S, const double* q, int i, int o) {
+#pragma clang fp contract(off)
+    double d[MRCC_DISK_MAX_RD];
+    for (int k = 0; k < S.rd; ++k) d[k] = S.sc64[o * S.rd + k] - q[i * S.rd + k];
+    return np_sqnorm_f64(d, S.rd) < S.srT64[i * S.ns + o];
+}
+
+// Rectangle.collides_with_sphere: sum((clip(p, min, max) - p)**2) <= r**2
+__device__ __attribute__((noinline)) bool disk_rect_f64(const DiskView& S, const double* q, int i, int o) {
+#pragma clang fp contract(off)
+    double sq[MRCC_DISK_MAX_RD];
+    for (int k = 0; k < S.rd; ++k) {
+        const double p = q[i * S.rd + k];
+        const double e = np_clip_f64(p, S.lo64[o * S.rd + k], S.hi64[o * S.rd + k]) - p;
+        sq[k] = e * e;
+    }
+    return np_sum_f64(sq, S.rd) <= S.rp2_64[i];
+}
+
+// literal binary64 evaluation (used for non-finite coordinates)
+__device__ __attribute__((noinline)) bool disk_collides_f64(const DiskView& S, const double* q, unsigned mask) {
+    for (int i = 0; i < S.nr; ++i)
+        for (int j = i + 1; j < S.nr; ++j)
+            if (((mask >> i) & 1u) && ((mask >> j) & 1u) && disk_rr_f64(S, q, i, j)) return true;
+    for (int i = 0; i < S.nr; ++i) {
+        if (!((mask >> i) & 1u)) continue;
+        for (int o = 0; o < S.ns; ++o)
+            if (disk_rs_f64(S, q, i, o)) return true;
+        for (int o = 0; o < S.nrect; ++o)
+            if (disk_rect_f64(S, q, i, o)) return true;
+    }
+    return false;
+}
+
+// Margin of the binary32 squared-distance tests of disk scenes.  Coordinates are rounded from
+// binary64 (error <= 2^-24 B each), so a squared distance near a threshold R^2 is off by at most
+// ~2^-20 rd (B + 2) in binary32; M keeps a factor >= 4 over that bound for B >= 1.
+__device__ inline float disk_margin(float B, int rd) {
+    float t = 1.0f + B;
+    return 0x1p-18f * t * t * (float)(rd > 2 ? rd : 2) * 0.5f;
+}
+
+template <int RDT, int NRT>
+__device__ inline bool disk_collides(const DiskView& S, const double* q, unsigned mask,
+                                     unsigned& rechecks) {
+    constexpr int MAXD = (RDT ? RDT : MRCC_DISK_MAX_RD) * (NRT ? NRT : MRCC_DISK_MAX_ROBOTS);
+    const int rd = RDT ? RDT : S.rd;
+    const int nr = NRT ? NRT : S.nr;
+    const int D = rd * nr;
+    float p[MAXD];
+    float B = S.Bscene;
+    bool finite = true;
+#pragma unroll
+    for (int k = 0; k < MAXD; ++k) {
+        if (k < D) {
+            p[k] = (float)q[k];
+            B = fmaxf(B, fabsf(p[k]));
+            finite = finite && __builtin_isfinite(q[k]);
+        }
+    }
+    if (!finite) return disk_collides_f64(S, q, mask);
+    const float M = disk_margin(B, rd);
+#pragma unroll
+    for (int i = 0; i < (NRT ? NRT : MRCC_DISK_MAX_ROBOTS); ++i) {
+        if (i >= nr) break;
+#pragma unroll
+        for (int j = i + 1; j < (NRT ? NRT : MRCC_DISK_MAX_ROBOTS); ++j) {
+            if (j >= nr) break;
+            if (!((mask >> i) & 1u) || !((mask >> j) & 1u)) continue;
+            float d2 = 0.0f;
+#pragma unroll
+            for (int k = 0; k < (RDT ? RDT : MRCC_DISK_MAX_RD); ++k) {
+                if (k >= rd) break;
+                float d = p[i * rd + k] - p[j * rd + k];
+                d2 = fmaf(d, d, d2);
+            }
+            const float T = S.rrT32[i * nr + j];
+            if (d2 < T + M) {
+                if (d2 < T - M) return true;
+                ++rechecks;
+                if (disk_rr_f64(S, q, i, j)) return true;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < (NRT ? NRT : MRCC_DISK_MAX_ROBOTS); ++i) {
+        if (i >= nr) break;
+        if (!((mask >> i) & 1u)) continue;
+        for (int o = 0; o < S.ns; ++o) {
+            float d2 = 0.0f;
+#pragma unroll
+            for (int k = 0; k < (RDT ? RDT : MRCC_DISK_MAX_RD); ++k) {
+                if (k >= rd) break;
+                float d = S.sc32[o * rd + k] - p[i * rd + k];
+                d2 = fmaf(d, d, d2);
+            }
+            const float T = S.srT32[i * S.ns + o];
+            if (d2 < T + M) {
+                if (d2 < T - M) return true;
+                ++rechecks;
+                if (disk_rs_f64(S, q, i, o)) return true;
+            }
+        }
+        const float T = S.rp2_32[i];
+        for (int o = 0; o < S.nrect; ++o) {
+            float d2 = 0.0f;
+#pragma unroll
+            for (int k = 0; k < (RDT ? RDT : MRCC_DISK_MAX_RD); ++k) {
+                if (k >= rd) break;
+                const float pk = p[i * rd + k];
+                const float e = fminf(fmaxf(pk, S.lo32[o * rd + k]), S.hi32[o * rd + k]) - pk;
+                d2 = fmaf(e, e, d2);
+            }
+            if (d2 <= T + M) {
+                if (d2 < T - M) return true;
+                ++rechecks;
+                if (disk_rect_f64(S, q, i, o)) return true;
+            }
+        }
+    }
+    return false;
+}
+
+// --------------------------------------------------------------------------------------------
+// planar oriented boxes: closed 2D separating-axis test.  Box = [x y ux uy vx vy h0 h1].
+__device__ __attribute__((noinline)) bool obb2_overlap_f64(const double* A, const double* B) {
+#pragma clang fp contract(off)
+    double tx = B[0] - A[0], ty = B[1] - A[1];
+    double C = A[2] * B[2] + A[3] * B[3];
+    double Sv = A[4] * B[2] + A[5] * B[3];
+    double aC = __builtin_fabs(C), aS = __builtin_fabs(Sv);
+    double tA0 = tx * A[2] + ty * A[3];
+    double tA1 = tx * A[4] + ty * A[5];
+    double tB0 = tx * B[2] + ty * B[3];
+    double tB1 = tx * B[4] + ty * B[5];
+    double s0 = __builtin_fabs(tA0) - ((A[6] + B[6] * aC) + B[7] * aS);
+    double s1 = __builtin_fabs(tA1) - ((A[7] + B[6] * aS) + B[7] * aC);
+    double s2 = __builtin_fabs(tB0) - ((B[6] + A[6] * aC) + A[7] * aS);
+    double s3 = __builtin_fabs(tB1) - ((B[7] + A[6] * aS) + A[7] * aC);
+    return s0 <= 0.0 && s1 <= 0.0 && s2 <= 0.0 && s3 <= 0.0;
+}
+
+// binary64 robot box [x y ux uy vx vy h0 h1] of robot i at q
+__device__ inline void planar_robot_box_f64(const PlanarView& S, const double* q, int i, double* A) {
+    SinCos t = sincos_f64(q[3 * i + 2]);
+    A[0] = q[3 * i];
+    A[1] = q[3 * i + 1];
+    A[2] = t.c; A[3] = t.s;
+    A[4] = -t.s; A[5] = t.c;
+    A[6] = S.rh64[2 * i];
+    A[7] = S.rh64[2 * i + 1];
+}
+
+__device__ __attribute__((noinline)) bool planar_rr_f64(const PlanarView& S, const double* q, int i, int j) {
+    double A[8], B[8];
+    planar_robot_box_f64(S, q, i, A);
+    planar_robot_box_f64(S, q, j, B);
+    return obb2_overlap_f64(A, B);
+}
+
+__device__ __attribute__((noinline)) bool planar_ro_f64(const PlanarView& S, const double* q, int i, int o) {
+    double A[8];
+    planar_robot_box_f64(S, q, i, A);
+    return obb2_overlap_f64(A, S.ob64 + 8 * o);
+}
+
+// binary32 separation max over the 4 axes (> 0: separated)
+__device__ inline float obb2_sep_f32(float ax, float ay, float ac, float as, float ah0, float ah1,
+                                     float bx, float by, float bc, float bs, float bh0, float bh1) {
+    // A axes u=(ac,as), v=(-as,ac); B axes likewise
+    const float tx = bx - ax, ty = by - ay;
+    const float C = ac * bc + as * bs;
+    const float Sv = ac * bs - as * bc;  // va.ub = -as*bc + ac*bs
+    const float aC = fabsf(C), aS = fabsf(Sv);
+    const float tA0 = tx * ac + ty * as;
+    const float tA1 = ty * ac - tx * as;
+    const float tB0 = tx * bc + ty * bs;
+    const float tB1 = ty * bc - tx * bs;
+    float s = fabsf(tA0) - (ah0 + bh0 * aC + bh1 * aS);
+    s = fmaxf(s, fabsf(tA1) - (ah1 + bh0 * aS + bh1 * aC));
+    s = fmaxf(s, fabsf(tB0) - (bh0 + ah0 * aC + ah1 * aS));
+    s = fmaxf(s, fabsf(tB1) - (bh1 + ah0 * aS + ah1 * aC));
+    return s;
+}
+
+// Margin of the binary32 axis values: positions and axes are rounded from binary64, so each axis
+// value is off by ~2^-22 (B + H); M = 2^-18 (1 + B) keeps a factor >= 8.
+__device__ inline float planar_margin(float B) { return 0x1p-18f * (1.0f + B); }
+
+template <int NRT>
+__device__ inline bool planar_collides(const PlanarView& S, const double* q, unsigned mask,
+                                       unsigned& rechecks) {
+    constexpr int NR_MAX = NRT ? NRT : MRCC_PLANAR_MAX_ROBOTS;
+    const int nr = NRT ? NRT : S.nr;
+    float x[NR_MAX], y[NR_MAX], c[NR_MAX], s[NR_MAX], h0[NR_MAX], h1[NR_MAX];
+    float B = S.Bscene;
+    bool finite = true;
+#pragma unroll
+    for (int i = 0; i < NR_MAX; ++i) {
+        if (i >= nr) break;
+        finite = finite && __builtin_isfinite(q[3 * i]) && __builtin_isfinite(q[3 * i + 1]) &&
+                 __builtin_isfinite(q[3 * i + 2]);
+        SinCos t = sincos_f64(q[3 * i + 2]);
+        x[i] = (float)q[3 * i];
+        y[i] = (float)q[3 * i + 1];
+        c[i] = (float)t.c;
+        s[i] = (float)t.s;
+        h0[i] = S.rh32[2 * i];
+        h1[i] = S.rh32[2 * i + 1];
+        B = fmaxf(B, fmaxf(fabsf(x[i]), fabsf(y[i])));
+    }
+    if (!finite) return true;  // non-finite coordinates are reported as collisions
+    const float M = planar_margin(B);
+#pragma unroll
+    for (int i = 0; i < NR_MAX; ++i) {
+        if (i >= nr) break;
+#pragma unroll
+        for (int j = i + 1; j < NR_MAX; ++j) {
+            if (j >= nr) break;
+            if (!((mask >> i) & 1u) || !((mask >> j) & 1u)) continue;
+            float v = obb2_sep_f32(x[i], y[i], c[i], s[i], h0[i], h1[i], x[j], y[j], c[j], s[j], h0[j], h1[j]);
+            if (v <= M) {
+                if (v < -M) return true;
+                ++rechecks;
+                if (planar_rr_f64(S, q, i, j)) return true;
+            }
+        }
+    }
+    for (int o = 0; o < S.no; ++o) {
+        const float* ob = S.ob32 + 8 * o;
+        const float ox = ob[0], oy = ob[1], oc = ob[2], os = ob[3], oh0 = ob[6], oh1 = ob[7];
+#pragma unroll
+        for (int i = 0; i < NR_MAX; ++i) {
+            if (i >= nr) break;
+            if (!((mask >> i) & 1u)) continue;
+            float v = obb2_sep_f32(x[i], y[i], c[i], s[i], h0[i], h1[i], ox, oy, oc, os, oh0, oh1);
+            if (v <= M) {
+                if (v < -M) return true;
+                ++rechecks;
+                if (planar_ro_f64(S, q, i, o)) return true;
+            }
+        }
+    }
+    return false;
+}
+
+// --------------------------------------------------------------------------------------------
+// arms: one state per wavefront (block of 64 lanes).  Per-wave LDS layout (doubles first):
+//   q[D] cs[D] sn[D] X[L*12] G64[na*12] | sph[S] float4 | G32[na*16] float
+struct ArmLds {
+    double* q;
+    double* cs;
+    double* sn;
+    double* X;
+    double* G64;
+    float4* sph;
+    float* G32;
+};
+
+// doubles before the float4 sphere table, rounded up to an even count (16-byte alignment)
+__host__ __device__ inline int arm_lds_doubles(int D, int L, int na) {
+    return (3 * D + 12 * L + 12 * na + 1) & ~1;
+}
+
+__host__ __device__ inline size_t arm_lds_bytes(int D, int L, int na, int S) {
+    return sizeof(double) * (size_t)arm_lds_doubles(D, L, na) + sizeof(float4) * (size_t)S +
+           sizeof(float) * 16 * (size_t)na;
+}
+
+__device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
+    ArmLds W;
+    double* d = (double*)base;
+    W.q = d;
+    W.cs = d + V.D;
+    W.sn = d + 2 * V.D;
+    W.X = d + 3 * V.D;
+    W.G64 = W.X + 12 * V.L;
+    W.sph = (float4*)(d + arm_lds_doubles(V.D, V.L, V.na));
+    W.G32 = (float*)(W.sph + V.S);
+    return W;
+}
+
+__device__ inline void sphere_center_f64(const ArmView& V, const ArmLds& W, int s, double* c) {
+    const double* l = V.sph64 + 4 * s;
+    xform_point_f64(W.X + 12 * V.sph_link[s], l[0], l[1], l[2], c);
+}
+
+__device__ __attribute__((noinline)) bool arm_recheck_sph_box(const ArmView& V, const ArmLds& W, int s, int b) {
+    double c[3];
+    sphere_center_f64(V, W, s, c);
+    return sph_box_f64(c, V.sph64[4 * s + 3], V.box64 + 6 * b, V.box64 + 6 * b + 3);
+}
+
+__device__ __attribute__((noinline)) bool arm_recheck_sph_sph(const ArmView& V, const ArmLds& W, int i, int j) {
+    double a[3], b[3];
+    sphere_center_f64(V, W, i, a);
+    sphere_center_f64(V, W, j, b);
+    return sph_sph_f64(a, V.sph64[4 * i + 3], b, V.sph64[4 * j + 3]);
+}
+
+__device__ __attribute__((noinline)) bool arm_recheck_sph_obb(const ArmView& V, const ArmLds& W, int s, int a, int mode) {
+    double c[3];
+    sphere_center_f64(V, W, s, c);
+    return sph_obb_f64(c, V.sph64[4 * s + 3], W.G64 + 12 * a, V.m_half + 3 * (mode * V.na + a));
+}
+
+__device__ __attribute__((noinline)) bool arm_recheck_obb_box(const ArmView& V, const ArmLds& W, int a, int b, int mode) {
+    double A[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+    const double* bc = V.boxch64 + 6 * b;
+    A[9] = bc[0];
+    A[10] = bc[1];
+    A[11] = bc[2];
+    return obb3_overlap_f64(A, bc + 3, W.G64 + 12 * a, V.m_half + 3 * (mode * V.na + a));
+}
+
+__device__ __attribute__((noinline)) bool arm_recheck_obb_obb(const ArmView& V, const ArmLds& W, int a, int b, int mode) {
+    return obb3_overlap_f64(W.G64 + 12 * a, V.m_half + 3 * (mode * V.na + a), W.G64 + 12 * b,
+                            V.m_half + 3 * (mode * V.na + b));
+}
+
+// Evaluates the state whose binary64 joint vector is in W.q (written by lanes < D before the
+// call).  All 64 lanes of the block must call.  Returns the wave-uniform result.
+__device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int mode,
+                                          unsigned mask, unsigned& rechecks) {
+    const int lane = threadIdx.x;
+    if (mode < 0 || mode >= V.nm) mode = 0;
+    const int na = V.m_active[mode];
+    const int dof = V.dof;
+    const int Dact = na * dof;
+    // 1. joint sines / cosines (binary64), finiteness
+    bool bad = false;
+    if (lane < Dact) {
+        const double qv = W.q[lane];
+        bad = !__builtin_isfinite(qv);
+        SinCos t = sincos_f64(qv);
+        W.cs[lane] = t.c;
+        W.sn[lane] = t.s;
+    }
+    if (__any(bad)) return true;  // non-finite coordinates are reported as collisions
+    __syncthreads();
+    // 2. forward kinematics (binary64), one arm per lane; grasped boxes
+    if (lane < na) {
+        double X[12];
+        for (int k = 0; k < 12; ++k) X[k] = V.base[12 * lane + k];
+        for (int k = 0; k < dof; ++k) {
+            const int l = lane * dof + k;
+            const double* p = V.dh + 4 * l;
+            SinCos j;
+            j.c = W.cs[l];
+            j.s = W.sn[l];
+            dh_step_f64(X, p[0], p[1], p[2], p[3], j);
+            for (int t = 0; t < 12; ++t) W.X[12 * l + t] = X[t];
+        }
+        if (V.m_grasp[mode * V.na + lane]) {
+            double G[12];
+            compose_f64(X, V.m_rel + 12 * (mode * V.na + lane), G);
+            const double* h = V.m_half + 3 * (mode * V.na + lane);
+            for (int t = 0; t < 12; ++t) {
+                W.G64[12 * lane + t] = G[t];
+                W.G32[16 * lane + t] = (float)G[t];
+            }
+            for (int t = 0; t < 3; ++t) W.G32[16 * lane + 12 + t] = (float)h[t];
+        }
+    }
+    __syncthreads();
+    // 3. sphere centres: binary64, rounded to binary32
+    const int Sact = V.link_first[Dact];
+    for (int s = lane; s < Sact; s += 64) {
+        double c[3];
+        sphere_center_f64(V, W, s, c);
+        W.sph[s] = make_float4((float)c[0], (float)c[1], (float)c[2], V.sph_r32[s]);
+    }
+    __syncthreads();
+    const float M = V.Msph;
+    bool hit = false;
+    // 4. spheres vs boxes
+    for (int base = 0; base < Sact; base += 64) {
+        const int s = base + lane;
+        const bool act = s < Sact && ((mask >> (V.sph_link[s < Sact ? s : 0] / dof)) & 1u);
+        const float4 c = W.sph[act ? s : 0];
+        const float T = c.w * c.w;
+        for (int b = 0; b < V.nb; ++b) {
+            const float* bx = V.box32 + 8 * b;
+            const float e0 = fminf(fmaxf(c.x, bx[0]), bx[4]) - c.x;
+            const float e1 = fminf(fmaxf(c.y, bx[1]), bx[5]) - c.y;
+            const float e2 = fminf(fmaxf(c.z, bx[2]), bx[6]) - c.z;
+            const float d2 = fmaf(e2, e2, fmaf(e1, e1, e0 * e0));
+            if (act && d2 <= T + M) {
+                if (d2 < T - M) hit = true;
+                else {
+                    ++rechecks;
+                    if (arm_recheck_sph_box(V, W, s, b)) hit = true;
+                }
+            }
+        }
+        if (__any(hit)) return true;
+    }
+    // 5. sphere pairs: same arm links >= 2 apart, and different arms
+    for (int base = 0; base < Sact; base += 64) {
+        const int i = base + lane;
+        const bool act = i < Sact && ((mask >> (V.sph_link[i < Sact ? i : 0] / dof)) & 1u);
+        const float4 ci = W.sph[act ? i : 0];
+        const int lo = act ? V.sph_lo[i] : 0x7fffffff;
+        int jstart = V.sph_lo[base];
+        int cnt = 0;
+        for (int j = jstart; j < Sact; ++j) {
+            const int aj = V.sph_link[j] / dof;
+            if (!((mask >> aj) & 1u)) continue;  // wave-uniform
+            const float4 cj = W.sph[j];
+            const float dx = ci.x - cj.x, dy = ci.y - cj.y, dz = ci.z - cj.z;
+            const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            const float R = ci.w + cj.w;
+            const float T = R * R;
+            if (j >= lo && d2 <= T + M) {
+                if (d2 < T - M) hit = true;
+                else {
+                    ++rechecks;
+                    if (arm_recheck_sph_sph(V, W, i, j)) hit = true;
+                }
+            }
+            if (++cnt == 16) {
+                cnt = 0;
+                if (__any(hit)) return true;
+            }
+        }
+        if (__any(hit)) return true;
+    }
+    // 6. held boxes
+    unsigned gmask = 0;
+    for (int a = 0; a < na; ++a)
+        if (V.m_grasp[mode * V.na + a] && ((mask >> a) & 1u)) gmask |= 1u << a;
+    if (gmask) {
+        const float Mb = V.Mobb;
+        // 6a. held box vs obstacle boxes
+        for (int a = 0; a < na; ++a) {
+            if (!((gmask >> a) & 1u)) continue;
+            const float* G = W.G32 + 16 * a;
+            for (int b = lane; b < V.nb; b += 64) {
+                float A[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+                const float* bc = V.boxch32 + 6 * b;
+                A[9] = bc[0]; A[10] = bc[1]; A[11] = bc[2];
+                int r = obb3_overlap_f32(A, bc + 3, G, G + 12, Mb);
+                if (r == 1) hit = true;
+                else if (r == 2) {
+                    ++rechecks;
+                    if (arm_recheck_obb_box(V, W, a, b, mode)) hit = true;
+                }
+            }
+        }
+        if (__any(hit)) return true;
+        // 6b. held box vs spheres of the other arms (in the mask)
+        for (int a = 0; a < na; ++a) {
+            if (!((gmask >> a) & 1u)) continue;
+            const float* G = W.G32 + 16 * a;
+            for (int base = 0; base < Sact; base += 64) {
+                const int s = base + lane;
+                const int as = s < Sact ? V.sph_link[s] / dof : a;
+                const bool act = s < Sact && as != a && ((mask >> as) & 1u);
+                if (!act) continue;
+                const float4 c = W.sph[s];
+                const float w0 = c.x - G[9], w1 = c.y - G[10], w2 = c.z - G[11];
+                float d2 = 0.0f;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const float l = G[k] * w0 + G[3 + k] * w1 + G[6 + k] * w2;
+                    const float e = fminf(fmaxf(l, -G[12 + k]), G[12 + k]) - l;
+                    d2 = fmaf(e, e, d2);
+                }
+                const float T = c.w * c.w;
+                if (d2 <= T + Mb) {
+                    if (d2 < T - Mb) hit = true;
+                    else {
+                        ++rechecks;
+                        if (arm_recheck_sph_obb(V, W, s, a, mode)) hit = true;
+                    }
+                }
+            }
+        }
+        if (__any(hit)) return true;
+        // 6c. held box vs held box
+        for (int a = 0; a < na; ++a) {
+            if (!((gmask >> a) & 1u)) continue;
+            for (int b = a + 1 + lane; b < na; b += 64) {
+                if (!((gmask >> b) & 1u)) continue;
+                const float* Ga = W.G32 + 16 * a;
+                const float* Gb = W.G32 + 16 * b;
+                int r = obb3_overlap_f32(Ga, Ga + 12, Gb, Gb + 12, Mb);
+                if (r == 1) hit = true;
+                else if (r == 2) {
+                    ++rechecks;
+                    if (arm_recheck_obb_obb(V, W, a, b, mode)) hit = true;
+                }
+            }
+        }
+    }
+    return __any(hit);
+}

@@ -1,0 +1,624 @@
+// kernels.hip — kernels of the collision engine (gfx950, wave64).
+//
+// Edge batches run as one pass-ordered work queue (see PlanWork):
+//   prep      : one thread per edge: N (abstract_env.py:321-323), window (:325-331), slot count M,
+//               passes S = ceil(M / G), histogram of S
+//   scan      : one block: E_p = #{edges with S > p}, pass offsets PS[p] = G * sum_{p'<p} E_p'
+//   scatter   : counting sort, edges longest-first
+//   persistent: waves claim queue slots with an atomic ticket; slot -> (pass p, rank r, k);
+//               edge = order[r]; state = k * S_e + p of the edge (coarse-to-fine: pass 0 visits G
+//               evenly spread states of every edge, later passes refine; an edge found in
+//               collision is skipped by all later slots)
+// The same state evaluators serve the direct kernels (one workgroup per edge: the plain per-call
+// form and the unsorted single-launch ablation) and the configuration kernels.
+#include "engine.hpp"
+
+using namespace mrcc;
+
+__device__ inline void stat_add(unsigned long long* stats, int idx, unsigned long long v) {
+    if (stats && v) atomicAdd(stats + idx, v);
+}
+
+// row pointers of edge e
+__device__ inline const double* row1(const EdgeBatchView& B, long long e) {
+    long long r = B.idx1 ? (long long)B.idx1[e] : e;
+    return B.q1 + r * B.rs;
+}
+__device__ inline const double* row2(const EdgeBatchView& B, long long e) {
+    long long r = B.idx2 ? (long long)B.idx2[e] : e;
+    return B.q2 + r * B.rs;
+}
+
+__device__ inline int edge_N_dev(const EdgeBatchView& B, long long e, int D) {
+    if (B.N) return B.N[e];
+    const double* a = row1(B, e);
+    const double* b = row2(B, e);
+    double m = __builtin_fabs(a[0] - b[0]);
+    for (int k = 1; k < D; ++k) {
+        double v = __builtin_fabs(a[k * B.ds] - b[k * B.ds]);
+        m = v > m ? v : m;
+    }
+    return edge_N_from_max(m, B.res);
+}
+
+// interpolated state (abstract_env.py:341, :348): q1 + ((q2 - q1) / (N - 1)) * i, unfused
+__device__ inline double interp_f64(double a, double b, int N, int i) {
+#pragma clang fp contract(off)
+    double dir = (b - a) / (double)(N - 1);
+    return a + dir * (double)i;
+}
+__device__ inline double interp_dir_f64(double a, double dir, int i) {
+#pragma clang fp contract(off)
+    return a + dir * (double)i;
+}
+
+// ============================================================================================
+// queue construction
+__global__ void k_plan_init(PlanWork W) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * blockDim.x;
+    for (int b = t; b <= MRCC_MAX_PASSES; b += stride) W.hist[b] = 0;
+    if (t < 8) W.meta[t] = 0;
+    if (t == 0) W.ticket[0] = 0;
+    if (t < 4 && W.stats) W.stats[t] = 0;
+}
+
+__global__ void k_plan_prep(EdgeBatchView B, PlanWork W, int D, int lgG, int want_dir) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= B.n) return;
+    const int N = edge_N_dev(B, e, D);
+    const int ns = B.N_start ? B.N_start[e] : 0;
+    const int nm = B.N_max ? B.N_max[e] : -1;
+    EdgeWin w = edge_window(N, ns, nm, B.incl);
+    int M = w.M;
+    long long S = ((long long)M + (1 << lgG) - 1) >> lgG;
+    if (w.err) atomicAdd((unsigned long long*)&W.meta[2], 1ull);
+    if (S > MRCC_MAX_PASSES) {
+        unsigned long long slot = atomicAdd((unsigned long long*)&W.meta[3], 1ull);
+        W.long_list[slot] = (int)e;
+        S = 0;
+    }
+    W.eN[e] = N;
+    W.eM[e] = M;
+    W.eS[e] = (int)S;
+    W.ew0[e] = w.w0;
+    W.efull[e] = (unsigned char)w.full;
+    B.out[e] = w.err ? 0 : 1;
+    if (want_dir && W.dir) {
+        const double* a = row1(B, e);
+        const double* b = row2(B, e);
+        for (int k = 0; k < D; ++k) {
+#pragma clang fp contract(off)
+            W.dir[e * D + k] = (b[k * B.ds] - a[k * B.ds]) / (double)(N - 1);
+        }
+    }
+    atomicAdd(&W.hist[S], 1);
+}
+
+// one block of 1024 threads: suffix sums of the pass histogram and prefix sums of E_p
+__global__ void __launch_bounds__(1024) k_plan_scan(PlanWork W, int lgG) {
+    constexpr int NB = MRCC_MAX_PASSES;         // passes 0..NB-1; hist has NB+1 buckets
+    constexpr int PER = NB / 1024;
+    __shared__ long long part[1024];
+    const int t = threadIdx.x;
+    // E_p = sum_{b > p} hist[b]; thread t owns p in [t*PER, (t+1)*PER)
+    long long loc[PER];
+    long long s = 0;
+    for (int i = PER - 1; i >= 0; --i) {
+        const int p = t * PER + i;
+        s += W.hist[p + 1];
+        loc[i] = s;  // partial: sum_{b in (p, (t+1)*PER]}
+    }
+    part[t] = s;
+    __syncthreads();
+    // inclusive suffix scan of part
+    for (int off = 1; off < 1024; off <<= 1) {
+        long long v = (t + off < 1024) ? part[t + off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    const long long after = (t + 1 < 1024) ? part[t + 1] : 0;
+    long long E[PER];
+    long long esum = 0;
+    for (int i = 0; i < PER; ++i) {
+        E[i] = loc[i] + after;
+        esum += E[i];
+        W.cursor[t * PER + i] = (int)E[i];  // descending-order start of bucket p
+    }
+    __syncthreads();
+    // exclusive prefix scan of E over p -> PS
+    part[t] = esum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        long long v = (t >= off) ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    long long run = part[t] - esum;
+    int npass_local = 0;
+    for (int i = 0; i < PER; ++i) {
+        W.pass_start[t * PER + i] = run << lgG;
+        run += E[i];
+        if (E[i] > 0) npass_local = t * PER + i + 1;
+    }
+    if (t == 1023) {
+        W.pass_start[NB] = run << lgG;
+        W.cursor[NB] = 0;
+    }
+    // n_pass = max over threads
+    __shared__ int npass;
+    if (t == 0) npass = 0;
+    __syncthreads();
+    atomicMax(&npass, npass_local);
+    __syncthreads();
+    if (t == 0) {
+        W.meta[1] = npass;
+        W.meta[0] = W.pass_start[npass];
+    }
+}
+
+__global__ void k_plan_scatter(PlanWork W, long long n) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int S = W.eS[e];
+    if (S <= 0) return;
+    const int pos = atomicAdd(&W.cursor[S], 1);
+    W.order[pos] = (int)e;
+}
+
+struct SlotRef {
+    int e;      // edge, -1 when the slot is padding
+    int local;  // slot within the edge
+};
+
+// `p` is a pass index with pass_start[p] <= s (a hint carried forward: slots claimed by one wave
+// only increase, so the walk over all slots of a wave is at most n_pass steps)
+__device__ inline SlotRef slot_ref(const PlanWork& W, int& p, long long s, int lgG, long long n) {
+    SlotRef ref;
+    ref.e = -1;
+    ref.local = 0;
+    while (p < MRCC_MAX_PASSES && W.pass_start[p + 1] <= s) ++p;
+    const long long off = s - W.pass_start[p];
+    const long long r = off >> lgG;
+    const int k = (int)(off & ((1 << lgG) - 1));
+    if (off < 0 || r >= n) return ref;  // unreachable for a consistent queue
+    ref.e = W.order[r];
+    if (ref.e < 0 || ref.e >= n) {
+        ref.e = -1;
+        return ref;
+    }
+    const int S = W.eS[ref.e];
+    ref.local = k * S + p;
+    if (ref.local >= W.eM[ref.e]) ref.e = -1;
+    return ref;
+}
+
+__device__ inline EdgeWin win_of(const PlanWork& W, int e, int incl) {
+    EdgeWin w;
+    w.N = W.eN[e];
+    w.w0 = W.ew0[e];
+    w.M = W.eM[e];
+    w.full = W.efull[e];
+    w.first = incl ? 0 : 1;
+    w.err = 0;
+    return w;
+}
+
+// ============================================================================================
+// lane-per-state kinds
+template <class View, int T1, int T2>
+__device__ inline bool lane_collides(const View& S, const double* q, unsigned mask, unsigned& rc);
+
+// register length of the per-lane state: exact for the specialised shapes
+template <class View, int T1, int T2>
+struct LaneDim {
+    static constexpr int value = MRCC_MAX_D;
+};
+template <>
+struct LaneDim<DiskView, 2, 2> {
+    static constexpr int value = 4;
+};
+template <>
+struct LaneDim<PlanarView, 4, 0> {
+    static constexpr int value = 12;
+};
+template <>
+struct LaneDim<PlanarView, 0, 0> {
+    static constexpr int value = 3 * MRCC_PLANAR_MAX_ROBOTS;
+};
+template <>
+struct LaneDim<DiskView, 0, 0> {
+    static constexpr int value = MRCC_DISK_MAX_RD * 4;
+};
+
+template <>
+__device__ inline bool lane_collides<DiskView, 2, 2>(const DiskView& S, const double* q, unsigned mask, unsigned& rc) {
+    return disk_collides<2, 2>(S, q, mask, rc);
+}
+template <>
+__device__ inline bool lane_collides<DiskView, 0, 0>(const DiskView& S, const double* q, unsigned mask, unsigned& rc) {
+    return disk_collides<0, 0>(S, q, mask, rc);
+}
+template <>
+__device__ inline bool lane_collides<PlanarView, 4, 0>(const PlanarView& S, const double* q, unsigned mask, unsigned& rc) {
+    return planar_collides<4>(S, q, mask, rc);
+}
+template <>
+__device__ inline bool lane_collides<PlanarView, 0, 0>(const PlanarView& S, const double* q, unsigned mask, unsigned& rc) {
+    return planar_collides<0>(S, q, mask, rc);
+}
+
+template <class View, int T1, int T2>
+__global__ void __launch_bounds__(256) k_lane_persistent(View S, EdgeBatchView B, PlanWork W,
+                                                         int lgG, int spt) {
+    constexpr int MAXD = LaneDim<View, T1, T2>::value;
+    const int D = S.D;
+    const long long T = W.meta[0];
+    unsigned long long n_eval = 0, n_dead = 0, n_slots = 0;
+    unsigned rechecks = 0;
+    const int lane = threadIdx.x & 63;
+    int pw = 0;  // pass hint of this wave
+    for (;;) {
+        unsigned t = 0;
+        if (lane == 0) t = atomicAdd(W.ticket, 1u);
+        t = __builtin_amdgcn_readfirstlane(t);
+        const long long s0 = (long long)t * spt * 64;
+        if (s0 >= T) break;
+        while (pw < MRCC_MAX_PASSES && W.pass_start[pw + 1] <= s0) ++pw;
+        for (int j = 0; j < spt; ++j) {
+            const long long s = s0 + (long long)j * 64 + lane;
+            if (s >= T) break;
+            int p = pw;
+            SlotRef ref = slot_ref(W, p, s, lgG, B.n);
+            ++n_slots;
+            if (ref.e < 0) continue;
+            if (B.out[ref.e] == 0) { ++n_dead; continue; }
+            EdgeWin w = win_of(W, ref.e, B.incl);
+            const int i = edge_state_index(w, ref.local, B.incl);
+            if (i < 0) continue;
+            const double* a = row1(B, ref.e);
+            double q[MAXD];
+#pragma unroll
+            for (int k = 0; k < MAXD; ++k)
+                if (k < D) q[k] = interp_dir_f64(a[k * B.ds], W.dir[(long long)ref.e * D + k], i);
+            ++n_eval;
+            const int mode = B.modes ? B.modes[ref.e] : 0;
+            (void)mode;
+            if (lane_collides<View, T1, T2>(S, q, B.mask, rechecks)) B.out[ref.e] = 0;
+        }
+    }
+    if (W.stats) {
+        stat_add(W.stats, 0, n_eval);
+        stat_add(W.stats, 1, rechecks);
+        stat_add(W.stats, 2, n_dead);
+        stat_add(W.stats, 3, n_slots);
+    }
+}
+
+// direct form: block = 64 lanes, `wpe` blocks per edge (block b -> edge b / wpe); lane-per-state
+template <class View, int T1, int T2>
+__global__ void __launch_bounds__(64) k_lane_direct(View S, EdgeBatchView B, const int* list,
+                                                    long long n_list, const long long* n_dev,
+                                                    int wpe, int init_out,
+                                                    unsigned long long* stats) {
+    constexpr int MAXD = LaneDim<View, T1, T2>::value;
+    const int D = S.D;
+    const int lane = threadIdx.x;
+    unsigned long long n_eval = 0;
+    unsigned rechecks = 0;
+    if (n_dev) n_list = *n_dev;
+    const long long nblocks = n_list * wpe;
+    for (long long blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
+        const long long li = blk / wpe;
+        const int u = (int)(blk % wpe);
+        const long long e = list ? (long long)list[li] : li;
+        if (init_out && lane == 0) B.out[e] = 1;  // only with wpe == 1
+        const int N = edge_N_dev(B, e, D);
+        const int ns = B.N_start ? B.N_start[e] : 0;
+        const int nm = B.N_max ? B.N_max[e] : -1;
+        EdgeWin w = edge_window(N, ns, nm, B.incl);
+        if (w.err) { B.out[e] = 0; continue; }
+        const double* a = row1(B, e);
+        const double* b = row2(B, e);
+        for (int base = u * 64; base < w.M; base += wpe * 64) {
+            if (B.out[e] == 0) break;  // wave-uniform: same address
+            const int local = base + lane;
+            bool coll = false;
+            if (local < w.M) {
+                const int i = edge_state_index(w, local, B.incl);
+                if (i >= 0) {
+                    double q[MAXD];
+#pragma unroll
+                    for (int k = 0; k < MAXD; ++k)
+                        if (k < D) q[k] = interp_f64(a[k * B.ds], b[k * B.ds], N, i);
+                    ++n_eval;
+                    coll = lane_collides<View, T1, T2>(S, q, B.mask, rechecks);
+                }
+            }
+            if (__any(coll)) {
+                if (lane == 0) B.out[e] = 0;
+                break;
+            }
+        }
+    }
+    if (stats) {
+        stat_add(stats, 0, n_eval);
+        stat_add(stats, 1, rechecks);
+    }
+}
+
+template <class View, int T1, int T2>
+__global__ void __launch_bounds__(256) k_lane_configs(View S, const double* qv, long long n,
+                                                      long long rs, long long ds, unsigned mask,
+                                                      unsigned char* out) {
+    constexpr int MAXD = LaneDim<View, T1, T2>::value;
+    const int D = S.D;
+    unsigned rechecks = 0;
+    for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < n;
+         r += (long long)gridDim.x * blockDim.x) {
+        double q[MAXD];
+#pragma unroll
+        for (int k = 0; k < MAXD; ++k)
+            if (k < D) q[k] = qv[r * rs + k * ds];
+        out[r] = lane_collides<View, T1, T2>(S, q, mask, rechecks) ? 0 : 1;
+    }
+}
+
+// ============================================================================================
+// arms: one state per block of 64 lanes
+__device__ inline void arm_load_state(const ArmView& V, const ArmLds& Wl, const EdgeBatchView& B,
+                                      long long e, int N, int i, int Dact) {
+    const int lane = threadIdx.x;
+    if (lane < Dact) {
+        const double* a = row1(B, e);
+        const double* b = row2(B, e);
+        Wl.q[lane] = interp_f64(a[lane * B.ds], b[lane * B.ds], N, i);
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(64) k_arm_persistent(ArmView V, EdgeBatchView B, PlanWork W,
+                                                       int lgG, int spt) {
+    extern __shared__ double lds_arm[];
+    ArmLds Wl = arm_lds(lds_arm, V);
+    const long long T = W.meta[0];
+    unsigned long long n_eval = 0, n_dead = 0, n_slots = 0;
+    unsigned rechecks = 0;
+    int p = 0;  // pass hint
+    for (;;) {
+        unsigned t = 0;
+        if (threadIdx.x == 0) t = atomicAdd(W.ticket, 1u);
+        t = __builtin_amdgcn_readfirstlane(t);
+        const long long s0 = (long long)t * spt;
+        if (s0 >= T) break;
+        for (int j = 0; j < spt; ++j) {
+            const long long s = s0 + j;
+            if (s >= T) break;
+            ++n_slots;
+            SlotRef ref = slot_ref(W, p, s, lgG, B.n);
+            if (ref.e < 0) continue;
+            if (B.out[ref.e] == 0) { ++n_dead; continue; }
+            EdgeWin w = win_of(W, ref.e, B.incl);
+            const int i = edge_state_index(w, ref.local, B.incl);
+            if (i < 0) continue;
+            const int mode = B.modes ? B.modes[ref.e] : 0;
+            const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
+            arm_load_state(V, Wl, B, ref.e, w.N, i, V.m_active[md] * V.dof);
+            ++n_eval;
+            if (arm_state_collides(V, Wl, mode, B.mask, rechecks)) {
+                if (threadIdx.x == 0) B.out[ref.e] = 0;
+            }
+            __syncthreads();
+        }
+    }
+    if (W.stats && threadIdx.x == 0) {
+        stat_add(W.stats, 0, n_eval);
+        stat_add(W.stats, 2, n_dead);
+        stat_add(W.stats, 3, n_slots);
+    }
+    if (W.stats) stat_add(W.stats, 1, rechecks);
+}
+
+__global__ void __launch_bounds__(64) k_arm_direct(ArmView V, EdgeBatchView B, const int* list,
+                                                   long long n_list, const long long* n_dev,
+                                                   int wpe, int init_out,
+                                                   unsigned long long* stats) {
+    extern __shared__ double lds_arm[];
+    ArmLds Wl = arm_lds(lds_arm, V);
+    unsigned long long n_eval = 0;
+    unsigned rechecks = 0;
+    if (n_dev) n_list = *n_dev;
+    const long long nblocks = n_list * wpe;
+    for (long long blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
+        const long long li = blk / wpe;
+        const int u = (int)(blk % wpe);
+        const long long e = list ? (long long)list[li] : li;
+        if (init_out && threadIdx.x == 0) B.out[e] = 1;  // only with wpe == 1
+        __syncthreads();
+        const int mode = B.modes ? B.modes[e] : 0;
+        const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
+        const int N = edge_N_dev(B, e, V.D);
+        const int ns = B.N_start ? B.N_start[e] : 0;
+        const int nm = B.N_max ? B.N_max[e] : -1;
+        EdgeWin w = edge_window(N, ns, nm, B.incl);
+        if (w.err) {
+            if (threadIdx.x == 0) B.out[e] = 0;
+            continue;
+        }
+        for (int local = u; local < w.M; local += wpe) {
+            if (B.out[e] == 0) break;
+            const int i = edge_state_index(w, local, B.incl);
+            if (i < 0) continue;
+            arm_load_state(V, Wl, B, e, N, i, V.m_active[md] * V.dof);
+            ++n_eval;
+            const bool c = arm_state_collides(V, Wl, mode, B.mask, rechecks);
+            __syncthreads();
+            if (c) {
+                if (threadIdx.x == 0) B.out[e] = 0;
+                break;
+            }
+        }
+        __syncthreads();
+    }
+    if (stats && threadIdx.x == 0) stat_add(stats, 0, n_eval);
+    if (stats) stat_add(stats, 1, rechecks);
+}
+
+__global__ void __launch_bounds__(64) k_arm_configs(ArmView V, const double* qv, long long n,
+                                                    long long rs, long long ds, const int* modes,
+                                                    unsigned mask, unsigned char* out) {
+    extern __shared__ double lds_arm[];
+    ArmLds Wl = arm_lds(lds_arm, V);
+    unsigned rechecks = 0;
+    for (long long r = blockIdx.x; r < n; r += gridDim.x) {
+        const int mode = modes ? modes[r] : 0;
+        const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
+        const int Dact = V.m_active[md] * V.dof;
+        if ((int)threadIdx.x < Dact) Wl.q[threadIdx.x] = qv[r * rs + (long long)threadIdx.x * ds];
+        __syncthreads();
+        const bool c = arm_state_collides(V, Wl, mode, mask, rechecks);
+        if (threadIdx.x == 0) out[r] = c ? 0 : 1;
+        __syncthreads();
+    }
+}
+
+// binary64 forward kinematics of every link, out[r][L][12]
+__global__ void k_arm_fk(ArmView V, const double* qv, long long n, long long rs, long long ds,
+                         double* out) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long r = t / V.na;
+    const int a = (int)(t % V.na);
+    if (r >= n) return;
+    double X[12];
+    for (int k = 0; k < 12; ++k) X[k] = V.base[12 * a + k];
+    for (int k = 0; k < V.dof; ++k) {
+        const int l = a * V.dof + k;
+        const double* p = V.dh + 4 * l;
+        SinCos j = sincos_f64(qv[r * rs + (long long)l * ds]);
+        dh_step_f64(X, p[0], p[1], p[2], p[3], j);
+        for (int c = 0; c < 12; ++c) out[(r * V.L + l) * 12 + c] = X[c];
+    }
+}
+
+__global__ void k_sincos(const double* x, long long n, double* s, double* c) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    SinCos v = sincos_f64(x[t]);
+    s[t] = v.s;
+    c[t] = v.c;
+}
+
+// ============================================================================================
+// launchers
+static int grid_for(long long n, int block) {
+    long long g = (n + block - 1) / block;
+    if (g > (1 << 30)) g = 1 << 30;
+    return (int)(g < 1 ? 1 : g);
+}
+
+void launch_plan_build(const SceneRec& sc, const EdgeBatchView& B, const PlanWork& W, int lgG,
+                       hipStream_t st) {
+    const int want_dir = sc.kind != KIND_ARMS;
+    k_plan_init<<<64, 256, 0, st>>>(W);
+    k_plan_prep<<<grid_for(B.n, 256), 256, 0, st>>>(B, W, sc.D, lgG, want_dir);
+    k_plan_scan<<<1, 1024, 0, st>>>(W, lgG);
+    k_plan_scatter<<<grid_for(B.n, 256), 256, 0, st>>>(W, B.n);
+}
+
+void launch_plan_run(const SceneRec& sc, const EdgeBatchView& B, const PlanWork& W, int lgG,
+                     int spt, int grid, hipStream_t st) {
+    switch (sc.kind) {
+        case KIND_DISKS:
+            if (sc.disk.rd == 2 && sc.disk.nr == 2)
+                k_lane_persistent<DiskView, 2, 2><<<grid, 256, 0, st>>>(sc.disk, B, W, lgG, spt);
+            else
+                k_lane_persistent<DiskView, 0, 0><<<grid, 256, 0, st>>>(sc.disk, B, W, lgG, spt);
+            break;
+        case KIND_PLANAR:
+            if (sc.planar.nr == 4)
+                k_lane_persistent<PlanarView, 4, 0><<<grid, 256, 0, st>>>(sc.planar, B, W, lgG, spt);
+            else
+                k_lane_persistent<PlanarView, 0, 0><<<grid, 256, 0, st>>>(sc.planar, B, W, lgG, spt);
+            break;
+        case KIND_ARMS:
+            k_arm_persistent<<<grid, 64, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
+            break;
+    }
+}
+
+__global__ void k_fill_u8(unsigned char* p, long long n, unsigned char v) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = v;
+}
+
+void launch_fill_u8(unsigned char* p, long long n, unsigned char v, hipStream_t st) {
+    k_fill_u8<<<grid_for(n, 256), 256, 0, st>>>(p, n, v);
+}
+
+void launch_direct(const SceneRec& sc, const EdgeBatchView& B, const int* list, long long n_list,
+                   const long long* n_dev, int wpe, int init_out, int max_grid,
+                   unsigned long long* stats, hipStream_t st) {
+    long long blocks = n_dev ? (long long)max_grid : n_list * wpe;
+    int grid = (int)(blocks < max_grid ? blocks : max_grid);
+    if (grid < 1) grid = 1;
+    switch (sc.kind) {
+        case KIND_DISKS:
+            if (sc.disk.rd == 2 && sc.disk.nr == 2)
+                k_lane_direct<DiskView, 2, 2><<<grid, 64, 0, st>>>(sc.disk, B, list, n_list, n_dev, wpe, init_out, stats);
+            else
+                k_lane_direct<DiskView, 0, 0><<<grid, 64, 0, st>>>(sc.disk, B, list, n_list, n_dev, wpe, init_out, stats);
+            break;
+        case KIND_PLANAR:
+            if (sc.planar.nr == 4)
+                k_lane_direct<PlanarView, 4, 0><<<grid, 64, 0, st>>>(sc.planar, B, list, n_list, n_dev, wpe, init_out, stats);
+            else
+                k_lane_direct<PlanarView, 0, 0><<<grid, 64, 0, st>>>(sc.planar, B, list, n_list, n_dev, wpe, init_out, stats);
+            break;
+        case KIND_ARMS:
+            k_arm_direct<<<grid, 64, sc.arm_lds, st>>>(sc.arm, B, list, n_list, n_dev, wpe, init_out, stats);
+            break;
+    }
+}
+
+void launch_configs(const SceneRec& sc, const double* q, long long n, long long rs, long long ds,
+                    const int* modes, unsigned mask, unsigned char* out, int max_grid,
+                    hipStream_t st) {
+    switch (sc.kind) {
+        case KIND_DISKS: {
+            int g = grid_for(n, 256);
+            if (g > max_grid) g = max_grid;
+            if (sc.disk.rd == 2 && sc.disk.nr == 2)
+                k_lane_configs<DiskView, 2, 2><<<g, 256, 0, st>>>(sc.disk, q, n, rs, ds, mask, out);
+            else
+                k_lane_configs<DiskView, 0, 0><<<g, 256, 0, st>>>(sc.disk, q, n, rs, ds, mask, out);
+            break;
+        }
+        case KIND_PLANAR: {
+            int g = grid_for(n, 256);
+            if (g > max_grid) g = max_grid;
+            if (sc.planar.nr == 4)
+                k_lane_configs<PlanarView, 4, 0><<<g, 256, 0, st>>>(sc.planar, q, n, rs, ds, mask, out);
+            else
+                k_lane_configs<PlanarView, 0, 0><<<g, 256, 0, st>>>(sc.planar, q, n, rs, ds, mask, out);
+            break;
+        }
+        case KIND_ARMS: {
+            long long g = n < (long long)max_grid * 4 ? n : (long long)max_grid * 4;
+            if (g < 1) g = 1;
+            k_arm_configs<<<(int)g, 64, sc.arm_lds, st>>>(sc.arm, q, n, rs, ds, modes, mask, out);
+            break;
+        }
+    }
+}
+
+void launch_arm_fk(const SceneRec& sc, const double* q, long long n, long long rs, long long ds,
+                   double* out, hipStream_t st) {
+    long long t = n * sc.arm.na;
+    k_arm_fk<<<grid_for(t, 128), 128, 0, st>>>(sc.arm, q, n, rs, ds, out);
+}
+
+void launch_sincos(const double* x, long long n, double* s, double* c, hipStream_t st) {
+    k_sincos<<<grid_for(n, 256), 256, 0, st>>>(x, n, s, c);
+}

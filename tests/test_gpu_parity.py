@@ -1,0 +1,398 @@
+"""GPU parity of the engine against the CPU oracle (bit-exact booleans, bit-exact FP64 kinematics).
+
+Every case prints its name before running so a failing GPU step names itself.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+from helpers import O, backend, envs, gpu_configs, gpu_edges, near_home, pkg, scenes
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def B():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return backend()
+
+
+# ---------------------------------------------------------------------------------------------
+# binary64 building blocks
+def test_sincos_bit_exact(B):
+    _say("sincos bit-exact vs oracle")
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.uniform(-10, 10, 20000), rng.uniform(-3.2, 3.2, 20000),
+                        np.array([0.0, -0.0, math.pi / 4, math.pi / 2, math.pi, 2.9, -2.9, 1e-300])])
+    xd = torch.from_numpy(x).cuda()
+    s = torch.empty_like(xd)
+    c = torch.empty_like(xd)
+    pkg._lib.check(pkg.lib().mrcc_sincos(ctypes.c_void_p(xd.data_ptr()), len(x), ctypes.c_void_p(s.data_ptr()),
+                                         ctypes.c_void_p(c.data_ptr()), None))
+    torch.cuda.synchronize()
+    ref = np.array([O.sincos(v) for v in x])
+    assert np.array_equal(s.cpu().numpy(), ref[:, 0])
+    assert np.array_equal(c.cpu().numpy(), ref[:, 1])
+
+
+def test_arm_fk_bit_exact(B):
+    _say("arm FK bit-exact vs oracle (C3 scene, 2000 configs)")
+    rng = np.random.default_rng(1)
+    desc = scenes.quad_arms(rng=rng)
+    sc = B.Scene(desc)
+    o = O.Scene(desc)
+    Q = scenes.uniform_configs(rng, desc, 2000)
+    qd = torch.from_numpy(Q).cuda()
+    out = torch.empty((2000, desc.dim, 12), dtype=torch.float64, device="cuda")
+    pkg._lib.check(pkg.lib().mrcc_arm_fk(sc.handle, ctypes.c_void_p(qd.data_ptr()), 2000, desc.dim, 1,
+                                         ctypes.c_void_p(out.data_ptr()), None))
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    for r in range(0, 2000, 97):
+        assert np.array_equal(got[r], o.arm_fk(Q[r]).reshape(desc.dim, 12)), r
+
+
+# ---------------------------------------------------------------------------------------------
+# C1: the reference's abstract environment
+@pytest.fixture(scope="module")
+def c1(B):
+    rng = np.random.default_rng(2)
+    desc = scenes.planar_disks(rng=rng)
+    return desc, B.Scene(desc), O.Scene(desc), rng
+
+
+def test_c1_configs_full_size(c1):
+    _say("C1 10,000 configs vs oracle")
+    desc, sc, o, rng = c1
+    Q = scenes.uniform_configs(rng, desc, 10000)
+    assert np.array_equal(gpu_configs(sc, Q), o.check_configs(Q))
+
+
+@pytest.mark.parametrize("form", ["persistent", "graph", "single_launch", "per_call"])
+def test_c1_edges_full_size(c1, form, B):
+    _say(f"C1 10,000 edges res 0.01 vs oracle ({form})")
+    desc, sc, o, _ = c1
+    rng = np.random.default_rng(3)
+    q1, q2 = scenes.random_pair_edges(rng, desc, 10000)
+    ref = o.check_edges(q1, q2, 0.01)
+    if form == "per_call":
+        got = gpu_edges(sc, q1, q2, 0.01, per_call=True)
+    elif form == "single_launch":
+        got = gpu_edges(sc, q1, q2, 0.01, tune=B.make_tune(persistent=0, sorted=0, use_graph=0))
+    elif form == "graph":
+        got = gpu_edges(sc, q1, q2, 0.01, tune=B.make_tune(use_graph=1))
+    else:
+        got = gpu_edges(sc, q1, q2, 0.01)
+    assert np.array_equal(got, ref)
+
+
+def test_c1_edges_free_endpoints(c1):
+    _say("C1 edges between free configurations (reference harness variant)")
+    desc, sc, o, _ = c1
+    rng = np.random.default_rng(4)
+    Q = scenes.uniform_configs(rng, desc, 40000)
+    F = Q[o.check_configs(Q)]
+    q1, q2 = F[: len(F) // 2], F[len(F) // 2: 2 * (len(F) // 2)]
+    assert np.array_equal(gpu_edges(sc, q1, q2, 0.01), o.check_edges(q1, q2, 0.01))
+
+
+@pytest.mark.parametrize("incl", [False, True])
+def test_c1_windows(c1, incl):
+    _say(f"C1 windowed edges (N_start / N_max / explicit N), include_endpoints={incl}")
+    desc, sc, o, _ = c1
+    rng = np.random.default_rng(5 + incl)
+    n = 6000
+    q1, q2 = scenes.random_pair_edges(rng, desc, n)
+    q2 = q1 + (q2 - q1) * rng.uniform(0, 1, size=(n, 1)) ** 2
+    N = np.maximum(2, (np.max(np.abs(q2 - q1), axis=1) / 0.01).astype(np.int64) + 1)
+    N = np.where(rng.uniform(size=n) < 0.3, N + rng.integers(0, 5, n), N).astype(np.int32)
+    S = (rng.uniform(size=n) * (N + 1)).astype(np.int32)
+    S = np.where(rng.uniform(size=n) < 0.3, 0, S)
+    M = np.where(rng.uniform(size=n) < 0.2, -1, S + rng.integers(0, 40, n)).astype(np.int32)
+    ref = o.check_edges(q1, q2, 0.01, N=N, N_start=S, N_max=M, include_endpoints=incl)
+    assert np.array_equal(gpu_edges(sc, q1, q2, 0.01, N=N, N_start=S, N_max=M, incl=incl), ref)
+
+
+def test_c1_near_contact(c1):
+    _say("C1 near-contact states (FP64 recheck path)")
+    desc, sc, o, _ = c1
+    rng = np.random.default_rng(7)
+    n = 20000
+    Q = scenes.uniform_configs(rng, desc, n)
+    phi = rng.uniform(-math.pi, math.pi, n)
+    R = 0.2 + rng.choice([-1e-7, -1e-12, -1e-15, 0.0, 1e-15, 1e-12, 1e-7], n)
+    Q[:, 2] = Q[:, 0] + R * np.cos(phi)
+    Q[:, 3] = Q[:, 1] + R * np.sin(phi)
+    # robots touching rectangle faces
+    k = n // 2
+    rc, hs = desc.rect_center, desc.rect_size / 2
+    j = rng.integers(0, len(rc), k)
+    Q[:k, 0] = rc[j, 0] + hs[j, 0] + 0.1 + rng.choice([-1e-13, 0.0, 1e-13], k)
+    Q[:k, 1] = rc[j, 1] + rng.uniform(-1, 1, k) * hs[j, 1]
+    assert np.array_equal(gpu_configs(sc, Q), o.check_configs(Q))
+
+
+def test_c1_non_finite(c1):
+    _say("C1 NaN / inf coordinates follow numpy semantics")
+    desc, sc, o, _ = c1
+    Q = np.array([[np.nan, 0, 1, 1], [np.inf, 0, 1, 1], [0.5, 0.5, np.nan, np.nan], [-np.inf, np.inf, 0, 0]])
+    assert np.array_equal(gpu_configs(sc, Q), o.check_configs(Q))
+
+
+def test_c1_masks(c1):
+    _say("C1 per-robot masks")
+    desc, sc, o, rng = c1
+    Q = scenes.uniform_configs(rng, desc, 5000)
+    for mask in (1, 2, 3):
+        assert np.array_equal(gpu_configs(sc, Q, mask=mask), o.check_configs(Q, mask=mask)), mask
+
+
+# ---------------------------------------------------------------------------------------------
+# abstract environments through the BaseProblem API (per-call host path)
+def test_abstract_test_kat(B):
+    _say("abstract.test KAT edge + per-call calls")
+    E = envs()
+    env = pkg.get_env_by_name("abstract.test")
+    q1 = env.start_pos.from_flat(np.array([-1, 0, 1, 1.0]))
+    q2 = env.start_pos.from_flat(np.array([-1, 1, 1, 0.0]))
+    q3 = env.start_pos.from_flat(np.array([-1, 2, 1, -1.0]))
+    assert env.is_edge_collision_free(q1, q2, env.start_mode)
+    s = [pkg.State(q, env.start_mode) for q in (q1, q2, q3)]
+    assert env.is_path_collision_free(s, resolution=0.5)
+    o = O.Scene(env.desc)
+    rng = np.random.default_rng(8)
+    for _ in range(200):
+        a, b = rng.uniform(-2, 2, 4), rng.uniform(-2, 2, 4)
+        res = float(rng.choice([0.01, 0.05, 0.3]))
+        qa, qb = env.start_pos.from_flat(a), env.start_pos.from_flat(b)
+        assert env.is_collision_free(qa, None) == o.config_free(a)
+        assert env.is_edge_collision_free(qa, qb, None, resolution=res) == o.edge_free(a, b, res)
+        N = max(2, int(np.max(np.abs(a - b)) / res) + 1)
+        ns = int(rng.integers(0, N + 1))
+        nm = int(ns + rng.integers(0, 12))
+        assert env.is_edge_collision_free(qa, qb, None, resolution=res, N_start=ns, N_max=nm, N=N) == \
+            o.edge_free(a, b, res, N=N, N_start=ns, N_max=nm)
+        assert env.is_collision_free_for_robot("a1", a, None) == o.config_free(a, mask=1)
+
+
+def test_path_schedules_match_sequential(B):
+    _say("batched is_path_collision_free == reference sequential schedules")
+    env = pkg.get_env_by_name("abstract.center_rect_2d")
+    base_cls = pkg.BaseProblem
+    rng = np.random.default_rng(9)
+    for trial in range(60):
+        K = int(rng.integers(2, 7))
+        P = [env.start_pos.from_flat(rng.uniform(-2, 2, 4)) for _ in range(K)]
+        path = [pkg.State(q, env.start_mode) for q in P]
+        for in_order in (False, True):
+            for se in (False, True):
+                got = env.is_path_collision_free(path, check_edges_in_order=in_order, check_start_and_end=se)
+                ref = base_cls.is_path_collision_free(env, path, check_edges_in_order=in_order,
+                                                      check_start_and_end=se)
+                assert got == ref, (trial, in_order, se)
+
+
+# ---------------------------------------------------------------------------------------------
+# C2: planar oriented boxes
+@pytest.fixture(scope="module")
+def c2(B):
+    rng = np.random.default_rng(10)
+    desc = scenes.planar_boxes(rng=rng)
+    return desc, B.Scene(desc), O.Scene(desc), rng
+
+
+def test_c2_configs(c2):
+    _say("C2 configs (uniform + sparse scene variants) vs oracle")
+    desc, sc, o, rng = c2
+    Q = scenes.uniform_configs(rng, desc, 100000)
+    assert np.array_equal(gpu_configs(sc, Q), o.check_configs(Q))
+
+
+def test_c2_edges(c2):
+    _say("C2 lognormal edges vs oracle")
+    desc, sc, o, rng = c2
+    q1, q2 = scenes.lognormal_edges(rng, desc, 20000, 0.01)
+    assert np.array_equal(gpu_edges(sc, q1, q2, 0.01), o.check_edges(q1, q2, 0.01))
+
+
+def test_c2_sparse_scene_and_masks(B):
+    _say("C2 sparse scene (5 obstacles): free edges, masks")
+    rng = np.random.default_rng(11)
+    desc = scenes.planar_boxes(rng=rng, n_obstacles=5)
+    sc, o = B.Scene(desc), O.Scene(desc)
+    Q = scenes.uniform_configs(rng, desc, 50000)
+    for mask in (0xFFFFFFFF, 1, 0b0110):
+        assert np.array_equal(gpu_configs(sc, Q, mask=mask), o.check_configs(Q, mask=mask))
+    q1, q2 = scenes.lognormal_edges(rng, desc, 10000, 0.01)
+    ref = o.check_edges(q1, q2, 0.01)
+    assert ref.mean() > 0.05
+    assert np.array_equal(gpu_edges(sc, q1, q2, 0.01), ref)
+
+
+# ---------------------------------------------------------------------------------------------
+# C3: 4 x 7-DoF arms
+@pytest.fixture(scope="module")
+def c3(B):
+    rng = np.random.default_rng(12)
+    desc = scenes.quad_arms(rng=rng)
+    return desc, B.Scene(desc), O.Scene(desc), rng
+
+
+def test_c3_configs(c3):
+    _say("C3 configs: uniform + near-home vs oracle")
+    desc, sc, o, rng = c3
+    Q = np.concatenate([scenes.uniform_configs(rng, desc, 4000), near_home(rng, desc, 8000)])
+    ref = o.check_configs(Q)
+    assert 0.05 < ref.mean() < 0.95
+    assert np.array_equal(gpu_configs(sc, Q), ref)
+
+
+def test_c3_edges(c3, B):
+    _say("C3 edges near home + lognormal uniform vs oracle (persistent, single launch, per call)")
+    desc, sc, o, rng = c3
+    a = near_home(rng, desc, 3000)
+    b = np.clip(a + rng.normal(0, 0.3, a.shape), desc.limits[0], desc.limits[1])
+    u1, u2 = scenes.lognormal_edges(rng, desc, 1000, 0.05)
+    q1, q2 = np.concatenate([a, u1]), np.concatenate([b, u2])
+    ref = o.check_edges(q1, q2, 0.05)
+    assert np.array_equal(gpu_edges(sc, q1, q2, 0.05), ref)
+    assert np.array_equal(gpu_edges(sc, q1, q2, 0.05, tune=B.make_tune(persistent=0, sorted=0, use_graph=0)), ref)
+    assert np.array_equal(gpu_edges(sc, q1[:300], q2[:300], 0.05, per_call=True), ref[:300])
+
+
+def test_c3_masks(c3):
+    _say("C3 per-arm masks")
+    desc, sc, o, rng = c3
+    Q = near_home(rng, desc, 3000, sigma=0.8)
+    for mask in (1, 0b0101, 0b1110):
+        assert np.array_equal(gpu_configs(sc, Q, mask=mask), o.check_configs(Q, mask=mask)), mask
+
+
+def test_c3_near_contact(c3):
+    _say("C3 near-contact: configs bisected onto the contact boundary")
+    desc, sc, o, rng = c3
+    A = near_home(rng, desc, 400)
+    Bq = near_home(rng, desc, 400, sigma=1.2)
+    fa, fb = o.check_configs(A), o.check_configs(Bq)
+    pairs = [(a, b) for a, b, x, y in zip(A, Bq, fa, fb) if x != y][:120]
+    pts = []
+    for a, b in pairs:
+        lo, hi = 0.0, 1.0  # free at lo side iff o.config_free(a)
+        fa_ = o.config_free(a)
+        for _ in range(48):
+            m = 0.5 * (lo + hi)
+            if o.config_free(a + m * (b - a)) == fa_:
+                lo = m
+            else:
+                hi = m
+        for t in (lo, hi):
+            pts.append(a + t * (b - a))
+    P = np.array(pts)
+    assert np.array_equal(gpu_configs(sc, P), o.check_configs(P))
+
+
+# ---------------------------------------------------------------------------------------------
+# C5: mixed arm counts and held boxes
+def test_c5_modes(B):
+    _say("C5 mixed modes: configs and edges with mode ids vs oracle")
+    rng = np.random.default_rng(13)
+    desc = scenes.mixed_arms(rng=rng, n_modes=256)
+    sc, o = B.Scene(desc), O.Scene(desc)
+    n = 6000
+    Q = near_home(rng, desc, n, sigma=0.5)
+    modes = rng.integers(0, 256, n).astype(np.int32)
+    ref = o.check_configs(Q, modes=modes)
+    assert 0.02 < ref.mean() < 0.98
+    assert np.array_equal(gpu_configs(sc, Q, modes=modes), ref)
+    for mask in (0b1, 0b1010_0101):
+        assert np.array_equal(gpu_configs(sc, Q[:2000], modes=modes[:2000], mask=mask),
+                              o.check_configs(Q[:2000], modes=modes[:2000], mask=mask))
+    q2 = np.clip(Q + rng.normal(0, 0.2, Q.shape), desc.limits[0], desc.limits[1])
+    refe = o.check_edges(Q[:3000], q2[:3000], 0.05, modes=modes[:3000])
+    assert np.array_equal(gpu_edges(sc, Q[:3000], q2[:3000], 0.05, modes=modes[:3000]), refe)
+
+
+# ---------------------------------------------------------------------------------------------
+# plans, graphs, errors
+def test_graph_replay_stable(c1, B):
+    _say("graph-captured plan replayed 4x with eager work in between")
+    desc, sc, o, _ = c1
+    rng = np.random.default_rng(14)
+    q1, q2 = scenes.random_pair_edges(rng, desc, 5000)
+    ref = o.check_edges(q1, q2, 0.01)
+    a, b = torch.from_numpy(q1).cuda(), torch.from_numpy(q2).cuda()
+    out = torch.empty(5000, dtype=torch.uint8, device="cuda")
+    plan = B.EdgePlan(sc, sc.edge_batch(a, b, out, 0.01), B.make_tune(use_graph=1, collect_stats=1))
+    for k in range(4):
+        out.fill_(9)
+        plan.run()
+        torch.cuda.synchronize()
+        _ = (a * 2).sum().item()
+        assert np.array_equal(out.cpu().numpy().astype(bool), ref), k
+    st = plan.stats()
+    assert st["states"] > 0
+
+
+def test_empty_and_errors(c1, B):
+    _say("empty batches, N_start > N error, injected failure")
+    desc, sc, o, _ = c1
+    assert gpu_edges(sc, np.zeros((0, 4)), np.zeros((0, 4)), 0.01).shape == (0,)
+    q1 = np.zeros((2, 4))
+    q2 = np.ones((2, 4))
+    with pytest.raises(pkg.MrccError):
+        gpu_edges(sc, q1, q2, 0.01, N_start=np.array([5, 1000], dtype=np.int32))
+    L = pkg.lib()
+    L.mrcc_debug_inject_failure(-5, 1)
+    with pytest.raises(pkg.MrccError):
+        gpu_edges(sc, q1, q2, 0.01)
+    assert gpu_edges(sc, q1, q2, 0.01).shape == (2,)
+
+
+# ---------------------------------------------------------------------------------------------
+# differentiable clearance (C4)
+def test_clearance_cost_and_grad(B):
+    _say("clearance cost vs FP64 oracle; gradient vs FP64 central FD")
+    rng = np.random.default_rng(15)
+    desc = scenes.quad_arms(rng=rng)
+    sc, o = B.Scene(desc), O.Scene(desc)
+    Q = near_home(rng, desc, 64, sigma=0.6)
+    qd = torch.from_numpy(Q.astype(np.float32)).cuda()
+    cost = torch.empty(64, dtype=torch.float32, device="cuda")
+    L = pkg.lib()
+    pkg._lib.check(L.mrcc_clearance_forward(sc.handle, ctypes.c_void_p(qd.data_ptr()), 64, ctypes.c_float(0.02),
+                                            ctypes.c_void_p(cost.data_ptr()), None))
+    g_out = torch.ones(64, dtype=torch.float32, device="cuda")
+    grad = torch.empty_like(qd)
+    pkg._lib.check(L.mrcc_clearance_backward(sc.handle, ctypes.c_void_p(qd.data_ptr()), 64, ctypes.c_float(0.02),
+                                             ctypes.c_void_p(g_out.data_ptr()), ctypes.c_void_p(grad.data_ptr()),
+                                             None))
+    torch.cuda.synchronize()
+    c = cost.cpu().numpy()
+    g = grad.cpu().numpy()
+    Q32 = Q.astype(np.float32).astype(np.float64)
+    ref_c = np.array([o.clearance_cost(q, 0.02) for q in Q32])
+    assert np.all(np.abs(c - ref_c) <= 1e-4 * np.maximum(ref_c, 1e-3))
+    h = 1e-6
+    errs = []
+    for r in range(16):
+        q = Q32[r]
+        if ref_c[r] <= 0:
+            continue
+        fd = np.zeros(desc.dim)
+        for k in range(desc.dim):
+            e = np.zeros(desc.dim)
+            e[k] = h
+            fd[k] = (o.clearance_cost(q + e, 0.02) - o.clearance_cost(q - e, 0.02)) / (2 * h)
+        an = o.clearance_grad(q, 0.02)
+        assert np.linalg.norm(an - fd) <= 1e-5 * max(np.linalg.norm(fd), 1e-6)
+        errs.append(np.linalg.norm(g[r] - fd) / max(np.linalg.norm(fd), 1e-6))
+    assert errs and max(errs) <= 1e-4, errs
