@@ -744,3 +744,12 @@ int or_clearance_cost_batch(const or_scene* s, const double* q, int64_t n, doubl
     (void)threads;
     return 0;
 }
+
+/* ------------------------------------------------------------------------------------------ */
+int or_test_obb2(const double* A8, const double* B8) { return obb2_overlap(A8, B8); }
+int or_test_obb3(const double* A12, const double* a3, const double* B12, const double* b3) {
+    return obb3_overlap(A12, a3, B12, b3);
+}
+int or_test_sph_obb(const double* c3, double r, const double* G12, const double* h3) {
+    return sph_obb(c3, r, G12, h3);
+}

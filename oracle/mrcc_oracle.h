@@ -77,6 +77,11 @@ int or_clearance_grad(const or_scene* s, const double* q, double margin, double*
 int or_clearance_cost_batch(const or_scene* s, const double* q, int64_t n, double margin,
                             double* cost, int threads);
 
+/* single predicates (for pinning tests): 1 overlap / 0 separated */
+int or_test_obb2(const double* A8, const double* B8);
+int or_test_obb3(const double* A12, const double* a3, const double* B12, const double* b3);
+int or_test_sph_obb(const double* c3, double r, const double* G12, const double* h3);
+
 #ifdef __cplusplus
 }
 #endif

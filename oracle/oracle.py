@@ -65,6 +65,9 @@ def lib():
         L.or_clearance_grad.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp]
         L.or_clearance_cost_batch.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64,
                                               ctypes.c_double, _dp, ctypes.c_int]
+        L.or_test_obb2.argtypes = [_dp, _dp]
+        L.or_test_obb3.argtypes = [_dp, _dp, _dp, _dp]
+        L.or_test_sph_obb.argtypes = [_dp, ctypes.c_double, _dp, _dp]
         _lib = L
     return _lib
 
@@ -85,6 +88,24 @@ def sincos(x: float):
     s, c = ctypes.c_double(), ctypes.c_double()
     lib().or_sincos(float(x), ctypes.byref(s), ctypes.byref(c))
     return s.value, c.value
+
+
+def obb2_overlap(A8, B8) -> bool:
+    a, pa = _d(A8)
+    b, pb = _d(B8)
+    return bool(lib().or_test_obb2(pa, pb))
+
+
+def obb3_overlap(A12, a3, B12, b3) -> bool:
+    arrs = [_d(x) for x in (A12, a3, B12, b3)]
+    return bool(lib().or_test_obb3(*[p for _, p in arrs]))
+
+
+def sph_obb(c3, r, G12, h3) -> bool:
+    c, pc = _d(c3)
+    g, pg = _d(G12)
+    h, ph = _d(h3)
+    return bool(lib().or_test_sph_obb(pc, float(r), pg, ph))
 
 
 def bfs_indices(N: int):
