@@ -96,6 +96,10 @@ int mrcc_scene_create_arms(int n_arms, int dof, const double* base, const double
 int mrcc_scene_set_modes(mrcc_scene* scene, int n_modes, const int32_t* n_active,
                          const uint8_t* has_grasp, const double* grasp_rel,
                          const double* grasp_half);
+/* Arm scenes: 1 (default) = link bounding spheres cull sphere tests whose link pair (or link and
+ * box) is certainly separated; 0 = every sphere pair of the pair set is tested (brute force, used
+ * to measure the narrow phase against the FP32 roofline).  Results are identical. */
+int mrcc_scene_set_broadphase(mrcc_scene* scene, int enabled);
 int mrcc_scene_destroy(mrcc_scene* scene);
 /* kind, config dimension D, robot count, mode count */
 int mrcc_scene_info(const mrcc_scene* scene, int32_t* out4);
@@ -149,9 +153,10 @@ typedef struct mrcc_tune {
 int mrcc_edge_plan_create(const mrcc_scene* scene, const mrcc_edge_batch* batch,
                           const mrcc_tune* tune, void* stream, mrcc_edge_plan** out);
 int mrcc_edge_plan_run(mrcc_edge_plan* plan, void* stream);
-/* Synchronises; out4 = {states evaluated, pair rechecks in FP64, slots skipped (edge already in
- * collision), queue slots issued} since plan creation (zeros unless collect_stats). */
-int mrcc_edge_plan_stats(mrcc_edge_plan* plan, int64_t* out4);
+/* Synchronises; out8 = {states evaluated, pair rechecks in FP64, slots skipped (edge already in
+ * collision), queue slots issued, and for arm scenes the tests executed: link-box, sphere-box,
+ * link-pair, sphere-pair} since plan creation (zeros unless collect_stats). */
+int mrcc_edge_plan_stats(mrcc_edge_plan* plan, int64_t* out8);
 int mrcc_edge_plan_destroy(mrcc_edge_plan* plan);
 /* One-shot batched check (plan + run + destroy; synchronises). */
 int mrcc_check_edges(const mrcc_scene* scene, const mrcc_edge_batch* batch,

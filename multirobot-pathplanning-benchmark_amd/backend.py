@@ -67,9 +67,15 @@ class Scene:
         self.handle = h
         if desc.kind == "arms" and desc.modes is not None:
             self.set_modes(desc.modes)
+        self.broadphase = True
         info = np.zeros(4, dtype=np.int32)
         check(L.mrcc_scene_info(self.handle, iptr(info)))
         self.kind, self.D, self.n_robots, self.n_modes = (int(v) for v in info)
+
+    def set_broadphase(self, enabled: bool) -> None:
+        """Arm scenes: link bounding-sphere culling on (default) or brute-force pair set."""
+        check(lib().mrcc_scene_set_broadphase(self.handle, int(bool(enabled))))
+        self.broadphase = bool(enabled)
 
     def set_modes(self, modes: dict) -> None:
         na = np.ascontiguousarray(modes["n_active"], dtype=np.int32)
@@ -142,9 +148,11 @@ class EdgePlan:
         check(lib().mrcc_edge_plan_run(self.handle, stream if stream is not None else _stream_handle()))
 
     def stats(self):
-        s = (ctypes.c_int64 * 4)()
+        s = (ctypes.c_int64 * 8)()
         check(lib().mrcc_edge_plan_stats(self.handle, s))
-        return {"states": s[0], "rechecks": s[1], "dead_skips": s[2], "slots": s[3]}
+        return {"states": s[0], "rechecks": s[1], "dead_skips": s[2], "slots": s[3],
+                "link_box_tests": s[4], "sphere_box_tests": s[5], "link_pair_tests": s[6],
+                "sphere_pair_tests": s[7]}
 
     def __del__(self):
         h = getattr(self, "handle", None)

@@ -169,12 +169,58 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     for (size_t i = 0; i < H.m_rel.size(); i += 12)
         grasp_reach = std::max(grasp_reach, fabs(H.m_rel[i + 9]) + fabs(H.m_rel[i + 10]) + fabs(H.m_rel[i + 11]));
     B += grasp_reach + 2.0 * hmax;
+    // broad phase: per link a bounding sphere (centre of the local sphere centres' box, radius
+    // covering every sphere, inflated by 1e-9 relative + 1e-9 absolute against rounding), and the
+    // list of link pairs the narrow phase tests: same arm with links >= 2 apart, or different arms.
+    std::vector<double> link_bc(3 * (size_t)L, 0.0);
+    std::vector<float> link_R(L, 0.0f);
+    double Rl_max = 0.0;
+    for (int l = 0; l < L; ++l) {
+        const int f = link_first[l], e = link_first[l + 1];
+        if (f == e) continue;
+        double lo[3], hi[3];
+        for (int k = 0; k < 3; ++k) lo[k] = hi[k] = H.sph[4 * f + k];
+        for (int t = f; t < e; ++t)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], H.sph[4 * t + k]);
+                hi[k] = std::max(hi[k], H.sph[4 * t + k]);
+            }
+        double R = 0.0;
+        for (int k = 0; k < 3; ++k) link_bc[3 * l + k] = 0.5 * (lo[k] + hi[k]);
+        for (int t = f; t < e; ++t) {
+            double d = 0.0;
+            for (int k = 0; k < 3; ++k) d += (H.sph[4 * t + k] - link_bc[3 * l + k]) * (H.sph[4 * t + k] - link_bc[3 * l + k]);
+            R = std::max(R, sqrt(d) + H.sph[4 * t + 3]);
+        }
+        R = R * (1.0 + 1e-9) + 1e-9;
+        link_R[l] = (float)R;
+        if ((double)link_R[l] < R) link_R[l] = nextafterf(link_R[l], INFINITY);
+        Rl_max = std::max(Rl_max, (double)link_R[l]);
+    }
+    std::vector<int> lpairs, lpair_count(na + 1, 0);
+    for (int bm = 0; bm < na; ++bm)       // pairs ordered by the arm of their second link
+        for (int l = 0; l < L; ++l)
+            for (int m = bm * dof; m < (bm + 1) * dof; ++m) {
+                if (m <= l) continue;
+                const int al = l / dof;
+                if (al == bm && m - l < 2) continue;
+                if (link_first[l] == link_first[l + 1] || link_first[m] == link_first[m + 1]) continue;
+                lpairs.push_back((l << 8) | m);
+            }
+    for (int na_ = 1; na_ <= na; ++na_) {
+        int c = 0;
+        for (int v : lpairs)
+            if ((v & 255) / dof < na_) ++c;
+        lpair_count[na_] = c;
+    }
     Packer P;
+    size_t o_lbc = P.add(link_bc), o_lR = P.add(link_R), o_lp = P.add(lpairs), o_lpc = P.add(lpair_count);
     size_t o_base = P.add(H.base), o_dh = P.add(dh4), o_lf = P.add(link_first), o_sl = P.add(sph_link),
            o_slo = P.add(sph_lo), o_s64 = P.add(H.sph), o_sr = P.add(sph_r32), o_b32 = P.add(box32),
            o_b64 = P.add(box64), o_bch64 = P.add(H.boxes), o_bch32 = P.add(boxch32),
            o_ma = P.add(H.m_active), o_mg = P.add(H.m_grasp), o_mr = P.add(H.m_rel),
            o_mh = P.add(H.m_half);
+    const int bp = sc->dev_mem ? sc->arm.broadphase : 1;  // keep the setting across mode updates
     void* old = sc->dev_mem;
     int rc = upload(sc, P);
     if (rc != MRCC_OK) return rc;
@@ -182,6 +228,7 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     void* d = sc->dev_mem;
     ArmView& V = sc->arm;
     V.na = na; V.dof = dof; V.D = na * dof; V.L = L; V.S = S; V.nb = nb; V.nm = H.nm;
+    V.broadphase = bp;
     // binary32 margins: squared distances near contact are off by <= ~2^-21 sqrt(3) R B; box SAT
     // axis values by <= ~2^-21 (B + 3 H)
     V.Msph = (float)(ldexp(1.0, -18) * (1.0 + B) * (1.0 + 2.0 * rmax));
@@ -201,6 +248,11 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     V.m_grasp = at<unsigned char>(d, o_mg);
     V.m_rel = at<double>(d, o_mr);
     V.m_half = at<double>(d, o_mh);
+    V.Mcull = (float)(ldexp(1.0, -18) * (1.0 + B) * (1.0 + 2.0 * Rl_max));
+    V.link_bc64 = at<double>(d, o_lbc);
+    V.link_R32 = at<float>(d, o_lR);
+    V.lpairs = at<int>(d, o_lp);
+    V.lpair_count = at<int>(d, o_lpc);
     sc->D = V.D;
     sc->n_robots = na;
     sc->arm_lds = arm_lds_bytes(V.D, L, na, S);
@@ -347,7 +399,7 @@ int mrcc_scene_create_disks(int n_robots, int robot_dim, const double* robot_rad
     sc->kind = KIND_DISKS;
     sc->D = nr * rd;
     sc->n_robots = nr;
-    hipGetDevice(&sc->device);
+    (void)hipGetDevice(&sc->device);
     Packer P;
     size_t o_r = P.add(r32), o_rrT32 = P.add(rrT32), o_rrT64 = P.add(rrT64), o_sc32 = P.add(sc32),
            o_sc64 = P.add(sc64), o_srT32 = P.add(srT32), o_srT64 = P.add(srT64), o_lo32 = P.add(lo32),
@@ -415,7 +467,7 @@ int mrcc_scene_create_planar_boxes(int n_robots, const double* robot_half_extent
     sc->kind = KIND_PLANAR;
     sc->D = 3 * nr;
     sc->n_robots = nr;
-    hipGetDevice(&sc->device);
+    (void)hipGetDevice(&sc->device);
     Packer P;
     size_t o_rh32 = P.add(rh32), o_rh64 = P.add(rh64), o_ob32 = P.add(ob32), o_ob64 = P.add(ob64);
     int rc = upload(sc, P);
@@ -447,7 +499,7 @@ int mrcc_scene_create_arms(int n_arms, int dof, const double* base, const double
     SceneRec* sc = &box->rec;
     memset(sc, 0, sizeof(SceneRec));
     sc->kind = KIND_ARMS;
-    hipGetDevice(&sc->device);
+    (void)hipGetDevice(&sc->device);
     ArmHost& H = box->arm_host;
     const int L = n_arms * dof;
     H.na = n_arms;
@@ -457,9 +509,9 @@ int mrcc_scene_create_arms(int n_arms, int dof, const double* base, const double
     H.counts.assign(link_sphere_count, link_sphere_count + L);
     int S = 0;
     for (int l = 0; l < L; ++l) {
-        if (H.counts[l] < 0) {
+        if (H.counts[l] < 0 || H.counts[l] > 8) {
             delete box;
-            return fail(MRCC_ERR_INVALID, "negative sphere count");
+            return fail(MRCC_ERR_INVALID, "link %d: %d spheres (0..8 per link)", l, H.counts[l]);
         }
         S += H.counts[l];
     }
@@ -500,6 +552,14 @@ int mrcc_scene_set_modes(mrcc_scene* scene, int n_modes, const int32_t* n_active
     H.m_half.assign(grasp_half, grasp_half + 3 * nma);
     HIP_OK(hipDeviceSynchronize());  // kernels may still read the old arrays
     return build_arm_device(&box->rec, H);
+}
+
+int mrcc_scene_set_broadphase(mrcc_scene* scene, int enabled) {
+    if (!scene) return fail(MRCC_ERR_INVALID, "scene is null");
+    SceneRec* sc = rec_of(scene);
+    if (sc->kind != KIND_ARMS) return fail(MRCC_ERR_UNSUPPORTED, "broad phase applies to arm scenes");
+    sc->arm.broadphase = enabled ? 1 : 0;
+    return MRCC_OK;
 }
 
 int mrcc_scene_destroy(mrcc_scene* scene) {
@@ -575,7 +635,7 @@ int mrcc_edge_plan_create(const mrcc_scene* scene, const mrcc_edge_batch* batch,
     size_t o_eN = sz(4 * n), o_eM = sz(4 * n), o_eS = sz(4 * n), o_w0 = sz(4 * n), o_full = sz(n),
            o_order = sz(4 * n), o_hist = sz(4 * (MRCC_MAX_PASSES + 1)), o_cur = sz(4 * (MRCC_MAX_PASSES + 1)),
            o_ps = sz(8 * (MRCC_MAX_PASSES + 1)), o_meta = sz(8 * 8), o_long = sz(4 * n), o_tick = sz(4),
-           o_stats = sz(8 * 4), o_dir = arms ? 0 : sz(8 * (size_t)n * D);
+           o_stats = sz(8 * 8), o_dir = arms ? 0 : sz(8 * (size_t)n * D);
     if (hipMalloc(&p->work_mem, P.buf.size()) != hipSuccess) {
         delete p;
         return fail(MRCC_ERR_NOMEM, "workspace of %zu bytes", P.buf.size());
@@ -595,7 +655,7 @@ int mrcc_edge_plan_create(const mrcc_scene* scene, const mrcc_edge_batch* batch,
     p->W.ticket = (unsigned*)(w + o_tick);
     p->W.stats = t.collect_stats ? (unsigned long long*)(w + o_stats) : nullptr;
     p->W.dir = arms ? nullptr : (double*)(w + o_dir);
-    if (t.collect_stats) HIP_OK(hipMemset(w + o_stats, 0, 32));
+    if (t.collect_stats) HIP_OK(hipMemset(w + o_stats, 0, 64));
     HIP_OK(hipMemset(w + o_meta, 0, 64));
     if (t.use_graph && batch->n_edges > 0) {
         HIP_OK(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
@@ -631,12 +691,12 @@ int mrcc_edge_plan_run(mrcc_edge_plan* p, void* stream) {
     return MRCC_OK;
 }
 
-int mrcc_edge_plan_stats(mrcc_edge_plan* p, int64_t* out4) {
-    if (!p || !out4) return fail(MRCC_ERR_INVALID, "null argument");
+int mrcc_edge_plan_stats(mrcc_edge_plan* p, int64_t* out8) {
+    if (!p || !out8) return fail(MRCC_ERR_INVALID, "null argument");
     HIP_OK(hipDeviceSynchronize());
-    unsigned long long s[4] = {0, 0, 0, 0};
+    unsigned long long s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (p->W.stats) HIP_OK(hipMemcpy(s, p->W.stats, sizeof(s), hipMemcpyDeviceToHost));
-    for (int i = 0; i < 4; ++i) out4[i] = (int64_t)s[i];
+    for (int i = 0; i < 8; ++i) out8[i] = (int64_t)s[i];
     return MRCC_OK;
 }
 

@@ -346,6 +346,13 @@ struct ArmView {
     const unsigned char* m_grasp;  // [nm*na]
     const double* m_rel;     // [nm*na*12]
     const double* m_half;    // [nm*na*3]
+    // broad phase: link bounding spheres and the link pairs to test
+    int broadphase;          // 1: cull sphere tests with link bounding spheres
+    float Mcull;             // binary32 margin of the bounding-sphere tests
+    const double* link_bc64; // [L*3] bounding-sphere centre in the link frame
+    const float* link_R32;   // [L] bounding radius (covers every sphere of the link, inflated)
+    const int* lpairs;       // [NP] link pairs (l << 8 | m), l < m, sorted by arm of m
+    const int* lpair_count;  // [na+1] pairs whose arms are all < n_active
 };
 
 // batch of edges as seen by kernels
@@ -380,6 +387,6 @@ struct PlanWork {
     long long* meta;   // [8]: 0 total slots, 1 n_pass, 2 error count, 3 long edges, 4 tickets
     int* long_list;    // [n] edges with more passes than MAX_PASSES
     unsigned* ticket;  // [1]
-    unsigned long long* stats;  // [4]
+    unsigned long long* stats;  // [8]
     double* dir;       // [n*D] (lane-per-state kinds) or null
 };

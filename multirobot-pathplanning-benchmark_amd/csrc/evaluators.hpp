@@ -308,7 +308,14 @@ struct ArmLds {
     double* X;
     double* G64;
     float4* sph;
+    float4* linkC;  // [L] bounding sphere (x, y, z, R) of each link
+    int* cand;      // [64] compacted candidates of one broad-phase round
     float* G32;
+};
+
+// tests executed by one lane (broad phase link-box / sphere-box / link-pair / sphere-pair)
+struct ArmCount {
+    unsigned lb, sb, lp, sp;
 };
 
 // doubles before the float4 sphere table, rounded up to an even count (16-byte alignment)
@@ -317,8 +324,8 @@ __host__ __device__ inline int arm_lds_doubles(int D, int L, int na) {
 }
 
 __host__ __device__ inline size_t arm_lds_bytes(int D, int L, int na, int S) {
-    return sizeof(double) * (size_t)arm_lds_doubles(D, L, na) + sizeof(float4) * (size_t)S +
-           sizeof(float) * 16 * (size_t)na;
+    return sizeof(double) * (size_t)arm_lds_doubles(D, L, na) + sizeof(float4) * (size_t)(S + L) +
+           sizeof(int) * 64 + sizeof(float) * 16 * (size_t)na;
 }
 
 __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
@@ -330,7 +337,9 @@ __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
     W.X = d + 3 * V.D;
     W.G64 = W.X + 12 * V.L;
     W.sph = (float4*)(d + arm_lds_doubles(V.D, V.L, V.na));
-    W.G32 = (float*)(W.sph + V.S);
+    W.linkC = W.sph + V.S;
+    W.cand = (int*)(W.linkC + V.L);
+    W.G32 = (float*)(W.cand + 64);
     return W;
 }
 
@@ -374,8 +383,12 @@ __device__ __attribute__((noinline)) bool arm_recheck_obb_obb(const ArmView& V, 
 
 // Evaluates the state whose binary64 joint vector is in W.q (written by lanes < D before the
 // call).  All 64 lanes of the block must call.  Returns the wave-uniform result.
+__device__ inline unsigned lanes_below(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
 __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int mode,
-                                          unsigned mask, unsigned& rechecks) {
+                                          unsigned mask, unsigned& rechecks, ArmCount& cnt) {
     const int lane = threadIdx.x;
     if (mode < 0 || mode >= V.nm) mode = 0;
     const int na = V.m_active[mode];
@@ -424,9 +437,116 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int
         sphere_center_f64(V, W, s, c);
         W.sph[s] = make_float4((float)c[0], (float)c[1], (float)c[2], V.sph_r32[s]);
     }
+    if (V.broadphase && lane < Dact) {
+        const double* b = V.link_bc64 + 3 * lane;
+        double c[3];
+        xform_point_f64(W.X + 12 * lane, b[0], b[1], b[2], c);
+        W.linkC[lane] = make_float4((float)c[0], (float)c[1], (float)c[2], V.link_R32[lane]);
+    }
     __syncthreads();
     const float M = V.Msph;
     bool hit = false;
+    if (V.broadphase) {
+        const float Mc = V.Mcull;
+        // 4'. links vs boxes, then the spheres of candidate (link, box) pairs: 8 candidates x 8
+        //     spheres per wave iteration
+        for (int l = 0; l < Dact; ++l) {
+            if (!((mask >> (l / dof)) & 1u)) continue;
+            const int f = V.link_first[l], ns = V.link_first[l + 1] - f;
+            if (ns == 0) continue;
+            const float4 Lc = W.linkC[l];
+            const float Tl = Lc.w * Lc.w;
+            for (int b0 = 0; b0 < V.nb; b0 += 64) {
+                const int b = b0 + lane;
+                bool cand = false;
+                if (b < V.nb) {
+                    const float* bx = V.box32 + 8 * b;
+                    const float e0 = fminf(fmaxf(Lc.x, bx[0]), bx[4]) - Lc.x;
+                    const float e1 = fminf(fmaxf(Lc.y, bx[1]), bx[5]) - Lc.y;
+                    const float e2 = fminf(fmaxf(Lc.z, bx[2]), bx[6]) - Lc.z;
+                    cand = fmaf(e2, e2, fmaf(e1, e1, e0 * e0)) <= Tl + Mc;
+                    ++cnt.lb;
+                }
+                const unsigned long long m = __ballot(cand);
+                if (m == 0ull) continue;
+                if (cand) W.cand[lanes_below(m)] = b;
+                __syncthreads();
+                const int n = __popcll(m);
+                for (int c0 = 0; c0 < n; c0 += 8) {
+                    const int k = c0 + (lane >> 3), j = lane & 7;
+                    if (k < n && j < ns) {
+                        const int bb = W.cand[k];
+                        const int s = f + j;
+                        const float4 c = W.sph[s];
+                        const float* bx = V.box32 + 8 * bb;
+                        const float e0 = fminf(fmaxf(c.x, bx[0]), bx[4]) - c.x;
+                        const float e1 = fminf(fmaxf(c.y, bx[1]), bx[5]) - c.y;
+                        const float e2 = fminf(fmaxf(c.z, bx[2]), bx[6]) - c.z;
+                        const float d2 = fmaf(e2, e2, fmaf(e1, e1, e0 * e0));
+                        const float T = c.w * c.w;
+                        ++cnt.sb;
+                        if (d2 <= T + M) {
+                            if (d2 < T - M) hit = true;
+                            else {
+                                ++rechecks;
+                                if (arm_recheck_sph_box(V, W, s, bb)) hit = true;
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+                if (__any(hit)) return true;
+            }
+        }
+        // 5'. link pairs of the pair set, then the sphere pairs of candidate link pairs (8 x 8)
+        const int NP = V.lpair_count[na];
+        for (int p0 = 0; p0 < NP; p0 += 64) {
+            const int pi = p0 + lane;
+            bool cand = false;
+            int v = 0;
+            if (pi < NP) {
+                v = V.lpairs[pi];
+                const int l = v >> 8, mm = v & 255;
+                if (((mask >> (l / dof)) & 1u) && ((mask >> (mm / dof)) & 1u)) {
+                    const float4 A = W.linkC[l], Bc = W.linkC[mm];
+                    const float dx = A.x - Bc.x, dy = A.y - Bc.y, dz = A.z - Bc.z;
+                    const float R = A.w + Bc.w;
+                    cand = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) <= R * R + Mc;
+                    ++cnt.lp;
+                }
+            }
+            const unsigned long long m = __ballot(cand);
+            if (m == 0ull) continue;
+            if (cand) W.cand[lanes_below(m)] = v;
+            __syncthreads();
+            const int n = __popcll(m);
+            for (int c = 0; c < n; ++c) {
+                const int w = W.cand[c];
+                const int l = w >> 8, mm = w & 255;
+                const int fl = V.link_first[l], nl = V.link_first[l + 1] - fl;
+                const int fm = V.link_first[mm], nm = V.link_first[mm + 1] - fm;
+                const int i = fl + (lane >> 3), j = fm + (lane & 7);
+                if ((lane >> 3) < nl && (lane & 7) < nm) {
+                    const float4 ci = W.sph[i], cj = W.sph[j];
+                    const float dx = ci.x - cj.x, dy = ci.y - cj.y, dz = ci.z - cj.z;
+                    const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    const float R = ci.w + cj.w;
+                    const float T = R * R;
+                    ++cnt.sp;
+                    if (d2 <= T + M) {
+                        if (d2 < T - M) hit = true;
+                        else {
+                            ++rechecks;
+                            if (arm_recheck_sph_sph(V, W, i, j)) hit = true;
+                        }
+                    }
+                }
+                if ((c & 3) == 3 && __any(hit)) return true;
+            }
+            __syncthreads();
+            if (__any(hit)) return true;
+        }
+    } else {
     // 4. spheres vs boxes
     for (int base = 0; base < Sact; base += 64) {
         const int s = base + lane;
@@ -439,6 +559,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int
             const float e1 = fminf(fmaxf(c.y, bx[1]), bx[5]) - c.y;
             const float e2 = fminf(fmaxf(c.z, bx[2]), bx[6]) - c.z;
             const float d2 = fmaf(e2, e2, fmaf(e1, e1, e0 * e0));
+            cnt.sb += act;
             if (act && d2 <= T + M) {
                 if (d2 < T - M) hit = true;
                 else {
@@ -456,7 +577,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int
         const float4 ci = W.sph[act ? i : 0];
         const int lo = act ? V.sph_lo[i] : 0x7fffffff;
         int jstart = V.sph_lo[base];
-        int cnt = 0;
+        int nj = 0;
         for (int j = jstart; j < Sact; ++j) {
             const int aj = V.sph_link[j] / dof;
             if (!((mask >> aj) & 1u)) continue;  // wave-uniform
@@ -465,6 +586,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int
             const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
             const float R = ci.w + cj.w;
             const float T = R * R;
+            cnt.sp += (j >= lo);
             if (j >= lo && d2 <= T + M) {
                 if (d2 < T - M) hit = true;
                 else {
@@ -472,13 +594,14 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int
                     if (arm_recheck_sph_sph(V, W, i, j)) hit = true;
                 }
             }
-            if (++cnt == 16) {
-                cnt = 0;
+            if (++nj == 16) {
+                nj = 0;
                 if (__any(hit)) return true;
             }
         }
         if (__any(hit)) return true;
     }
+    }  // brute force
     // 6. held boxes
     unsigned gmask = 0;
     for (int a = 0; a < na; ++a)

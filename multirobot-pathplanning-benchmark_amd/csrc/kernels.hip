@@ -60,7 +60,7 @@ __global__ void k_plan_init(PlanWork W) {
     for (int b = t; b <= MRCC_MAX_PASSES; b += stride) W.hist[b] = 0;
     if (t < 8) W.meta[t] = 0;
     if (t == 0) W.ticket[0] = 0;
-    if (t < 4 && W.stats) W.stats[t] = 0;
+    if (t < 8 && W.stats) W.stats[t] = 0;
 }
 
 __global__ void k_plan_prep(EdgeBatchView B, PlanWork W, int D, int lgG, int want_dir) {
@@ -386,6 +386,7 @@ __global__ void __launch_bounds__(64) k_arm_persistent(ArmView V, EdgeBatchView 
     const long long T = W.meta[0];
     unsigned long long n_eval = 0, n_dead = 0, n_slots = 0;
     unsigned rechecks = 0;
+    ArmCount cnt = {0, 0, 0, 0};
     int p = 0;  // pass hint
     for (;;) {
         unsigned t = 0;
@@ -407,7 +408,7 @@ __global__ void __launch_bounds__(64) k_arm_persistent(ArmView V, EdgeBatchView 
             const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
             arm_load_state(V, Wl, B, ref.e, w.N, i, V.m_active[md] * V.dof);
             ++n_eval;
-            if (arm_state_collides(V, Wl, mode, B.mask, rechecks)) {
+            if (arm_state_collides(V, Wl, mode, B.mask, rechecks, cnt)) {
                 if (threadIdx.x == 0) B.out[ref.e] = 0;
             }
             __syncthreads();
@@ -418,7 +419,13 @@ __global__ void __launch_bounds__(64) k_arm_persistent(ArmView V, EdgeBatchView 
         stat_add(W.stats, 2, n_dead);
         stat_add(W.stats, 3, n_slots);
     }
-    if (W.stats) stat_add(W.stats, 1, rechecks);
+    if (W.stats) {
+        stat_add(W.stats, 1, rechecks);
+        stat_add(W.stats, 4, cnt.lb);
+        stat_add(W.stats, 5, cnt.sb);
+        stat_add(W.stats, 6, cnt.lp);
+        stat_add(W.stats, 7, cnt.sp);
+    }
 }
 
 __global__ void __launch_bounds__(64) k_arm_direct(ArmView V, EdgeBatchView B, const int* list,
@@ -429,6 +436,7 @@ __global__ void __launch_bounds__(64) k_arm_direct(ArmView V, EdgeBatchView B, c
     ArmLds Wl = arm_lds(lds_arm, V);
     unsigned long long n_eval = 0;
     unsigned rechecks = 0;
+    ArmCount cnt = {0, 0, 0, 0};
     if (n_dev) n_list = *n_dev;
     const long long nblocks = n_list * wpe;
     for (long long blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
@@ -453,7 +461,7 @@ __global__ void __launch_bounds__(64) k_arm_direct(ArmView V, EdgeBatchView B, c
             if (i < 0) continue;
             arm_load_state(V, Wl, B, e, N, i, V.m_active[md] * V.dof);
             ++n_eval;
-            const bool c = arm_state_collides(V, Wl, mode, B.mask, rechecks);
+            const bool c = arm_state_collides(V, Wl, mode, B.mask, rechecks, cnt);
             __syncthreads();
             if (c) {
                 if (threadIdx.x == 0) B.out[e] = 0;
@@ -463,7 +471,13 @@ __global__ void __launch_bounds__(64) k_arm_direct(ArmView V, EdgeBatchView B, c
         __syncthreads();
     }
     if (stats && threadIdx.x == 0) stat_add(stats, 0, n_eval);
-    if (stats) stat_add(stats, 1, rechecks);
+    if (stats) {
+        stat_add(stats, 1, rechecks);
+        stat_add(stats, 4, cnt.lb);
+        stat_add(stats, 5, cnt.sb);
+        stat_add(stats, 6, cnt.lp);
+        stat_add(stats, 7, cnt.sp);
+    }
 }
 
 __global__ void __launch_bounds__(64) k_arm_configs(ArmView V, const double* qv, long long n,
@@ -472,13 +486,14 @@ __global__ void __launch_bounds__(64) k_arm_configs(ArmView V, const double* qv,
     extern __shared__ double lds_arm[];
     ArmLds Wl = arm_lds(lds_arm, V);
     unsigned rechecks = 0;
+    ArmCount cnt = {0, 0, 0, 0};
     for (long long r = blockIdx.x; r < n; r += gridDim.x) {
         const int mode = modes ? modes[r] : 0;
         const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
         const int Dact = V.m_active[md] * V.dof;
         if ((int)threadIdx.x < Dact) Wl.q[threadIdx.x] = qv[r * rs + (long long)threadIdx.x * ds];
         __syncthreads();
-        const bool c = arm_state_collides(V, Wl, mode, mask, rechecks);
+        const bool c = arm_state_collides(V, Wl, mode, mask, rechecks, cnt);
         if (threadIdx.x == 0) out[r] = c ? 0 : 1;
         __syncthreads();
     }

@@ -247,18 +247,23 @@ def c3(B):
     return desc, B.Scene(desc), O.Scene(desc), rng
 
 
-def test_c3_configs(c3):
-    _say("C3 configs: uniform + near-home vs oracle")
+@pytest.mark.parametrize("bp", [True, False], ids=["broadphase", "bruteforce"])
+def test_c3_configs(c3, bp):
+    _say(f"C3 configs: uniform + near-home vs oracle (broadphase={bp})")
     desc, sc, o, rng = c3
+    sc.set_broadphase(bp)
     Q = np.concatenate([scenes.uniform_configs(rng, desc, 4000), near_home(rng, desc, 8000)])
     ref = o.check_configs(Q)
     assert 0.05 < ref.mean() < 0.95
     assert np.array_equal(gpu_configs(sc, Q), ref)
+    sc.set_broadphase(True)
 
 
-def test_c3_edges(c3, B):
-    _say("C3 edges near home + lognormal uniform vs oracle (persistent, single launch, per call)")
+@pytest.mark.parametrize("bp", [True, False], ids=["broadphase", "bruteforce"])
+def test_c3_edges(c3, B, bp):
+    _say(f"C3 edges near home + lognormal uniform vs oracle (persistent, single launch, per call; broadphase={bp})")
     desc, sc, o, rng = c3
+    sc.set_broadphase(bp)
     a = near_home(rng, desc, 3000)
     b = np.clip(a + rng.normal(0, 0.3, a.shape), desc.limits[0], desc.limits[1])
     u1, u2 = scenes.lognormal_edges(rng, desc, 1000, 0.05)
@@ -267,6 +272,7 @@ def test_c3_edges(c3, B):
     assert np.array_equal(gpu_edges(sc, q1, q2, 0.05), ref)
     assert np.array_equal(gpu_edges(sc, q1, q2, 0.05, tune=B.make_tune(persistent=0, sorted=0, use_graph=0)), ref)
     assert np.array_equal(gpu_edges(sc, q1[:300], q2[:300], 0.05, per_call=True), ref[:300])
+    sc.set_broadphase(True)
 
 
 def test_c3_masks(c3):
@@ -277,9 +283,11 @@ def test_c3_masks(c3):
         assert np.array_equal(gpu_configs(sc, Q, mask=mask), o.check_configs(Q, mask=mask)), mask
 
 
-def test_c3_near_contact(c3):
-    _say("C3 near-contact: configs bisected onto the contact boundary")
+@pytest.mark.parametrize("bp", [True, False], ids=["broadphase", "bruteforce"])
+def test_c3_near_contact(c3, bp):
+    _say(f"C3 near-contact: configs bisected onto the contact boundary (broadphase={bp})")
     desc, sc, o, rng = c3
+    sc.set_broadphase(bp)
     A = near_home(rng, desc, 400)
     Bq = near_home(rng, desc, 400, sigma=1.2)
     fa, fb = o.check_configs(A), o.check_configs(Bq)
@@ -298,15 +306,18 @@ def test_c3_near_contact(c3):
             pts.append(a + t * (b - a))
     P = np.array(pts)
     assert np.array_equal(gpu_configs(sc, P), o.check_configs(P))
+    sc.set_broadphase(True)
 
 
 # ---------------------------------------------------------------------------------------------
 # C5: mixed arm counts and held boxes
-def test_c5_modes(B):
-    _say("C5 mixed modes: configs and edges with mode ids vs oracle")
+@pytest.mark.parametrize("bp", [True, False], ids=["broadphase", "bruteforce"])
+def test_c5_modes(B, bp):
+    _say(f"C5 mixed modes: configs and edges with mode ids vs oracle (broadphase={bp})")
     rng = np.random.default_rng(13)
     desc = scenes.mixed_arms(rng=rng, n_modes=256)
     sc, o = B.Scene(desc), O.Scene(desc)
+    sc.set_broadphase(bp)
     n = 6000
     Q = near_home(rng, desc, n, sigma=0.5)
     modes = rng.integers(0, 256, n).astype(np.int32)
