@@ -55,6 +55,14 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    # C++ per-call driver over the C ABI (cpp/percall_bench.cpp)
+    cpp = os.path.join(PKG, "cpp")
+    r = subprocess.run([HIPCC, "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}",
+                        os.path.join(cpp, "percall_bench.cpp"), f"-L{PKG}", "-lmrcc",
+                        "-Wl,-rpath,$ORIGIN/..", "-o", os.path.join(cpp, "percall_bench")],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"percall_bench failed:\n{r.stderr}")
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -93,6 +93,31 @@ __host__ __device__ inline void dh_step_f64(double* X, double d, double a, doubl
     for (int k = 0; k < 12; ++k) X[k] = Y[k];
 }
 
+// one row (R0 R1 R2 | t) of X <- X * A(q): the row-r entries of dh_step_f64, same operations
+__host__ __device__ inline void dh_row_f64(double& R0, double& R1, double& R2, double& t,
+                                           const double* p, double cs, double sn) {
+#pragma clang fp contract(off)
+    const double d = p[0], a = p[1], ca = p[2], sa = p[3];
+    const double A01 = -(sn * ca), A02 = sn * sa, A03 = a * cs;
+    const double A11 = cs * ca, A12 = -(cs * sa), A13 = a * sn;
+    const double y0 = (R0 * cs + R1 * sn) + R2 * 0.0;
+    const double y1 = (R0 * A01 + R1 * A11) + R2 * sa;
+    const double y2 = (R0 * A02 + R1 * A12) + R2 * ca;
+    const double ty = ((R0 * A03 + R1 * A13) + R2 * d) + t;
+    R0 = y0;
+    R1 = y1;
+    R2 = y2;
+    t = ty;
+}
+
+// one row of G = X * Rel (compose_f64)
+__host__ __device__ inline void compose_row_f64(double R0, double R1, double R2, double t,
+                                                const double* Rel, double* g) {
+#pragma clang fp contract(off)
+    for (int c = 0; c < 3; ++c) g[c] = (R0 * Rel[c] + R1 * Rel[3 + c]) + R2 * Rel[6 + c];
+    g[3] = ((R0 * Rel[9] + R1 * Rel[10]) + R2 * Rel[11]) + t;
+}
+
 __host__ __device__ inline void xform_point_f64(const double* X, double l0, double l1, double l2,
                                                 double* c) {
 #pragma clang fp contract(off)

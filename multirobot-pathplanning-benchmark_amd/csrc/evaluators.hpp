@@ -405,28 +405,32 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int
     }
     if (__any(bad)) return true;  // non-finite coordinates are reported as collisions
     __syncthreads();
-    // 2. forward kinematics (binary64), one arm per lane; grasped boxes
-    if (lane < na) {
-        double X[12];
-        for (int k = 0; k < 12; ++k) X[k] = V.base[12 * lane + k];
+    // 2. forward kinematics (binary64): lane (arm a, row r) carries row r of the arm's transform
+    //    through the chain (each entry with the operations of dh_step_f64); grasped boxes
+    if (lane < 3 * na) {
+        const int a = lane / 3, r = lane - 3 * a;
+        double R0 = V.base[12 * a + 3 * r], R1 = V.base[12 * a + 3 * r + 1], R2 = V.base[12 * a + 3 * r + 2];
+        double t = V.base[12 * a + 9 + r];
         for (int k = 0; k < dof; ++k) {
-            const int l = lane * dof + k;
-            const double* p = V.dh + 4 * l;
-            SinCos j;
-            j.c = W.cs[l];
-            j.s = W.sn[l];
-            dh_step_f64(X, p[0], p[1], p[2], p[3], j);
-            for (int t = 0; t < 12; ++t) W.X[12 * l + t] = X[t];
+            const int l = a * dof + k;
+            dh_row_f64(R0, R1, R2, t, V.dh + 4 * l, W.cs[l], W.sn[l]);
+            double* X = W.X + 12 * l;
+            X[3 * r] = R0;
+            X[3 * r + 1] = R1;
+            X[3 * r + 2] = R2;
+            X[9 + r] = t;
         }
-        if (V.m_grasp[mode * V.na + lane]) {
-            double G[12];
-            compose_f64(X, V.m_rel + 12 * (mode * V.na + lane), G);
-            const double* h = V.m_half + 3 * (mode * V.na + lane);
-            for (int t = 0; t < 12; ++t) {
-                W.G64[12 * lane + t] = G[t];
-                W.G32[16 * lane + t] = (float)G[t];
+        if (V.m_grasp[mode * V.na + a]) {
+            const double* Rel = V.m_rel + 12 * (mode * V.na + a);
+            double g[4];
+            compose_row_f64(R0, R1, R2, t, Rel, g);
+            for (int c = 0; c < 3; ++c) {
+                W.G64[12 * a + 3 * r + c] = g[c];
+                W.G32[16 * a + 3 * r + c] = (float)g[c];
             }
-            for (int t = 0; t < 3; ++t) W.G32[16 * lane + 12 + t] = (float)h[t];
+            W.G64[12 * a + 9 + r] = g[3];
+            W.G32[16 * a + 9 + r] = (float)g[3];
+            W.G32[16 * a + 12 + r] = (float)V.m_half[3 * (mode * V.na + a) + r];
         }
     }
     __syncthreads();
