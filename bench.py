@@ -65,6 +65,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--per-call-edges", type=int, default=4096)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-secondary", action="store_true")
+    ap.add_argument("--mixed-instances", type=int, default=65536)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one device copy of 256 MiB (calibrates FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
@@ -240,6 +244,14 @@ def main():
         checked += kc
         log(f"cpu oracle: {cpu['value']:.4g} edge-states/s on {threads} threads; mismatches {mism} / {checked}")
 
+    secondary = {} if args.skip_secondary else secondary_workloads(args, backend, scenes, dev)
+    if args.copy_probe:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: 268435456 bytes read, 268435456 bytes written")
+
     line = {
         "metric": "edge-state collision checks/sec (bit-exact), batched persistent, 4x7-DoF arms",
         "value": value,
@@ -272,8 +284,110 @@ def main():
         "checks_compared": checked,
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
         "speedup_vs_per_call": value / per_call_rate,
+        "secondary": secondary,
     }
     print(json.dumps(line), flush=True)
+
+
+def secondary_workloads(args, backend, scenes, dev):
+    """C1 (the reference's abstract environment) and C2 (planar boxes): batched GPU rate, CPU oracle
+    on the host cores, and for C1 the numpy restatement of abstract_env.py called per edge."""
+    import torch
+    from oracle import oracle as O
+    from oracle.abstract_np import AbstractNp
+
+    out = {}
+    threads = len(os.sched_getaffinity(0))
+    for name, mk, gen, res in (
+            ("c1_planar_disks_10k_edges", scenes.planar_disks, "pairs", 0.01),
+            ("c2_planar_boxes_262k_edges", scenes.planar_boxes, "lognormal", 0.01)):
+        rng = np.random.default_rng(1)
+        desc = mk(rng=rng)
+        sc = backend.Scene(desc)
+        if gen == "pairs":
+            q1, q2 = scenes.random_pair_edges(rng, desc, 10000)
+        else:
+            q1, q2 = scenes.lognormal_edges(rng, desc, 262144, res)
+        n_states, N = logical_states(q1, q2, res)
+        a, b = torch.from_numpy(q1).to(dev), torch.from_numpy(q2).to(dev)
+        o_ = torch.empty(len(q1), dtype=torch.uint8, device=dev)
+        plan = backend.EdgePlan(sc, sc.edge_batch(a, b, o_, res), backend.make_tune(use_graph=1))
+        plan.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reps = 10
+        for _ in range(reps):
+            plan.run()
+        torch.cuda.synchronize()
+        gpu_rate = n_states / ((time.perf_counter() - t0) / reps)
+        got = o_.cpu().numpy().astype(bool)
+        ent = {"edges": len(q1), "edge_states": n_states, "gpu_batched": gpu_rate, "unit": "edge-states/s"}
+        if not args.skip_cpu:
+            o = O.Scene(desc)
+            k = min(len(q1), 20000 if gen == "pairs" else 40000)
+            t0 = time.perf_counter()
+            ref = o.check_edges(q1[:k], q2[:k], res, threads=threads)
+            dt = time.perf_counter() - t0
+            ent["cpu_oracle"] = {"value": int(np.sum(N[:k] - 2)) / dt, "cores": threads, "edges": k}
+            ent["mismatches"] = int(np.sum(ref != got[:k]))
+            if gen == "pairs":
+                mirror = AbstractNp(desc)
+                t0 = time.perf_counter()
+                kk = 0
+                st = 0
+                while time.perf_counter() - t0 < 3.0 and kk < len(q1):
+                    mirror.is_edge_collision_free(q1[kk], q2[kk], res)
+                    st += int(N[kk] - 2)
+                    kk += 1
+                ent["numpy_per_call"] = {"value": st / (time.perf_counter() - t0), "cores": 1, "edges": kk,
+                                         "note": "numpy restatement of abstract_env.py, one call per edge"}
+        out[name] = ent
+        log(f"{name}: {ent}")
+    # C5: mixed arm counts and held boxes, one launch over all instances (paths of 1..64 edges)
+    n_inst = args.mixed_instances
+    rng = np.random.default_rng(5)
+    desc = scenes.mixed_arms(rng=rng, n_modes=n_inst)
+    sc = backend.Scene(desc)
+    V, i1, i2, modes, inst = scenes.mixed_instances(rng, desc, n_inst, 0.05)
+    n_states, N = logical_states(V[i1], V[i2], 0.05)
+    Vd = torch.from_numpy(V).to(dev)
+    I1, I2 = torch.from_numpy(i1).to(dev), torch.from_numpy(i2).to(dev)
+    Md = torch.from_numpy(modes).to(dev)
+    o_ = torch.empty(len(i1), dtype=torch.uint8, device=dev)
+    batch = sc.edge_batch(Vd, Vd, o_, 0.05, modes=Md, idx1=I1, idx2=I2)
+    plan = backend.EdgePlan(sc, batch, backend.make_tune(use_graph=1))
+    plan.run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        plan.run()
+    torch.cuda.synchronize()
+    gpu_rate = n_states / ((time.perf_counter() - t0) / 5)
+    got = o_.cpu().numpy().astype(bool)
+    ent = {"instances": n_inst, "edges": len(i1), "edge_states": n_states, "gpu_batched": gpu_rate,
+           "unit": "edge-states/s", "free_frac": float(got.mean())}
+    # plain per-instance form: one single-launch check per instance (first 256 instances)
+    k_inst = min(256, n_inst)
+    k_e = int(np.searchsorted(inst, k_inst))
+    t0 = time.perf_counter()
+    for i in range(k_inst):
+        lo, hi = int(np.searchsorted(inst, i)), int(np.searchsorted(inst, i + 1))
+        bi = sc.edge_batch(Vd, Vd, o_[lo:hi], 0.05, modes=Md[lo:hi], idx1=I1[lo:hi], idx2=I2[lo:hi])
+        sc.check_edges_device(bi, backend.make_tune(persistent=0, sorted=0, use_graph=0))
+    torch.cuda.synchronize()
+    ent["gpu_per_instance"] = int(np.sum(N[:k_e] - 2)) / (time.perf_counter() - t0)
+    ent["per_instance_matches"] = bool(np.array_equal(o_[:k_e].cpu().numpy().astype(bool), got[:k_e]))
+    if not args.skip_cpu:
+        o = O.Scene(desc)
+        k = min(len(i1), 60000)
+        t0 = time.perf_counter()
+        ref = o.check_edges(V[i1[:k]], V[i2[:k]], 0.05, modes=modes[:k], threads=threads)
+        dt = time.perf_counter() - t0
+        ent["cpu_oracle"] = {"value": int(np.sum(N[:k] - 2)) / dt, "cores": threads, "edges": k}
+        ent["mismatches"] = int(np.sum(ref != got[:k]))
+    out["c5_mixed_arms"] = ent
+    log(f"c5_mixed_arms: {ent}")
+    return out
 
 
 def free_edges(rng, desc, sc, n, res, dev):

@@ -260,6 +260,8 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     sc->arm_dof = dof;
     sc->arm_S = S;
     sc->arm_nm = H.nm;
+    sc->arm_has_grasp = 0;
+    for (unsigned char g : H.m_grasp) sc->arm_has_grasp |= (g != 0);
     return MRCC_OK;
 }
 
@@ -634,7 +636,7 @@ int mrcc_edge_plan_create(const mrcc_scene* scene, const mrcc_edge_batch* batch,
     auto sz = [&](size_t bytes) { size_t off = (P.buf.size() + 255) & ~size_t(255); P.buf.resize(off + bytes); return off; };
     size_t o_eN = sz(4 * n), o_eM = sz(4 * n), o_eS = sz(4 * n), o_w0 = sz(4 * n), o_full = sz(n),
            o_order = sz(4 * n), o_hist = sz(4 * (MRCC_MAX_PASSES + 1)), o_cur = sz(4 * (MRCC_MAX_PASSES + 1)),
-           o_ps = sz(8 * (MRCC_MAX_PASSES + 1)), o_meta = sz(8 * 8), o_long = sz(4 * n), o_tick = sz(4),
+           o_ps = sz(8 * (MRCC_MAX_PASSES + 1)), o_meta = sz(8 * 8), o_long = sz(4 * n), o_tick = sz(8),
            o_stats = sz(8 * 8), o_dir = arms ? 0 : sz(8 * (size_t)n * D);
     if (hipMalloc(&p->work_mem, P.buf.size()) != hipSuccess) {
         delete p;
@@ -652,7 +654,7 @@ int mrcc_edge_plan_create(const mrcc_scene* scene, const mrcc_edge_batch* batch,
     p->W.pass_start = (long long*)(w + o_ps);
     p->W.meta = (long long*)(w + o_meta);
     p->W.long_list = (int*)(w + o_long);
-    p->W.ticket = (unsigned*)(w + o_tick);
+    p->W.ticket = (unsigned long long*)(w + o_tick);
     p->W.stats = t.collect_stats ? (unsigned long long*)(w + o_stats) : nullptr;
     p->W.dir = arms ? nullptr : (double*)(w + o_dir);
     if (t.collect_stats) HIP_OK(hipMemset(w + o_stats, 0, 64));
@@ -738,7 +740,8 @@ int mrcc_check_edges(const mrcc_scene* scene, const mrcc_edge_batch* batch, cons
     }
     mrcc_edge_plan_destroy(p);
     if (rc == MRCC_OK && meta[2] > 0)
-        return fail(MRCC_ERR_INVALID, "%lld edges have N_start > N (reference asserts)", meta[2]);
+        return fail(MRCC_ERR_INVALID, "%lld edges have N_start > N (the reference asserts) or more than %d states",
+                    meta[2], MRCC_MAX_EDGE_STATES);
     return rc;
 }
 
@@ -825,6 +828,14 @@ int mrcc_check_edge_host(const mrcc_scene* scene, const double* q1, const double
     if (N <= 0 && !(resolution > 0.0)) return fail(MRCC_ERR_INVALID, "resolution must be > 0");
     if (N_start < 0) return fail(MRCC_ERR_INVALID, "N_start < 0");
     SceneRec* sc = (SceneRec*)rec_of(scene);
+    int Nh = N;
+    if (Nh <= 0) {  // the same binary64 rule as the device (abstract_env.py:321-323)
+        double m = fabs(q1[0] - q2[0]);
+        for (int k = 1; k < sc->D; ++k) m = std::max(m, fabs(q1[k] - q2[k]));
+        Nh = mrcc::edge_N_from_max(m, resolution);
+    }
+    if (N_start > Nh) return fail(MRCC_ERR_INVALID, "N_start %d > N %d (the reference asserts)", N_start, Nh);
+    if (Nh > MRCC_MAX_EDGE_STATES) return fail(MRCC_ERR_INVALID, "edge of %d states (at most %d)", Nh, MRCC_MAX_EDGE_STATES);
     hipStream_t st = nullptr;
     EdgeBatchView v;
     int rc = stage_edge(sc, q1, q2, N, N_start, N_max, mode_id, st, &v);

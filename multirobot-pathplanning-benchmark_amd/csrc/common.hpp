@@ -22,6 +22,7 @@
 #define MRCC_DISK_MAX_ROBOTS 16
 #define MRCC_PLANAR_MAX_ROBOTS 16
 #define MRCC_MAX_PASSES 16384
+#define MRCC_MAX_EDGE_STATES (1 << 24)  // longer edges are rejected (error), not iterated
 
 enum { KIND_DISKS = 1, KIND_PLANAR = 2, KIND_ARMS = 3 };
 
@@ -304,7 +305,7 @@ __host__ __device__ inline int edge_N_from_max(double m, double res) {
 __host__ __device__ inline EdgeWin edge_window(int N, int N_start, int N_max, int incl) {
     EdgeWin w;
     w.N = N;
-    w.err = (N_start < 0 || N_start > N) ? 1 : 0;
+    w.err = (N_start < 0 || N_start > N || N > MRCC_MAX_EDGE_STATES) ? 1 : 0;
     int hi = (N_max < 0 || N_max > N) ? N : N_max;
     w.full = (N_start == 0 && hi == N) ? 1 : 0;
     w.first = incl ? 0 : 1;
@@ -411,7 +412,7 @@ struct PlanWork {
     long long* pass_start;  // [MAX_PASSES+1] slot offset of each pass (G * sum E)
     long long* meta;   // [8]: 0 total slots, 1 n_pass, 2 error count, 3 long edges, 4 tickets
     int* long_list;    // [n] edges with more passes than MAX_PASSES
-    unsigned* ticket;  // [1]
+    unsigned long long* ticket;  // [1]
     unsigned long long* stats;  // [8]
     double* dir;       // [n*D] (lane-per-state kinds) or null
 };

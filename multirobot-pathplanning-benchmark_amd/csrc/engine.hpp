@@ -17,6 +17,7 @@ struct SceneRec {
     size_t dev_bytes;
     // host copies needed by host code
     int arm_na, arm_dof, arm_S, arm_nm;
+    int arm_has_grasp;    // some mode holds a box: the evaluator with the held-box phase
     // staging buffers of the host-pointer per-call entry points (lazily allocated)
     double* stage_dev;
     double* stage_host;

@@ -387,6 +387,8 @@ __device__ inline unsigned lanes_below(unsigned long long m) {
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
+// GRASP = false instantiates the evaluator for scenes without held boxes (fewer registers).
+template <bool GRASP>
 __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int mode,
                                           unsigned mask, unsigned& rechecks, ArmCount& cnt) {
     const int lane = threadIdx.x;
@@ -420,7 +422,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int
             X[3 * r + 2] = R2;
             X[9 + r] = t;
         }
-        if (V.m_grasp[mode * V.na + a]) {
+        if (GRASP && V.m_grasp[mode * V.na + a]) {
             const double* Rel = V.m_rel + 12 * (mode * V.na + a);
             double g[4];
             compose_row_f64(R0, R1, R2, t, Rel, g);
@@ -607,6 +609,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int
     }
     }  // brute force
     // 6. held boxes
+    if constexpr (!GRASP) return false;
     unsigned gmask = 0;
     for (int a = 0; a < na; ++a)
         if (V.m_grasp[mode * V.na + a] && ((mask >> a) & 1u)) gmask |= 1u << a;

@@ -261,9 +261,10 @@ __global__ void __launch_bounds__(256) k_lane_persistent(View S, EdgeBatchView B
     const int lane = threadIdx.x & 63;
     int pw = 0;  // pass hint of this wave
     for (;;) {
-        unsigned t = 0;
-        if (lane == 0) t = atomicAdd(W.ticket, 1u);
-        t = __builtin_amdgcn_readfirstlane(t);
+        unsigned long long t = 0;
+        if (lane == 0) t = atomicAdd(W.ticket, 1ull);
+        t = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(t >> 32)) << 32) |
+            (unsigned)__builtin_amdgcn_readfirstlane((unsigned)t);
         const long long s0 = (long long)t * spt * 64;
         if (s0 >= T) break;
         while (pw < MRCC_MAX_PASSES && W.pass_start[pw + 1] <= s0) ++pw;
@@ -379,6 +380,7 @@ __device__ inline void arm_load_state(const ArmView& V, const ArmLds& Wl, const 
     __syncthreads();
 }
 
+template <bool GRASP>
 __global__ void __launch_bounds__(64) k_arm_persistent(ArmView V, EdgeBatchView B, PlanWork W,
                                                        int lgG, int spt) {
     extern __shared__ double lds_arm[];
@@ -389,9 +391,10 @@ __global__ void __launch_bounds__(64) k_arm_persistent(ArmView V, EdgeBatchView 
     ArmCount cnt = {0, 0, 0, 0};
     int p = 0;  // pass hint
     for (;;) {
-        unsigned t = 0;
-        if (threadIdx.x == 0) t = atomicAdd(W.ticket, 1u);
-        t = __builtin_amdgcn_readfirstlane(t);
+        unsigned long long t = 0;
+        if (threadIdx.x == 0) t = atomicAdd(W.ticket, 1ull);
+        t = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(t >> 32)) << 32) |
+            (unsigned)__builtin_amdgcn_readfirstlane((unsigned)t);
         const long long s0 = (long long)t * spt;
         if (s0 >= T) break;
         for (int j = 0; j < spt; ++j) {
@@ -408,7 +411,7 @@ __global__ void __launch_bounds__(64) k_arm_persistent(ArmView V, EdgeBatchView 
             const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
             arm_load_state(V, Wl, B, ref.e, w.N, i, V.m_active[md] * V.dof);
             ++n_eval;
-            if (arm_state_collides(V, Wl, mode, B.mask, rechecks, cnt)) {
+            if (arm_state_collides<GRASP>(V, Wl, mode, B.mask, rechecks, cnt)) {
                 if (threadIdx.x == 0) B.out[ref.e] = 0;
             }
             __syncthreads();
@@ -428,6 +431,7 @@ __global__ void __launch_bounds__(64) k_arm_persistent(ArmView V, EdgeBatchView 
     }
 }
 
+template <bool GRASP>
 __global__ void __launch_bounds__(64) k_arm_direct(ArmView V, EdgeBatchView B, const int* list,
                                                    long long n_list, const long long* n_dev,
                                                    int wpe, int init_out,
@@ -461,7 +465,7 @@ __global__ void __launch_bounds__(64) k_arm_direct(ArmView V, EdgeBatchView B, c
             if (i < 0) continue;
             arm_load_state(V, Wl, B, e, N, i, V.m_active[md] * V.dof);
             ++n_eval;
-            const bool c = arm_state_collides(V, Wl, mode, B.mask, rechecks, cnt);
+            const bool c = arm_state_collides<GRASP>(V, Wl, mode, B.mask, rechecks, cnt);
             __syncthreads();
             if (c) {
                 if (threadIdx.x == 0) B.out[e] = 0;
@@ -480,6 +484,7 @@ __global__ void __launch_bounds__(64) k_arm_direct(ArmView V, EdgeBatchView B, c
     }
 }
 
+template <bool GRASP>
 __global__ void __launch_bounds__(64) k_arm_configs(ArmView V, const double* qv, long long n,
                                                     long long rs, long long ds, const int* modes,
                                                     unsigned mask, unsigned char* out) {
@@ -493,7 +498,7 @@ __global__ void __launch_bounds__(64) k_arm_configs(ArmView V, const double* qv,
         const int Dact = V.m_active[md] * V.dof;
         if ((int)threadIdx.x < Dact) Wl.q[threadIdx.x] = qv[r * rs + (long long)threadIdx.x * ds];
         __syncthreads();
-        const bool c = arm_state_collides(V, Wl, mode, mask, rechecks, cnt);
+        const bool c = arm_state_collides<GRASP>(V, Wl, mode, mask, rechecks, cnt);
         if (threadIdx.x == 0) out[r] = c ? 0 : 1;
         __syncthreads();
     }
@@ -558,7 +563,10 @@ void launch_plan_run(const SceneRec& sc, const EdgeBatchView& B, const PlanWork&
                 k_lane_persistent<PlanarView, 0, 0><<<grid, 256, 0, st>>>(sc.planar, B, W, lgG, spt);
             break;
         case KIND_ARMS:
-            k_arm_persistent<<<grid, 64, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
+            if (sc.arm_has_grasp)
+                k_arm_persistent<true><<<grid, 64, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
+            else
+                k_arm_persistent<false><<<grid, 64, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
             break;
     }
 }
@@ -592,7 +600,10 @@ void launch_direct(const SceneRec& sc, const EdgeBatchView& B, const int* list, 
                 k_lane_direct<PlanarView, 0, 0><<<grid, 64, 0, st>>>(sc.planar, B, list, n_list, n_dev, wpe, init_out, stats);
             break;
         case KIND_ARMS:
-            k_arm_direct<<<grid, 64, sc.arm_lds, st>>>(sc.arm, B, list, n_list, n_dev, wpe, init_out, stats);
+            if (sc.arm_has_grasp)
+                k_arm_direct<true><<<grid, 64, sc.arm_lds, st>>>(sc.arm, B, list, n_list, n_dev, wpe, init_out, stats);
+            else
+                k_arm_direct<false><<<grid, 64, sc.arm_lds, st>>>(sc.arm, B, list, n_list, n_dev, wpe, init_out, stats);
             break;
     }
 }
@@ -622,7 +633,10 @@ void launch_configs(const SceneRec& sc, const double* q, long long n, long long 
         case KIND_ARMS: {
             long long g = n < (long long)max_grid * 4 ? n : (long long)max_grid * 4;
             if (g < 1) g = 1;
-            k_arm_configs<<<(int)g, 64, sc.arm_lds, st>>>(sc.arm, q, n, rs, ds, modes, mask, out);
+            if (sc.arm_has_grasp)
+                k_arm_configs<true><<<(int)g, 64, sc.arm_lds, st>>>(sc.arm, q, n, rs, ds, modes, mask, out);
+            else
+                k_arm_configs<false><<<(int)g, 64, sc.arm_lds, st>>>(sc.arm, q, n, rs, ds, modes, mask, out);
             break;
         }
     }

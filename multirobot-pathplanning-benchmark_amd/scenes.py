@@ -267,3 +267,31 @@ def lognormal_edges(rng, desc: SceneDesc, n: int, resolution: float, median: flo
         q2[filled:filled + k] = b[:k]
         filled += k
     return q1, q2
+
+
+def mixed_instances(rng, desc: SceneDesc, n_instances: int, resolution: float, max_edges: int = 64,
+                    sigma_home: float = 0.4, median: float = 0.3, sigma: float = 1.0):
+    """C5 batch: instance i uses mode i (its arm count and held boxes) and is a path of U{1..64}
+    edges: the first waypoint near home, each next one a lognormal step (max-abs metric over the
+    instance's active joints).  Returns vertices [V, D], edge row indices idx1 / idx2 into them,
+    the mode of every edge, and the instance of every edge."""
+    D, dof = desc.dim, desc.dof
+    n_active = desc.modes["n_active"][:n_instances]
+    K = rng.integers(1, max_edges + 1, size=n_instances)
+    V = np.zeros((int(np.sum(K + 1)), D))
+    idx1 = np.zeros(int(np.sum(K)), dtype=np.int32)
+    inst = np.repeat(np.arange(n_instances, dtype=np.int32), K)
+    v0 = np.concatenate([[0], np.cumsum(K + 1)[:-1]])
+    e0 = np.concatenate([[0], np.cumsum(K)[:-1]])
+    for i in range(n_instances):
+        da = int(n_active[i]) * dof
+        w = np.zeros((K[i] + 1, D))
+        w[0, :da] = rng.normal(0.0, sigma_home, da)
+        L = rng.lognormal(math.log(median), sigma, size=K[i])
+        d = rng.uniform(-1, 1, size=(K[i], da))
+        d /= np.max(np.abs(d), axis=1, keepdims=True)
+        w[1:, :da] = w[0, :da] + np.cumsum(L[:, None] * d, axis=0)
+        w = np.clip(w, desc.limits[0], desc.limits[1])
+        V[v0[i]:v0[i] + K[i] + 1] = w
+        idx1[e0[i]:e0[i] + K[i]] = np.arange(v0[i], v0[i] + K[i], dtype=np.int32)
+    return V, idx1, idx1 + 1, inst.copy(), inst
