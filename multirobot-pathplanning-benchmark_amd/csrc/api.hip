@@ -611,7 +611,7 @@ int mrcc_edge_plan_create(const mrcc_scene* scene, const mrcc_edge_batch* batch,
     memset(&t, 0, sizeof(t));
     if (tune) t = *tune;
     const bool arms = sc->kind == KIND_ARMS;
-    if (t.slots_per_pass <= 0) t.slots_per_pass = arms ? 4 : 8;
+    if (t.slots_per_pass <= 0) t.slots_per_pass = arms ? 1 : 8;
     if (t.slots_per_ticket <= 0) t.slots_per_ticket = 1;
     if (t.waves_per_cu <= 0) t.waves_per_cu = arms ? 16 : 16;
     if (!tune) {

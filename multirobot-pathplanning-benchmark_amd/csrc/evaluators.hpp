@@ -454,7 +454,56 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int
     bool hit = false;
     if (V.broadphase) {
         const float Mc = V.Mcull;
-        // 4'. links vs boxes, then the spheres of candidate (link, box) pairs: 8 candidates x 8
+        // 4'. link pairs of the pair set, then the sphere pairs of candidate link pairs (8 x 8);
+        //     first, since self and inter-arm contacts dominate collisions of random states
+        const int NP = V.lpair_count[na];
+        for (int p0 = 0; p0 < NP; p0 += 64) {
+            const int pi = p0 + lane;
+            bool cand = false;
+            int v = 0;
+            if (pi < NP) {
+                v = V.lpairs[pi];
+                const int l = v >> 8, mm = v & 255;
+                if (((mask >> (l / dof)) & 1u) && ((mask >> (mm / dof)) & 1u)) {
+                    const float4 A = W.linkC[l], Bc = W.linkC[mm];
+                    const float dx = A.x - Bc.x, dy = A.y - Bc.y, dz = A.z - Bc.z;
+                    const float R = A.w + Bc.w;
+                    cand = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) <= R * R + Mc;
+                    ++cnt.lp;
+                }
+            }
+            const unsigned long long m = __ballot(cand);
+            if (m == 0ull) continue;
+            if (cand) W.cand[lanes_below(m)] = v;
+            __syncthreads();
+            const int n = __popcll(m);
+            for (int c = 0; c < n; ++c) {
+                const int w = W.cand[c];
+                const int l = w >> 8, mm = w & 255;
+                const int fl = V.link_first[l], nl = V.link_first[l + 1] - fl;
+                const int fm = V.link_first[mm], nm = V.link_first[mm + 1] - fm;
+                const int i = fl + (lane >> 3), j = fm + (lane & 7);
+                if ((lane >> 3) < nl && (lane & 7) < nm) {
+                    const float4 ci = W.sph[i], cj = W.sph[j];
+                    const float dx = ci.x - cj.x, dy = ci.y - cj.y, dz = ci.z - cj.z;
+                    const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    const float R = ci.w + cj.w;
+                    const float T = R * R;
+                    ++cnt.sp;
+                    if (d2 <= T + M) {
+                        if (d2 < T - M) hit = true;
+                        else {
+                            ++rechecks;
+                            if (arm_recheck_sph_sph(V, W, i, j)) hit = true;
+                        }
+                    }
+                }
+                if ((c & 3) == 3 && __any(hit)) return true;
+            }
+            __syncthreads();
+            if (__any(hit)) return true;
+        }
+        // 5'. links vs boxes, then the spheres of candidate (link, box) pairs: 8 candidates x 8
         //     spheres per wave iteration
         for (int l = 0; l < Dact; ++l) {
             if (!((mask >> (l / dof)) & 1u)) continue;
@@ -503,54 +552,6 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int
                 __syncthreads();
                 if (__any(hit)) return true;
             }
-        }
-        // 5'. link pairs of the pair set, then the sphere pairs of candidate link pairs (8 x 8)
-        const int NP = V.lpair_count[na];
-        for (int p0 = 0; p0 < NP; p0 += 64) {
-            const int pi = p0 + lane;
-            bool cand = false;
-            int v = 0;
-            if (pi < NP) {
-                v = V.lpairs[pi];
-                const int l = v >> 8, mm = v & 255;
-                if (((mask >> (l / dof)) & 1u) && ((mask >> (mm / dof)) & 1u)) {
-                    const float4 A = W.linkC[l], Bc = W.linkC[mm];
-                    const float dx = A.x - Bc.x, dy = A.y - Bc.y, dz = A.z - Bc.z;
-                    const float R = A.w + Bc.w;
-                    cand = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) <= R * R + Mc;
-                    ++cnt.lp;
-                }
-            }
-            const unsigned long long m = __ballot(cand);
-            if (m == 0ull) continue;
-            if (cand) W.cand[lanes_below(m)] = v;
-            __syncthreads();
-            const int n = __popcll(m);
-            for (int c = 0; c < n; ++c) {
-                const int w = W.cand[c];
-                const int l = w >> 8, mm = w & 255;
-                const int fl = V.link_first[l], nl = V.link_first[l + 1] - fl;
-                const int fm = V.link_first[mm], nm = V.link_first[mm + 1] - fm;
-                const int i = fl + (lane >> 3), j = fm + (lane & 7);
-                if ((lane >> 3) < nl && (lane & 7) < nm) {
-                    const float4 ci = W.sph[i], cj = W.sph[j];
-                    const float dx = ci.x - cj.x, dy = ci.y - cj.y, dz = ci.z - cj.z;
-                    const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    const float R = ci.w + cj.w;
-                    const float T = R * R;
-                    ++cnt.sp;
-                    if (d2 <= T + M) {
-                        if (d2 < T - M) hit = true;
-                        else {
-                            ++rechecks;
-                            if (arm_recheck_sph_sph(V, W, i, j)) hit = true;
-                        }
-                    }
-                }
-                if ((c & 3) == 3 && __any(hit)) return true;
-            }
-            __syncthreads();
-            if (__any(hit)) return true;
         }
     } else {
     // 4. spheres vs boxes
