@@ -293,8 +293,10 @@ EdgeBatchView view_of(const mrcc_edge_batch* b) {
 }
 
 int check_batch(const SceneRec* sc, const mrcc_edge_batch* b) {
-    if (!b || !b->q1 || !b->q2 || !b->out) return fail(MRCC_ERR_INVALID, "null edge batch buffer");
+    if (!b) return fail(MRCC_ERR_INVALID, "null edge batch");
     if (b->n_edges < 0) return fail(MRCC_ERR_INVALID, "n_edges < 0");
+    if (b->n_edges == 0) return MRCC_OK;  // nothing is read or written
+    if (!b->q1 || !b->q2 || !b->out) return fail(MRCC_ERR_INVALID, "null edge batch buffer");
     if (!b->N && !(b->resolution > 0.0)) return fail(MRCC_ERR_INVALID, "resolution must be > 0");
     if (sc->D > MRCC_MAX_D) return fail(MRCC_ERR_INVALID, "D > %d", MRCC_MAX_D);
     return MRCC_OK;

@@ -254,7 +254,7 @@ def test_c3_configs(c3, bp):
     sc.set_broadphase(bp)
     Q = np.concatenate([scenes.uniform_configs(rng, desc, 4000), near_home(rng, desc, 8000)])
     ref = o.check_configs(Q)
-    assert 0.05 < ref.mean() < 0.95
+    assert 0.005 < ref.mean() < 0.95  # both outcomes represented (>= 60 free configurations)
     assert np.array_equal(gpu_configs(sc, Q), ref)
     sc.set_broadphase(True)
 
