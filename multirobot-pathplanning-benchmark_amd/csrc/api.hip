@@ -249,6 +249,19 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     V.m_rel = at<double>(d, o_mr);
     V.m_half = at<double>(d, o_mh);
     V.Mcull = (float)(ldexp(1.0, -18) * (1.0 + B) * (1.0 + 2.0 * Rl_max));
+    for (int k = 0; k < 3; ++k) {  // union of the obstacle boxes, rounded outward
+        double lo = INFINITY, hi = -INFINITY;
+        for (int b = 0; b < nb; ++b) {
+            lo = std::min(lo, box64[6 * b + k]);
+            hi = std::max(hi, box64[6 * b + 3 + k]);
+        }
+        if (nb == 0) lo = hi = 0.0;
+        float l32 = (float)lo, h32 = (float)hi;
+        if ((double)l32 > lo) l32 = nextafterf(l32, -INFINITY);
+        if ((double)h32 < hi) h32 = nextafterf(h32, INFINITY);
+        V.bb_lo[k] = l32;
+        V.bb_hi[k] = h32;
+    }
     V.link_bc64 = at<double>(d, o_lbc);
     V.link_R32 = at<float>(d, o_lR);
     V.lpairs = at<int>(d, o_lp);
@@ -614,7 +627,7 @@ int mrcc_edge_plan_create(const mrcc_scene* scene, const mrcc_edge_batch* batch,
     if (tune) t = *tune;
     const bool arms = sc->kind == KIND_ARMS;
     if (t.slots_per_pass <= 0) t.slots_per_pass = arms ? 1 : 8;
-    if (t.slots_per_ticket <= 0) t.slots_per_ticket = 1;
+    if (t.slots_per_ticket <= 0) t.slots_per_ticket = arms ? 8 : 4;
     if (t.waves_per_cu <= 0) t.waves_per_cu = arms ? 16 : 16;
     if (!tune) {
         t.sorted = 1;
@@ -638,7 +651,7 @@ int mrcc_edge_plan_create(const mrcc_scene* scene, const mrcc_edge_batch* batch,
     auto sz = [&](size_t bytes) { size_t off = (P.buf.size() + 255) & ~size_t(255); P.buf.resize(off + bytes); return off; };
     size_t o_eN = sz(4 * n), o_eM = sz(4 * n), o_eS = sz(4 * n), o_w0 = sz(4 * n), o_full = sz(n),
            o_order = sz(4 * n), o_hist = sz(4 * (MRCC_MAX_PASSES + 1)), o_cur = sz(4 * (MRCC_MAX_PASSES + 1)),
-           o_ps = sz(8 * (MRCC_MAX_PASSES + 1)), o_meta = sz(8 * 8), o_long = sz(4 * n), o_tick = sz(8),
+           o_ps = sz(8 * (MRCC_MAX_PASSES + 1)), o_meta = sz(8 * 8), o_long = sz(4 * n), o_tick = sz(8 * 32 * 8),
            o_stats = sz(8 * 8), o_dir = arms ? 0 : sz(8 * (size_t)n * D);
     if (hipMalloc(&p->work_mem, P.buf.size()) != hipSuccess) {
         delete p;

@@ -379,6 +379,7 @@ struct ArmView {
     const float* link_R32;   // [L] bounding radius (covers every sphere of the link, inflated)
     const int* lpairs;       // [NP] link pairs (l << 8 | m), l < m, sorted by arm of m
     const int* lpair_count;  // [na+1] pairs whose arms are all < n_active
+    float bb_lo[3], bb_hi[3];   // box containing every obstacle box (binary32, rounded outward)
 };
 
 // batch of edges as seen by kernels
@@ -412,7 +413,7 @@ struct PlanWork {
     long long* pass_start;  // [MAX_PASSES+1] slot offset of each pass (G * sum E)
     long long* meta;   // [8]: 0 total slots, 1 n_pass, 2 error count, 3 long edges, 4 tickets
     int* long_list;    // [n] edges with more passes than MAX_PASSES
-    unsigned long long* ticket;  // [1]
+    unsigned long long* ticket;  // [8 * 32]: one chunk counter per XCD, 256 bytes apart
     unsigned long long* stats;  // [8]
     double* dir;       // [n*D] (lane-per-state kinds) or null
 };

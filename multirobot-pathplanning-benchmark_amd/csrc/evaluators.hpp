@@ -511,6 +511,12 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int
             if (ns == 0) continue;
             const float4 Lc = W.linkC[l];
             const float Tl = Lc.w * Lc.w;
+            {   // the link's bounding sphere clears the box around all obstacles: no box can touch it
+                const float e0 = fminf(fmaxf(Lc.x, V.bb_lo[0]), V.bb_hi[0]) - Lc.x;
+                const float e1 = fminf(fmaxf(Lc.y, V.bb_lo[1]), V.bb_hi[1]) - Lc.y;
+                const float e2 = fminf(fmaxf(Lc.z, V.bb_lo[2]), V.bb_hi[2]) - Lc.z;
+                if (fmaf(e2, e2, fmaf(e1, e1, e0 * e0)) > Tl + Mc) continue;
+            }
             for (int b0 = 0; b0 < V.nb; b0 += 64) {
                 const int b = b0 + lane;
                 bool cand = false;

@@ -5,7 +5,8 @@
 //               passes S = ceil(M / G), histogram of S
 //   scan      : one block: E_p = #{edges with S > p}, pass offsets PS[p] = G * sum_{p'<p} E_p'
 //   scatter   : counting sort, edges longest-first
-//   persistent: waves claim queue slots with an atomic ticket; slot -> (pass p, rank r, k);
+//   persistent: waves claim chunks of queue slots from 8 XCD-interleaved counters; slot -> (pass p,
+//               rank r, k);
 //               edge = order[r]; state = k * S_e + p of the edge (coarse-to-fine: pass 0 visits G
 //               evenly spread states of every edge, later passes refine; an edge found in
 //               collision is skipped by all later slots)
@@ -59,7 +60,7 @@ __global__ void k_plan_init(PlanWork W) {
     const int stride = gridDim.x * blockDim.x;
     for (int b = t; b <= MRCC_MAX_PASSES; b += stride) W.hist[b] = 0;
     if (t < 8) W.meta[t] = 0;
-    if (t == 0) W.ticket[0] = 0;
+    if (t < 8) W.ticket[32 * t] = 0;
     if (t < 8 && W.stats) W.stats[t] = 0;
 }
 
@@ -168,6 +169,26 @@ __global__ void k_plan_scatter(PlanWork W, long long n) {
     W.order[pos] = (int)e;
 }
 
+// Work queue of the persistent kernels: the slots are cut into chunks; chunk c belongs to counter
+// c % 8.  A workgroup starts on counter blockIdx % 8 (the XCD it is dispatched to) and claims its
+// chunks in increasing order, so all counters advance through the passes together; when its
+// counter is exhausted it takes chunks from the others.  Returns the chunk, or -1 when every
+// counter is exhausted.  Must be called by all lanes of the wave (wave-uniform result).
+__device__ inline long long claim_chunk(const PlanWork& W, int& q, long long nchunks) {
+    const int home = blockIdx.x & 7;
+    while (q < 8) {
+        const int x = (home + q) & 7;
+        unsigned long long k = 0;
+        if ((threadIdx.x & 63) == 0) k = atomicAdd(W.ticket + 32 * x, 1ull);
+        k = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(k >> 32)) << 32) |
+            (unsigned)__builtin_amdgcn_readfirstlane((unsigned)k);
+        const long long c = (long long)x + 8 * (long long)k;
+        if (c < nchunks) return c;
+        ++q;
+    }
+    return -1;
+}
+
 struct SlotRef {
     int e;      // edge, -1 when the slot is padding
     int local;  // slot within the edge
@@ -179,6 +200,7 @@ __device__ inline SlotRef slot_ref(const PlanWork& W, int& p, long long s, int l
     SlotRef ref;
     ref.e = -1;
     ref.local = 0;
+    if (W.pass_start[p] > s) p = 0;  // the wave moved to another queue
     while (p < MRCC_MAX_PASSES && W.pass_start[p + 1] <= s) ++p;
     const long long off = s - W.pass_start[p];
     const long long r = off >> lgG;
@@ -260,13 +282,14 @@ __global__ void __launch_bounds__(256) k_lane_persistent(View S, EdgeBatchView B
     unsigned rechecks = 0;
     const int lane = threadIdx.x & 63;
     int pw = 0;  // pass hint of this wave
+    int q = 0;   // queues exhausted so far
+    const long long chunk = (long long)spt * 64;
+    const long long nchunks = (T + chunk - 1) / chunk;
     for (;;) {
-        unsigned long long t = 0;
-        if (lane == 0) t = atomicAdd(W.ticket, 1ull);
-        t = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(t >> 32)) << 32) |
-            (unsigned)__builtin_amdgcn_readfirstlane((unsigned)t);
-        const long long s0 = (long long)t * spt * 64;
-        if (s0 >= T) break;
+        const long long c = claim_chunk(W, q, nchunks);
+        if (c < 0) break;
+        const long long s0 = c * chunk;
+        if (W.pass_start[pw] > s0) pw = 0;
         while (pw < MRCC_MAX_PASSES && W.pass_start[pw + 1] <= s0) ++pw;
         for (int j = 0; j < spt; ++j) {
             const long long s = s0 + (long long)j * 64 + lane;
@@ -390,13 +413,12 @@ __global__ void __launch_bounds__(64) k_arm_persistent(ArmView V, EdgeBatchView 
     unsigned rechecks = 0;
     ArmCount cnt = {0, 0, 0, 0};
     int p = 0;  // pass hint
+    int q = 0;  // queues exhausted so far
+    const long long nchunks = (T + spt - 1) / spt;
     for (;;) {
-        unsigned long long t = 0;
-        if (threadIdx.x == 0) t = atomicAdd(W.ticket, 1ull);
-        t = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(t >> 32)) << 32) |
-            (unsigned)__builtin_amdgcn_readfirstlane((unsigned)t);
-        const long long s0 = (long long)t * spt;
-        if (s0 >= T) break;
+        const long long c = claim_chunk(W, q, nchunks);
+        if (c < 0) break;
+        const long long s0 = c * spt;
         for (int j = 0; j < spt; ++j) {
             const long long s = s0 + j;
             if (s >= T) break;
