@@ -55,6 +55,66 @@ def logical_states(q1, q2, res):
     return int(np.sum(N - 2)), N
 
 
+def roofline_all_free(args, backend, sc, desc, rng, dev, res, fl_state, pair_info):
+    """Roofline workload of the persistent arm kernel: edges whose every state is free, so no
+    state exits early and the kernel runs its whole (broad phase + narrow phase) test set.
+    The kernel time comes from HIP events recorded around the kernel on the run's stream."""
+    import torch
+
+    stream = torch.cuda.current_stream()
+    # every state of these edges is free, so no early exit: the kernel runs its whole pair set.
+    fq1, fq2 = free_edges(rng, desc, sc, args.edges, res, dev)
+    f_states, fN = logical_states(fq1, fq2, res)
+    fa, fb = torch.from_numpy(fq1).to(dev), torch.from_numpy(fq2).to(dev)
+    fout = torch.empty(len(fq1), dtype=torch.uint8, device=dev)
+    fbatch = sc.edge_batch(fa, fb, fout, res)
+
+    def timed(plan):
+        plan.run()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            plan.run()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
+    # (a) as shipped (link broad phase on): executed tests counted by the kernel
+    f_ms = timed(backend.EdgePlan(sc, fbatch, backend.make_tune(use_graph=1)))
+    f_split = timed_split(backend.EdgePlan(sc, fbatch, backend.make_tune(use_graph=0, time_kernels=1)),
+                          args.steps)
+    k_ms = f_split["kernel_ms"]
+    assert fout.cpu().numpy().all(), "all-free workload reported a collision"
+    cst = backend.EdgePlan(sc, fbatch, backend.make_tune(use_graph=0, collect_stats=1))
+    cst.run()
+    fst = cst.stats()
+    del cst
+    S_, L_, D_ = pair_info["spheres"], desc.n_arms * desc.dof, desc.dim
+    per_state_fixed = 2 * D_ + 72 * D_ + 18 * S_ + 18 * L_
+    exec_flops = (fst["states"] * per_state_fixed + 15 * fst["link_box_tests"] + 15 * fst["sphere_box_tests"]
+                  + 11 * fst["link_pair_tests"] + 11 * fst["sphere_pair_tests"])
+    achieved = exec_flops / (k_ms * 1e-3) / 1e12
+    free_rate = f_states / (f_ms * 1e-3)
+    log(f"all-free (broad phase): {len(fq1)} edges, {f_states} states, {f_ms:.3f} ms/run "
+        f"(persistent kernel {k_ms:.3f} ms) -> {free_rate:.4g} states/s, "
+        f"executed {exec_flops / max(f_states, 1):.0f} flop/state -> {achieved:.2f} TFLOP/s "
+        f"({achieved / PEAK_FP32_TFLOPS * 100:.1f}% of FP32 VALU peak); stats {fst}")
+    # (b) brute force: the full pair set of every state
+    sc.set_broadphase(False)
+    bf_split = timed_split(backend.EdgePlan(sc, fbatch, backend.make_tune(use_graph=0, time_kernels=1)),
+                           args.steps)
+    sc.set_broadphase(True)
+    bf_ms = bf_split["kernel_ms"]
+    bf_achieved = f_states * fl_state / (bf_ms * 1e-3) / 1e12
+    log(f"all-free (brute force): {bf_ms:.3f} ms -> {bf_achieved:.2f} TFLOP/s "
+        f"({bf_achieved / PEAK_FP32_TFLOPS * 100:.1f}% of FP32 VALU peak)")
+
+    return {"achieved": achieved, "k_ms": k_ms, "f_ms": f_ms, "f_split": f_split, "exec_flops": exec_flops,
+            "f_states": f_states, "free_rate": free_rate, "n_edges": len(fq1), "bf_achieved": bf_achieved,
+            "bf_ms": bf_ms, "fst": fst}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +127,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-secondary", action="store_true")
     ap.add_argument("--mixed-instances", type=int, default=65536)
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the roofline workload of the persistent kernel (for profiling runs)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one device copy of 256 MiB (calibrates FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -98,6 +160,16 @@ def main():
     n_states, N = logical_states(q1, q2, res)
     log(f"workload: {args.edges} edges, {n_states} logical edge-states (mean N {N.mean():.2f}), "
         f"{fl_state} flops/state ({pair_info})")
+    if args.roofline_only:  # same random stream as the full run up to this point
+        rf = roofline_all_free(args, backend, sc, desc, rng, dev, res, fl_state, pair_info)
+        print(json.dumps({"roofline_only": True, "kernel": "k_arm_persistent (all-free edges)",
+                          "ms_per_launch": rf["k_ms"], "run_split_ms": rf["f_split"],
+                          "achieved_tflops": rf["achieved"], "flops_per_launch": rf["exec_flops"],
+                          "states_per_launch": rf["f_states"], "brute_force_ms": rf["bf_ms"],
+                          "stats": rf["fst"]}), flush=True)
+        if args.copy_probe:
+            copy_probe(dev)
+        return
     a = torch.from_numpy(q1).to(dev)
     b = torch.from_numpy(q2).to(dev)
     out = torch.empty(args.edges, dtype=torch.uint8, device=dev)
@@ -126,6 +198,10 @@ def main():
     splan.run()
     stats = splan.stats()
     del splan
+    # queue build / persistent kernel split of a run (HIP events on the run's stream, no graph)
+    head_split = timed_split(backend.EdgePlan(sc, batch, backend.make_tune(use_graph=0, time_kernels=1)),
+                             args.steps)
+    log(f"headline run split: {head_split}")
 
     # ---- unsorted single launch (ablation) ---------------------------------------------------
     out2 = torch.empty_like(out)
@@ -171,47 +247,8 @@ def main():
     log(f"configs: {cfg_rate:.4g} checks/s, {cfg_free.mean():.4f} free")
 
     # ---- all-free variant: roofline of the persistent kernel ----------------------------------
-    # every state of these edges is free, so no early exit: the kernel runs its whole pair set.
-    fq1, fq2 = free_edges(rng, desc, sc, args.edges, res, dev)
-    f_states, fN = logical_states(fq1, fq2, res)
-    fa, fb = torch.from_numpy(fq1).to(dev), torch.from_numpy(fq2).to(dev)
-    fout = torch.empty(len(fq1), dtype=torch.uint8, device=dev)
-    fbatch = sc.edge_batch(fa, fb, fout, res)
-
-    def timed(plan):
-        plan.run()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(args.steps):
-            plan.run()
-        e1.record(stream)
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / args.steps
-
-    # (a) as shipped (link broad phase on): executed tests counted by the kernel
-    f_ms = timed(backend.EdgePlan(sc, fbatch, backend.make_tune(use_graph=1)))
-    assert fout.cpu().numpy().all(), "all-free workload reported a collision"
-    cst = backend.EdgePlan(sc, fbatch, backend.make_tune(use_graph=0, collect_stats=1))
-    cst.run()
-    fst = cst.stats()
-    del cst
-    S_, L_, D_ = pair_info["spheres"], desc.n_arms * desc.dof, desc.dim
-    per_state_fixed = 2 * D_ + 72 * D_ + 18 * S_ + 18 * L_
-    exec_flops = (fst["states"] * per_state_fixed + 15 * fst["link_box_tests"] + 15 * fst["sphere_box_tests"]
-                  + 11 * fst["link_pair_tests"] + 11 * fst["sphere_pair_tests"])
-    achieved = exec_flops / (f_ms * 1e-3) / 1e12
-    free_rate = f_states / (f_ms * 1e-3)
-    log(f"all-free (broad phase): {len(fq1)} edges, {f_states} states, {f_ms:.3f} ms -> {free_rate:.4g} states/s, "
-        f"executed {exec_flops / max(f_states, 1):.0f} flop/state -> {achieved:.2f} TFLOP/s "
-        f"({achieved / PEAK_FP32_TFLOPS * 100:.1f}% of FP32 VALU peak); stats {fst}")
-    # (b) brute force: the full pair set of every state
-    sc.set_broadphase(False)
-    bf_ms = timed(backend.EdgePlan(sc, fbatch, backend.make_tune(use_graph=1)))
-    sc.set_broadphase(True)
-    bf_achieved = f_states * fl_state / (bf_ms * 1e-3) / 1e12
-    log(f"all-free (brute force): {bf_ms:.3f} ms -> {bf_achieved:.2f} TFLOP/s "
-        f"({bf_achieved / PEAK_FP32_TFLOPS * 100:.1f}% of FP32 VALU peak)")
+    rf = roofline_all_free(args, backend, sc, desc, rng, dev, res, fl_state, pair_info)
+    achieved, k_ms, f_ms = rf["achieved"], rf["k_ms"], rf["f_ms"]
 
     # ---- CPU baseline: the oracle on the host cores -------------------------------------------
     cpu = None
@@ -250,11 +287,7 @@ def main():
 
     secondary = {} if args.skip_secondary else secondary_workloads(args, backend, scenes, dev)
     if args.copy_probe:
-        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log("copy probe: 268435456 bytes read, 268435456 bytes written")
+        copy_probe(dev)
 
     line = {
         "metric": "edge-state collision checks/sec (bit-exact), batched persistent, 4x7-DoF arms",
@@ -275,15 +308,18 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
                      "kernel": "k_arm_persistent, all-free variant (no early exit), link broad phase on",
-                     "flops_per_launch": exec_flops, "states_per_launch": f_states, "ms_per_launch": f_ms,
-                     "brute_force": {"achieved": bf_achieved, "frac": bf_achieved / PEAK_FP32_TFLOPS,
-                                     "flops_per_state": fl_state, "ms_per_launch": bf_ms}},
+                     "flops_per_launch": rf["exec_flops"], "states_per_launch": rf["f_states"],
+                     "ms_per_launch": k_ms, "run_split_ms": rf["f_split"],
+                     "brute_force": {"achieved": rf["bf_achieved"], "frac": rf["bf_achieved"] / PEAK_FP32_TFLOPS,
+                                     "flops_per_state": fl_state, "ms_per_launch": rf["bf_ms"]}},
         "cpu_baseline": cpu,
         "plain_per_call": {"value": per_call_rate, "unit": "edge-states/s", "edges": m, "matches": per_call_match},
         "single_launch": {"value": single_rate, "unit": "edge-states/s", "matches": single_match},
-        "all_free": {"value": free_rate, "unit": "edge-states/s", "edges": len(fq1), "edge_states": f_states},
+        "all_free": {"value": rf["free_rate"], "unit": "edge-states/s", "edges": rf["n_edges"],
+                     "edge_states": rf["f_states"]},
         "configs": {"value": cfg_rate, "unit": "configs/s", "n": len(Qc), "free_frac": float(cfg_free.mean())},
         "stats": stats,
+        "run_split_ms": head_split,
         "mismatches": None if mism is None else mism + sum(v.get("mismatches", 0) for v in secondary.values()),
         "checks_compared": checked + sum(v.get("checks_compared", 0) for v in secondary.values()),
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
@@ -291,6 +327,30 @@ def main():
         "secondary": secondary,
     }
     print(json.dumps(line), flush=True)
+
+
+def copy_probe(dev):
+    """One device copy of 256 MiB: its FETCH_SIZE / WRITE_SIZE readings calibrate the counters."""
+    import torch
+
+    src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log("copy probe: 268435456 bytes read, 268435456 bytes written")
+
+
+def timed_split(plan, steps):
+    """Mean queue-build / main-kernel / whole-run milliseconds over `steps` runs of a plan made
+    with time_kernels=1 (HIP events recorded by the library on the run's stream)."""
+    plan.run()
+    acc = {"build_ms": 0.0, "kernel_ms": 0.0, "run_ms": 0.0}
+    for _ in range(steps):
+        plan.run()
+        t = plan.timing()
+        for k in acc:
+            acc[k] += t[k] / steps
+    return acc
 
 
 def secondary_workloads(args, backend, scenes, dev):
@@ -392,7 +452,59 @@ def secondary_workloads(args, backend, scenes, dev):
         ent["checks_compared"] = int(np.sum(N[:k] - 2))
     out["c5_mixed_arms"] = ent
     log(f"c5_mixed_arms: {ent}")
+    out["c4_clearance"] = clearance_workload(args, backend, scenes, dev, threads)
     return out
+
+
+def clearance_workload(args, backend, scenes, dev, threads):
+    """C4: clearance cost + hand-written backward on 4096 paths x 64 waypoints x 28 DoF (FP32),
+    gradient checked against FP64 central differences (h = 1e-6) of the oracle on 64 paths."""
+    import torch
+    from oracle import oracle as O
+
+    clr = importlib.import_module(f"{PKG}.clearance")
+    rng = np.random.default_rng(4)
+    desc = scenes.quad_arms(rng=rng)
+    sc = backend.Scene(desc)
+    P, W, D = 4096, 64, desc.dim
+    a = np.clip(rng.normal(0, 0.4, (P, D)), -2.9, 2.9)
+    b = np.clip(rng.normal(0, 0.4, (P, D)), -2.9, 2.9)
+    t = np.linspace(0, 1, W)[None, :, None]
+    paths = np.clip(a[:, None] + t * (b - a)[:, None] + rng.normal(0, 0.1, (P, W, D)), -2.9, 2.9)
+    q = torch.from_numpy(paths.astype(np.float32)).to(dev).requires_grad_(True)
+    for _ in range(2):
+        c = clr.clearance_cost(sc, q, 0.02)
+        c.sum().backward()
+        q.grad = None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        c = clr.clearance_cost(sc, q, 0.02)
+        c.sum().backward()
+    torch.cuda.synchronize()
+    fb = (time.perf_counter() - t0) / reps
+    g = (q.grad / reps).detach().cpu().numpy().astype(np.float64)
+    ent = {"paths": P, "waypoints": P * W, "dof": D, "ms_forward_backward": fb * 1e3,
+           "waypoints_per_s": P * W / fb, "mean_cost": float(c.detach().mean())}
+    if not args.skip_cpu:
+        o = O.Scene(desc)
+        pick = rng.choice(P, 64, replace=False)
+        q64 = paths.astype(np.float32).astype(np.float64)[pick]          # [64, W, D]
+        h = 1e-6
+        base = q64.reshape(-1, D)
+        E = np.eye(D)
+        plus = (base[:, None, :] + h * E[None]).reshape(-1, D)
+        minus = (base[:, None, :] - h * E[None]).reshape(-1, D)
+        fd = ((o.clearance_cost_batch(plus, 0.02, threads) - o.clearance_cost_batch(minus, 0.02, threads))
+              / (2 * h)).reshape(64, W * D)
+        gp = g[pick].reshape(64, W * D)
+        rel = np.linalg.norm(gp - fd, axis=1) / np.maximum(np.linalg.norm(fd, axis=1), 1e-12)
+        ent["grad_rel_err_max"] = float(rel[np.linalg.norm(fd, axis=1) > 1e-9].max()) if np.any(np.linalg.norm(fd, axis=1) > 1e-9) else None
+        ent["grad_rel_err_median"] = float(np.median(rel))
+        ent["fd_paths"] = 64
+    log(f"c4_clearance: {ent}")
+    return ent
 
 
 def free_edges(rng, desc, sc, n, res, dev):

@@ -83,7 +83,8 @@ int mrcc_scene_create_planar_boxes(int n_robots, const double* robot_half_extent
  * mr_vamp_env.py:682-689).  base: [n_arms,12] = row-major 3x3 rotation then translation.
  * dh: [n_arms,dof,3] = (d, a, alpha).  link_sphere_count: [n_arms*dof]; spheres are listed
  * arm-major then link-major in sphere_local [S,4] = (x, y, z, r) in the link frame.
- * boxes: [n_boxes,6] = (center xyz, half-extents xyz).  n_arms <= 8, dof <= 8, 4..8 spheres/link. */
+ * boxes: [n_boxes,6] = (center xyz, half-extents xyz).  n_arms <= 8, dof <= 8, 0..8 spheres/link (at most 512),
+ * n_boxes <= 65535. */
 int mrcc_scene_create_arms(int n_arms, int dof, const double* base, const double* dh,
                            const int32_t* link_sphere_count, const double* sphere_local,
                            int n_boxes, const double* boxes, mrcc_scene** out);
@@ -139,13 +140,15 @@ typedef struct mrcc_edge_batch {
 /* Execution shape of the batched edge check (chosen by the autotuner when NULL / zero). */
 typedef struct mrcc_tune {
     int32_t slots_per_pass;    /* states of one edge issued per pass (coarse-to-fine order) */
-    int32_t slots_per_ticket;  /* queue slots a wave claims per atomic ticket */
+    int32_t slots_per_ticket;  /* queue slots a wave claims per atomic ticket (arm scenes:
+                                  <= 64, one lane resolves one slot) */
     int32_t waves_per_cu;      /* persistent grid = waves_per_cu * CUs */
     int32_t sorted;            /* 1: edges ordered longest-first */
     int32_t persistent;        /* 1: persistent work-queue kernel; 0: one workgroup per edge */
     int32_t use_graph;         /* 1: capture the kernel sequence in a hipGraph at plan creation */
     int32_t collect_stats;     /* 1: count evaluated states and FP64 rechecks */
-    int32_t reserved;
+    int32_t time_kernels;      /* 1: HIP events around the queue build and the main kernel of
+                                  each run (plans without a graph); see mrcc_edge_plan_timing */
 } mrcc_tune;
 
 /* Plan = workspace + (optionally) an instantiated hipGraph bound to the batch's buffers.
@@ -157,6 +160,10 @@ int mrcc_edge_plan_run(mrcc_edge_plan* plan, void* stream);
  * collision), queue slots issued, and for arm scenes the tests executed: link-box, sphere-box,
  * link-pair, sphere-pair} since plan creation (zeros unless collect_stats). */
 int mrcc_edge_plan_stats(mrcc_edge_plan* plan, int64_t* out8);
+/* Synchronises; out3 = {queue build, main kernel, whole run} in milliseconds of the last run,
+ * from HIP events recorded on the run's stream.  Needs time_kernels = 1 and use_graph = 0
+ * (MRCC_ERR_UNSUPPORTED otherwise).  No counterpart in the reference: instrumentation. */
+int mrcc_edge_plan_timing(mrcc_edge_plan* plan, double* out3);
 int mrcc_edge_plan_destroy(mrcc_edge_plan* plan);
 /* One-shot batched check (plan + run + destroy; synchronises). */
 int mrcc_check_edges(const mrcc_scene* scene, const mrcc_edge_batch* batch,

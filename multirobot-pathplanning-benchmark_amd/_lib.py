@@ -48,7 +48,7 @@ class EdgeBatch(_c.Structure):
 class Tune(_c.Structure):
     _fields_ = [("slots_per_pass", _i32), ("slots_per_ticket", _i32), ("waves_per_cu", _i32),
                 ("sorted", _i32), ("persistent", _i32), ("use_graph", _i32),
-                ("collect_stats", _i32), ("reserved", _i32)]
+                ("collect_stats", _i32), ("time_kernels", _i32)]
 
 
 # name -> (restype, argtypes); every symbol of include/mrcc.h
@@ -71,6 +71,7 @@ SIGNATURES = {
                                          _c.POINTER(_vp)]),
     "mrcc_edge_plan_run": (_c.c_int, [_vp, _vp]),
     "mrcc_edge_plan_stats": (_c.c_int, [_vp, _c.POINTER(_i64)]),
+    "mrcc_edge_plan_timing": (_c.c_int, [_vp, _dp]),
     "mrcc_edge_plan_destroy": (_c.c_int, [_vp]),
     "mrcc_check_edges": (_c.c_int, [_vp, _c.POINTER(EdgeBatch), _c.POINTER(Tune), _vp]),
     "mrcc_check_edge": (_c.c_int, [_vp, _vp, _vp, _c.c_double, _i32, _i32, _i32, _i32, _i32, _u32, _vp, _vp]),

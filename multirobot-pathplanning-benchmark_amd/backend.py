@@ -154,6 +154,13 @@ class EdgePlan:
                 "link_box_tests": s[4], "sphere_box_tests": s[5], "link_pair_tests": s[6],
                 "sphere_pair_tests": s[7]}
 
+    def timing(self):
+        """Milliseconds of the last run (plans made with time_kernels=1 and use_graph=0):
+        queue build, main kernel, whole run; from HIP events on the run's stream."""
+        t = (ctypes.c_double * 3)()
+        check(lib().mrcc_edge_plan_timing(self.handle, t))
+        return {"build_ms": t[0], "kernel_ms": t[1], "run_ms": t[2]}
+
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value:
@@ -165,10 +172,11 @@ class EdgePlan:
 
 
 def make_tune(slots_per_pass=0, slots_per_ticket=0, waves_per_cu=0, sorted=1, persistent=1,
-              use_graph=1, collect_stats=0) -> Tune:
+              use_graph=1, collect_stats=0, time_kernels=0) -> Tune:
     t = Tune()
     t.slots_per_pass, t.slots_per_ticket, t.waves_per_cu = slots_per_pass, slots_per_ticket, waves_per_cu
     t.sorted, t.persistent, t.use_graph, t.collect_stats = sorted, persistent, use_graph, collect_stats
+    t.time_kernels = time_kernels
     return t
 
 

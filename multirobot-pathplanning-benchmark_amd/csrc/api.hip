@@ -197,7 +197,7 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
         if ((double)link_R[l] < R) link_R[l] = nextafterf(link_R[l], INFINITY);
         Rl_max = std::max(Rl_max, (double)link_R[l]);
     }
-    std::vector<int> lpairs, lpair_count(na + 1, 0);
+    std::vector<int> lpairs, lpairs2, lpair_count(na + 1, 0);
     for (int bm = 0; bm < na; ++bm)       // pairs ordered by the arm of their second link
         for (int l = 0; l < L; ++l)
             for (int m = bm * dof; m < (bm + 1) * dof; ++m) {
@@ -206,6 +206,8 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
                 if (al == bm && m - l < 2) continue;
                 if (link_first[l] == link_first[l + 1] || link_first[m] == link_first[m + 1]) continue;
                 lpairs.push_back((l << 8) | m);
+                lpairs2.push_back((l << 8) | m);
+                lpairs2.push_back(link_first[l] | (H.counts[l] << 9) | (link_first[m] << 13) | (H.counts[m] << 22));
             }
     for (int na_ = 1; na_ <= na; ++na_) {
         int c = 0;
@@ -214,7 +216,7 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
         lpair_count[na_] = c;
     }
     Packer P;
-    size_t o_lbc = P.add(link_bc), o_lR = P.add(link_R), o_lp = P.add(lpairs), o_lpc = P.add(lpair_count);
+    size_t o_lbc = P.add(link_bc), o_lR = P.add(link_R), o_lp = P.add(lpairs2), o_lpc = P.add(lpair_count);
     size_t o_base = P.add(H.base), o_dh = P.add(dh4), o_lf = P.add(link_first), o_sl = P.add(sph_link),
            o_slo = P.add(sph_lo), o_s64 = P.add(H.sph), o_sr = P.add(sph_r32), o_b32 = P.add(box32),
            o_b64 = P.add(box64), o_bch64 = P.add(H.boxes), o_bch32 = P.add(boxch32),
@@ -264,11 +266,27 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     }
     V.link_bc64 = at<double>(d, o_lbc);
     V.link_R32 = at<float>(d, o_lR);
-    V.lpairs = at<int>(d, o_lp);
+    V.NP = (int)lpairs.size();
+    V.lpairs2 = at<int2>(d, o_lp);
     V.lpair_count = at<int>(d, o_lpc);
     sc->D = V.D;
     sc->n_robots = na;
-    sc->arm_lds = arm_lds_bytes(V.D, L, na, S);
+    // waves per workgroup: the most resident waves per CU (160 KiB of LDS) within 64 KiB per
+    // workgroup; the scene is staged once per workgroup
+    sc->arm_nw = 0;
+    int best = 0;
+    for (int nw = 4; nw >= 1; nw >>= 1) {
+        const size_t b = arm_lds_bytes(V.D, L, na, nb, S, V.NP, nw);
+        if (b > 65536) continue;
+        const int resident = (int)(163840 / b) * nw;
+        if (resident > best) {
+            best = resident;
+            sc->arm_nw = nw;
+        }
+    }
+    if (sc->arm_nw == 0) return fail(MRCC_ERR_INVALID, "arm scene needs more than 64 KiB of LDS per wave");
+    sc->arm_lds = arm_lds_bytes(V.D, L, na, nb, S, V.NP, sc->arm_nw);
+    sc->arm_lds1 = arm_lds_bytes(V.D, L, na, nb, S, V.NP, 1);
     sc->arm_na = na;
     sc->arm_dof = dof;
     sc->arm_S = S;
@@ -328,17 +346,29 @@ struct mrcc_edge_plan {
     hipGraph_t graph;
     hipGraphExec_t exec;
     hipStream_t cap_stream;
+    hipEvent_t ev[4];  // time_kernels: start, queue built, main kernel done, end
+    int timed_runs;
 };
 
 static void plan_launch_sequence(mrcc_edge_plan* p, hipStream_t st) {
+    const bool ev = p->ev[0] != nullptr;
+    if (ev) (void)hipEventRecord(p->ev[0], st);
     if (p->tune.persistent) {
         launch_plan_build(*p->sc, p->B, p->W, p->lgG, st);
+        if (ev) (void)hipEventRecord(p->ev[1], st);
         launch_plan_run(*p->sc, p->B, p->W, p->lgG, p->tune.slots_per_ticket, p->grid, st);
+        if (ev) (void)hipEventRecord(p->ev[2], st);
         // edges with more passes than the queue holds: one workgroup per edge
         launch_direct(*p->sc, p->B, p->W.long_list, 0, p->W.meta + 3, 1, 0, 4 * device_cus(),
                       p->W.stats, st);
     } else {
+        if (ev) (void)hipEventRecord(p->ev[1], st);
         launch_direct(*p->sc, p->B, nullptr, p->B.n, nullptr, 1, 1, 1 << 20, p->W.stats, st);
+        if (ev) (void)hipEventRecord(p->ev[2], st);
+    }
+    if (ev) {
+        (void)hipEventRecord(p->ev[3], st);
+        ++p->timed_runs;
     }
 }
 
@@ -511,7 +541,7 @@ int mrcc_scene_create_arms(int n_arms, int dof, const double* base, const double
     if (!out) return fail(MRCC_ERR_INVALID, "out is null");
     if (n_arms < 1 || n_arms > MRCC_MAX_ARMS || dof < 1 || dof > MRCC_MAX_DOF)
         return fail(MRCC_ERR_INVALID, "arms: 1..%d arms of 1..%d joints", MRCC_MAX_ARMS, MRCC_MAX_DOF);
-    if (n_boxes < 0) return fail(MRCC_ERR_INVALID, "negative box count");
+    if (n_boxes < 0 || n_boxes > 65535) return fail(MRCC_ERR_INVALID, "arms: 0..65535 boxes");
     SceneBox* box = new SceneBox();
     SceneRec* sc = &box->rec;
     memset(sc, 0, sizeof(SceneRec));
@@ -627,7 +657,7 @@ int mrcc_edge_plan_create(const mrcc_scene* scene, const mrcc_edge_batch* batch,
     if (tune) t = *tune;
     const bool arms = sc->kind == KIND_ARMS;
     if (t.slots_per_pass <= 0) t.slots_per_pass = arms ? 1 : 8;
-    if (t.slots_per_ticket <= 0) t.slots_per_ticket = arms ? 8 : 4;
+    if (t.slots_per_ticket <= 0) t.slots_per_ticket = arms ? 16 : 4;
     if (t.waves_per_cu <= 0) t.waves_per_cu = arms ? 16 : 16;
     if (!tune) {
         t.sorted = 1;
@@ -636,6 +666,8 @@ int mrcc_edge_plan_create(const mrcc_scene* scene, const mrcc_edge_batch* batch,
     }
     if ((t.slots_per_pass & (t.slots_per_pass - 1)) || t.slots_per_pass > 64)
         return fail(MRCC_ERR_INVALID, "slots_per_pass must be a power of two <= 64");
+    if (arms && t.slots_per_ticket > 64)  // one lane resolves one slot of a claimed chunk
+        return fail(MRCC_ERR_INVALID, "arm scenes: slots_per_ticket <= 64");
     mrcc_edge_plan* p = new mrcc_edge_plan();
     memset(p, 0, sizeof(*p));
     p->sc = sc;
@@ -674,6 +706,8 @@ int mrcc_edge_plan_create(const mrcc_scene* scene, const mrcc_edge_batch* batch,
     p->W.dir = arms ? nullptr : (double*)(w + o_dir);
     if (t.collect_stats) HIP_OK(hipMemset(w + o_stats, 0, 64));
     HIP_OK(hipMemset(w + o_meta, 0, 64));
+    if (t.time_kernels && !t.use_graph)
+        for (int i = 0; i < 4; ++i) HIP_OK(hipEventCreate(&p->ev[i]));
     if (t.use_graph && batch->n_edges > 0) {
         HIP_OK(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
         HIP_OK(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
@@ -717,9 +751,26 @@ int mrcc_edge_plan_stats(mrcc_edge_plan* p, int64_t* out8) {
     return MRCC_OK;
 }
 
+int mrcc_edge_plan_timing(mrcc_edge_plan* p, double* out3) {
+    if (!p || !out3) return fail(MRCC_ERR_INVALID, "null argument");
+    if (!p->ev[0]) return fail(MRCC_ERR_UNSUPPORTED, "plan created without time_kernels (or with a graph)");
+    if (p->timed_runs == 0) return fail(MRCC_ERR_INVALID, "plan has not run");
+    HIP_OK(hipEventSynchronize(p->ev[3]));
+    float a = 0.f, b = 0.f, c = 0.f;
+    HIP_OK(hipEventElapsedTime(&a, p->ev[0], p->ev[1]));
+    HIP_OK(hipEventElapsedTime(&b, p->ev[1], p->ev[2]));
+    HIP_OK(hipEventElapsedTime(&c, p->ev[0], p->ev[3]));
+    out3[0] = a;
+    out3[1] = b;
+    out3[2] = c;
+    return MRCC_OK;
+}
+
 int mrcc_edge_plan_destroy(mrcc_edge_plan* p) {
     if (!p) return MRCC_OK;
     (void)hipDeviceSynchronize();
+    for (int i = 0; i < 4; ++i)
+        if (p->ev[i]) (void)hipEventDestroy(p->ev[i]);
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);

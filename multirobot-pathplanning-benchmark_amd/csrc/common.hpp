@@ -377,7 +377,9 @@ struct ArmView {
     float Mcull;             // binary32 margin of the bounding-sphere tests
     const double* link_bc64; // [L*3] bounding-sphere centre in the link frame
     const float* link_R32;   // [L] bounding radius (covers every sphere of the link, inflated)
-    const int* lpairs;       // [NP] link pairs (l << 8 | m), l < m, sorted by arm of m
+    int NP;                  // link pairs in the pair list
+    const int2* lpairs2;     // [NP] (l << 8 | m, fl | nl << 9 | fm << 13 | nm << 22): link pair
+                             // l < m and the sphere ranges [fl, fl+nl), [fm, fm+nm); sorted by arm of m
     const int* lpair_count;  // [na+1] pairs whose arms are all < n_active
     float bb_lo[3], bb_hi[3];   // box containing every obstacle box (binary32, rounded outward)
 };

@@ -12,7 +12,9 @@ struct SceneRec {
     DiskView disk;
     PlanarView planar;
     ArmView arm;
-    size_t arm_lds;       // dynamic LDS bytes of the arm kernels
+    size_t arm_lds;       // dynamic LDS bytes of the arm kernels (arm_nw waves per workgroup)
+    size_t arm_lds1;      // dynamic LDS bytes of an arm kernel with one wave per workgroup
+    int arm_nw;           // waves per workgroup of the arm kernels
     void* dev_mem;        // one allocation holding every array the views point to
     size_t dev_bytes;
     // host copies needed by host code

@@ -299,9 +299,59 @@ __device__ inline bool planar_collides(const PlanarView& S, const double* q, uns
 }
 
 // --------------------------------------------------------------------------------------------
-// arms: one state per wavefront (block of 64 lanes).  Per-wave LDS layout (doubles first):
-//   q[D] cs[D] sn[D] X[L*12] G64[na*12] | sph[S] float4 | G32[na*16] float
+// arms: one state per wavefront; workgroups of 1, 2 or 4 waves.
+//
+// LDS of a workgroup = one static region (the scene, staged once per kernel by the whole
+// workgroup) + one dynamic region per wave (the state being evaluated).  Waves advance
+// independently: inside the evaluator they synchronise with wave-level barriers only.
+#define ARM_PAIRS_LDS_MAX 1024  // link-pair lists up to this length are staged in LDS
+#define ARM_BOXES_LDS_MAX 256   // obstacle box tables up to this length are staged in LDS
+
+struct ArmStaticLayout {
+    int o_dh, o_base, o_sphL, o_linkBC, o_box, o_sphR, o_sphLink, o_sphLo, o_linkR, o_linkFC, o_pairs;
+    int bytes;
+};
+
+__host__ __device__ inline int align16(int x) { return (x + 15) & ~15; }
+
+__host__ __device__ inline ArmStaticLayout arm_static_layout(int L, int na, int nb, int S, int NP) {
+    ArmStaticLayout o;
+    int b = 0;
+    o.o_dh = b;      b += 8 * 4 * L;
+    o.o_base = b;    b += 8 * 12 * na;
+    o.o_sphL = b;    b += 8 * 3 * S;
+    o.o_linkBC = b;  b += 8 * 3 * L;
+    b = align16(b);
+    o.o_box = b;     b += (nb <= ARM_BOXES_LDS_MAX ? 32 * nb : 0);  // lo xyz pad, hi xyz pad
+    o.o_sphR = b;    b += 4 * S;
+    o.o_sphLink = b; b += 4 * S;
+    o.o_sphLo = b;   b += 4 * S;
+    o.o_linkR = b;   b += 4 * L;
+    o.o_linkFC = b;  b += 4 * L;
+    b = align16(b);
+    o.o_pairs = b;   b += (NP <= ARM_PAIRS_LDS_MAX ? 8 * NP : 0);
+    o.bytes = align16(b);
+    return o;
+}
+
+// doubles of a wave region before the float4 sphere table, rounded up to an even count
+__host__ __device__ inline int arm_lds_doubles(int D, int L, int na) {
+    return (3 * D + 12 * L + 12 * na + 1) & ~1;
+}
+
+#define ARM_PEND_MAX 128  // band pairs awaiting their binary64 recheck, per wave
+
+__host__ __device__ inline int arm_wave_bytes(int D, int L, int na, int S) {
+    return align16(8 * arm_lds_doubles(D, L, na) + 16 * (S + L) + 8 * ARM_PEND_MAX + 4 * 64 + 4 * 16 * na);
+}
+
+// dynamic LDS of an arm kernel launched with `nw` waves per workgroup
+__host__ __device__ inline size_t arm_lds_bytes(int D, int L, int na, int nb, int S, int NP, int nw) {
+    return (size_t)arm_static_layout(L, na, nb, S, NP).bytes + (size_t)nw * arm_wave_bytes(D, L, na, S);
+}
+
 struct ArmLds {
+    // per wave
     double* q;
     double* cs;
     double* sn;
@@ -309,8 +359,21 @@ struct ArmLds {
     double* G64;
     float4* sph;
     float4* linkC;  // [L] bounding sphere (x, y, z, R) of each link
+    int2* pend;     // [ARM_PEND_MAX] (kind, i << 16 | j): kind 0 sphere i - box j, 1 sphere pair
     int* cand;      // [64] compacted candidates of one broad-phase round
     float* G32;
+    // per workgroup (the scene)
+    const double* dh;      // [L*4] d a cos(alpha) sin(alpha)
+    const double* base;    // [na*12]
+    const double* sphL;    // [S*3] sphere centres in the link frame
+    const double* linkBC;  // [L*3] bounding-sphere centres in the link frame
+    const float4* box;     // [nb*2] lo, hi
+    const float* sphR;     // [S]
+    const int* sphLink;    // [S]
+    const int* sphLo;      // [S]
+    const float* linkR;    // [L]
+    const int* linkFC;     // [L] first sphere | count << 16
+    const int2* pairs;     // [NP] link pairs with their sphere ranges (LDS or global)
 };
 
 // tests executed by one lane (broad phase link-box / sphere-box / link-pair / sphere-pair)
@@ -318,19 +381,22 @@ struct ArmCount {
     unsigned lb, sb, lp, sp;
 };
 
-// doubles before the float4 sphere table, rounded up to an even count (16-byte alignment)
-__host__ __device__ inline int arm_lds_doubles(int D, int L, int na) {
-    return (3 * D + 12 * L + 12 * na + 1) & ~1;
-}
-
-__host__ __device__ inline size_t arm_lds_bytes(int D, int L, int na, int S) {
-    return sizeof(double) * (size_t)arm_lds_doubles(D, L, na) + sizeof(float4) * (size_t)(S + L) +
-           sizeof(int) * 64 + sizeof(float) * 16 * (size_t)na;
-}
-
 __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
     ArmLds W;
-    double* d = (double*)base;
+    unsigned char* b = (unsigned char*)base;
+    const ArmStaticLayout o = arm_static_layout(V.L, V.na, V.nb, V.S, V.NP);
+    W.dh = (const double*)(b + o.o_dh);
+    W.base = (const double*)(b + o.o_base);
+    W.sphL = (const double*)(b + o.o_sphL);
+    W.linkBC = (const double*)(b + o.o_linkBC);
+    W.box = V.nb <= ARM_BOXES_LDS_MAX ? (const float4*)(b + o.o_box) : (const float4*)V.box32;
+    W.sphR = (const float*)(b + o.o_sphR);
+    W.sphLink = (const int*)(b + o.o_sphLink);
+    W.sphLo = (const int*)(b + o.o_sphLo);
+    W.linkR = (const float*)(b + o.o_linkR);
+    W.linkFC = (const int*)(b + o.o_linkFC);
+    W.pairs = V.NP <= ARM_PAIRS_LDS_MAX ? (const int2*)(b + o.o_pairs) : V.lpairs2;
+    double* d = (double*)(b + o.bytes + (threadIdx.x >> 6) * arm_wave_bytes(V.D, V.L, V.na, V.S));
     W.q = d;
     W.cs = d + V.D;
     W.sn = d + 2 * V.D;
@@ -338,60 +404,163 @@ __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
     W.G64 = W.X + 12 * V.L;
     W.sph = (float4*)(d + arm_lds_doubles(V.D, V.L, V.na));
     W.linkC = W.sph + V.S;
-    W.cand = (int*)(W.linkC + V.L);
+    W.pend = (int2*)(W.linkC + V.L);
+    W.cand = (int*)(W.pend + ARM_PEND_MAX);
     W.G32 = (float*)(W.cand + 64);
     return W;
 }
 
-__device__ inline void sphere_center_f64(const ArmView& V, const ArmLds& W, int s, double* c) {
-    const double* l = V.sph64 + 4 * s;
-    xform_point_f64(W.X + 12 * V.sph_link[s], l[0], l[1], l[2], c);
+// Stages the scene into the static LDS region (all threads of the workgroup, then a barrier).
+__device__ inline void arm_stage_scene(void* base, const ArmView& V) {
+    unsigned char* b = (unsigned char*)base;
+    const ArmStaticLayout o = arm_static_layout(V.L, V.na, V.nb, V.S, V.NP);
+    const int t = threadIdx.x, nt = blockDim.x;
+    double* dh = (double*)(b + o.o_dh);
+    for (int i = t; i < 4 * V.L; i += nt) dh[i] = V.dh[i];
+    double* base_ = (double*)(b + o.o_base);
+    for (int i = t; i < 12 * V.na; i += nt) base_[i] = V.base[i];
+    double* sphL = (double*)(b + o.o_sphL);
+    for (int i = t; i < V.S; i += nt) {
+        sphL[3 * i] = V.sph64[4 * i];
+        sphL[3 * i + 1] = V.sph64[4 * i + 1];
+        sphL[3 * i + 2] = V.sph64[4 * i + 2];
+        ((float*)(b + o.o_sphR))[i] = V.sph_r32[i];
+        ((int*)(b + o.o_sphLink))[i] = V.sph_link[i];
+        ((int*)(b + o.o_sphLo))[i] = V.sph_lo[i];
+    }
+    double* bc = (double*)(b + o.o_linkBC);
+    for (int i = t; i < 3 * V.L; i += nt) bc[i] = V.link_bc64[i];
+    for (int i = t; i < V.L; i += nt) {
+        ((float*)(b + o.o_linkR))[i] = V.link_R32[i];
+        ((int*)(b + o.o_linkFC))[i] = V.link_first[i] | ((V.link_first[i + 1] - V.link_first[i]) << 16);
+    }
+    if (V.nb <= ARM_BOXES_LDS_MAX) {
+        float4* box = (float4*)(b + o.o_box);
+        const float4* src = (const float4*)V.box32;
+        for (int i = t; i < 2 * V.nb; i += nt) box[i] = src[i];
+    }
+    if (V.NP <= ARM_PAIRS_LDS_MAX) {
+        int2* pr = (int2*)(b + o.o_pairs);
+        for (int i = t; i < V.NP; i += nt) pr[i] = V.lpairs2[i];
+    }
+    __syncthreads();
 }
 
-__device__ __attribute__((noinline)) bool arm_recheck_sph_box(const ArmView& V, const ArmLds& W, int s, int b) {
-    double c[3];
-    sphere_center_f64(V, W, s, c);
-    return sph_box_f64(c, V.sph64[4 * s + 3], V.box64 + 6 * b, V.box64 + 6 * b + 3);
+// LDS written by some lanes of a wave is read by other lanes of the same wave after this point
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-__device__ __attribute__((noinline)) bool arm_recheck_sph_sph(const ArmView& V, const ArmLds& W, int i, int j) {
-    double a[3], b[3];
-    sphere_center_f64(V, W, i, a);
-    sphere_center_f64(V, W, j, b);
-    return sph_sph_f64(a, V.sph64[4 * i + 3], b, V.sph64[4 * j + 3]);
-}
+__device__ inline int wave_lane() { return threadIdx.x & 63; }
 
-__device__ __attribute__((noinline)) bool arm_recheck_sph_obb(const ArmView& V, const ArmLds& W, int s, int a, int mode) {
-    double c[3];
-    sphere_center_f64(V, W, s, c);
-    return sph_obb_f64(c, V.sph64[4 * s + 3], W.G64 + 12 * a, V.m_half + 3 * (mode * V.na + a));
-}
-
-__device__ __attribute__((noinline)) bool arm_recheck_obb_box(const ArmView& V, const ArmLds& W, int a, int b, int mode) {
-    double A[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
-    const double* bc = V.boxch64 + 6 * b;
-    A[9] = bc[0];
-    A[10] = bc[1];
-    A[11] = bc[2];
-    return obb3_overlap_f64(A, bc + 3, W.G64 + 12 * a, V.m_half + 3 * (mode * V.na + a));
-}
-
-__device__ __attribute__((noinline)) bool arm_recheck_obb_obb(const ArmView& V, const ArmLds& W, int a, int b, int mode) {
-    return obb3_overlap_f64(W.G64 + 12 * a, V.m_half + 3 * (mode * V.na + a), W.G64 + 12 * b,
-                            V.m_half + 3 * (mode * V.na + b));
-}
-
-// Evaluates the state whose binary64 joint vector is in W.q (written by lanes < D before the
-// call).  All 64 lanes of the block must call.  Returns the wave-uniform result.
 __device__ inline unsigned lanes_below(unsigned long long m) {
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
+__device__ inline float readlane_f(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// Band pairs (binary32 test within the margin) are queued in LDS and rechecked in binary64 a
+// batch at a time, lane-parallel: the recheck code then runs where little else is live, instead
+// of a call inside the candidate loops (whose register cost would apply to the whole kernel).
+// Appends the pairs of the lanes with `band` set; `np` is the wave-uniform queue length.
+__device__ inline void arm_pend(const ArmLds& W, bool band, int kind, int i, int j, int& np) {
+    const unsigned long long m = __ballot(band);
+    if (band) W.pend[np + (int)lanes_below(m)] = make_int2(kind, (i << 16) | j);
+    np += __popcll(m);
+}
+
+// Rechecks the queued pairs in binary64 (same predicates as the oracle); empties the queue.
+// Returns the wave-uniform "some pair collides".
+__device__ inline bool arm_flush(const ArmView& V, const ArmLds& W, int& np, unsigned& rechecks) {
+    wave_sync();
+    bool hit = false;
+    for (int k = wave_lane(); k < np; k += 64) {
+        const int2 e = W.pend[k];
+        const int i = e.y >> 16, j = e.y & 0xffff;
+        double a[3];
+        xform_point_f64(W.X + 12 * W.sphLink[i], W.sphL[3 * i], W.sphL[3 * i + 1], W.sphL[3 * i + 2], a);
+        if (e.x == 0) {
+            hit |= sph_box_f64(a, V.sph64[4 * i + 3], V.box64 + 6 * j, V.box64 + 6 * j + 3);
+        } else {
+            double b[3];
+            xform_point_f64(W.X + 12 * W.sphLink[j], W.sphL[3 * j], W.sphL[3 * j + 1], W.sphL[3 * j + 2], b);
+            hit |= sph_sph_f64(a, V.sph64[4 * i + 3], b, V.sph64[4 * j + 3]);
+        }
+    }
+    if (wave_lane() == 0) rechecks += np;
+    np = 0;
+    wave_sync();
+    return __any(hit);
+}
+
+// Rechecks queued held-box pairs in binary64: kind 2 held box i - obstacle box j, 3 sphere i -
+// held box j, 4 held boxes i, j.  Empties the queue; returns the wave-uniform "some pair collides".
+__device__ inline bool arm_flush_held(const ArmView& V, const ArmLds& W, int mode, int& np,
+                                      unsigned& rechecks) {
+    wave_sync();
+    bool hit = false;
+    for (int k = wave_lane(); k < np; k += 64) {
+        const int2 e = W.pend[k];
+        const int i = e.y >> 16, j = e.y & 0xffff;
+        if (e.x == 2) {
+            double A[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+            const double* bc = V.boxch64 + 6 * j;
+            A[9] = bc[0];
+            A[10] = bc[1];
+            A[11] = bc[2];
+            hit |= obb3_overlap_f64(A, bc + 3, W.G64 + 12 * i, V.m_half + 3 * (mode * V.na + i));
+        } else if (e.x == 3) {
+            double c[3];
+            xform_point_f64(W.X + 12 * W.sphLink[i], W.sphL[3 * i], W.sphL[3 * i + 1], W.sphL[3 * i + 2], c);
+            hit |= sph_obb_f64(c, V.sph64[4 * i + 3], W.G64 + 12 * j, V.m_half + 3 * (mode * V.na + j));
+        } else {
+            hit |= obb3_overlap_f64(W.G64 + 12 * i, V.m_half + 3 * (mode * V.na + i), W.G64 + 12 * j,
+                                    V.m_half + 3 * (mode * V.na + j));
+        }
+    }
+    if (wave_lane() == 0) rechecks += np;
+    np = 0;
+    wave_sync();
+    return __any(hit);
+}
+
+// Per-lane registers loaded once per kernel: obstacle boxes lane and lane + 64 (up to 128 boxes
+// in registers) and the lane's link (first sphere, count).
+struct ArmRegs {
+    float4 lo0, hi0, lo1, hi1;
+    int linkFC;
+};
+
+__device__ inline ArmRegs arm_regs(const ArmView& V, const ArmLds& W) {
+    ArmRegs R;
+    const int lane = wave_lane();
+    const float4 far = make_float4(3.0e38f, 3.0e38f, 3.0e38f, 0.f);
+    R.lo0 = lane < V.nb ? W.box[2 * lane] : far;
+    R.hi0 = lane < V.nb ? W.box[2 * lane + 1] : far;
+    R.lo1 = lane + 64 < V.nb ? W.box[2 * (lane + 64)] : far;
+    R.hi1 = lane + 64 < V.nb ? W.box[2 * (lane + 64) + 1] : far;
+    R.linkFC = lane < V.L ? W.linkFC[lane] : 0;
+    return R;
+}
+
+__device__ inline float box_d2(float4 c, float4 lo, float4 hi) {
+    const float e0 = fminf(fmaxf(c.x, lo.x), hi.x) - c.x;
+    const float e1 = fminf(fmaxf(c.y, lo.y), hi.y) - c.y;
+    const float e2 = fminf(fmaxf(c.z, lo.z), hi.z) - c.z;
+    return fmaf(e2, e2, fmaf(e1, e1, e0 * e0));
+}
+
+// Evaluates the state whose binary64 joint vector is in W.q (written by lanes < D before the
+// call).  All 64 lanes of the wave must call.  Returns the wave-uniform result.
 // GRASP = false instantiates the evaluator for scenes without held boxes (fewer registers).
 template <bool GRASP>
-__device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int mode,
-                                          unsigned mask, unsigned& rechecks, ArmCount& cnt) {
-    const int lane = threadIdx.x;
+__device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, const ArmRegs& R,
+                                          int mode, unsigned mask, unsigned& rechecks, ArmCount& cnt) {
+    const int lane = wave_lane();
     if (mode < 0 || mode >= V.nm) mode = 0;
     const int na = V.m_active[mode];
     const int dof = V.dof;
@@ -406,16 +575,16 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int
         W.sn[lane] = t.s;
     }
     if (__any(bad)) return true;  // non-finite coordinates are reported as collisions
-    __syncthreads();
+    wave_sync();
     // 2. forward kinematics (binary64): lane (arm a, row r) carries row r of the arm's transform
     //    through the chain (each entry with the operations of dh_step_f64); grasped boxes
     if (lane < 3 * na) {
         const int a = lane / 3, r = lane - 3 * a;
-        double R0 = V.base[12 * a + 3 * r], R1 = V.base[12 * a + 3 * r + 1], R2 = V.base[12 * a + 3 * r + 2];
-        double t = V.base[12 * a + 9 + r];
+        double R0 = W.base[12 * a + 3 * r], R1 = W.base[12 * a + 3 * r + 1], R2 = W.base[12 * a + 3 * r + 2];
+        double t = W.base[12 * a + 9 + r];
         for (int k = 0; k < dof; ++k) {
             const int l = a * dof + k;
-            dh_row_f64(R0, R1, R2, t, V.dh + 4 * l, W.cs[l], W.sn[l]);
+            dh_row_f64(R0, R1, R2, t, W.dh + 4 * l, W.cs[l], W.sn[l]);
             double* X = W.X + 12 * l;
             X[3 * r] = R0;
             X[3 * r + 1] = R1;
@@ -435,23 +604,28 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int
             W.G32[16 * a + 12 + r] = (float)V.m_half[3 * (mode * V.na + a) + r];
         }
     }
-    __syncthreads();
-    // 3. sphere centres: binary64, rounded to binary32
-    const int Sact = V.link_first[Dact];
-    for (int s = lane; s < Sact; s += 64) {
+    wave_sync();
+    // 3. sphere centres and link bounding spheres: binary64, rounded to binary32
+    const int lastFC = __builtin_amdgcn_readlane(R.linkFC, Dact - 1);
+    const int S_act = (lastFC & 0xffff) + (lastFC >> 16);
+    for (int s = lane; s < S_act; s += 64) {
         double c[3];
-        sphere_center_f64(V, W, s, c);
-        W.sph[s] = make_float4((float)c[0], (float)c[1], (float)c[2], V.sph_r32[s]);
+        const double* l = W.sphL + 3 * s;
+        xform_point_f64(W.X + 12 * W.sphLink[s], l[0], l[1], l[2], c);
+        W.sph[s] = make_float4((float)c[0], (float)c[1], (float)c[2], W.sphR[s]);
     }
+    float4 myLink = make_float4(0.f, 0.f, 0.f, 0.f);
     if (V.broadphase && lane < Dact) {
-        const double* b = V.link_bc64 + 3 * lane;
+        const double* b = W.linkBC + 3 * lane;
         double c[3];
         xform_point_f64(W.X + 12 * lane, b[0], b[1], b[2], c);
-        W.linkC[lane] = make_float4((float)c[0], (float)c[1], (float)c[2], V.link_R32[lane]);
+        myLink = make_float4((float)c[0], (float)c[1], (float)c[2], W.linkR[lane]);
+        W.linkC[lane] = myLink;
     }
-    __syncthreads();
+    wave_sync();
     const float M = V.Msph;
     bool hit = false;
+    int np = 0;  // queued band pairs
     if (V.broadphase) {
         const float Mc = V.Mcull;
         // 4'. link pairs of the pair set, then the sphere pairs of candidate link pairs (8 x 8);
@@ -462,54 +636,58 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int
             bool cand = false;
             int v = 0;
             if (pi < NP) {
-                v = V.lpairs[pi];
-                const int l = v >> 8, mm = v & 255;
+                const int2 pr = W.pairs[pi];
+                const int l = pr.x >> 8, mm = pr.x & 255;
+                v = pi;
                 if (((mask >> (l / dof)) & 1u) && ((mask >> (mm / dof)) & 1u)) {
                     const float4 A = W.linkC[l], Bc = W.linkC[mm];
                     const float dx = A.x - Bc.x, dy = A.y - Bc.y, dz = A.z - Bc.z;
-                    const float R = A.w + Bc.w;
-                    cand = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) <= R * R + Mc;
+                    const float Rs = A.w + Bc.w;
+                    cand = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) <= Rs * Rs + Mc;
                     ++cnt.lp;
                 }
             }
             const unsigned long long m = __ballot(cand);
             if (m == 0ull) continue;
             if (cand) W.cand[lanes_below(m)] = v;
-            __syncthreads();
+            wave_sync();
             const int n = __popcll(m);
             for (int c = 0; c < n; ++c) {
-                const int w = W.cand[c];
-                const int l = w >> 8, mm = w & 255;
-                const int fl = V.link_first[l], nl = V.link_first[l + 1] - fl;
-                const int fm = V.link_first[mm], nm = V.link_first[mm + 1] - fm;
+                const int2 pr = W.pairs[W.cand[c]];
+                const int fl = pr.y & 511, nl = (pr.y >> 9) & 15, fm = (pr.y >> 13) & 511, nm = (pr.y >> 22) & 15;
                 const int i = fl + (lane >> 3), j = fm + (lane & 7);
+                bool band = false;
                 if ((lane >> 3) < nl && (lane & 7) < nm) {
                     const float4 ci = W.sph[i], cj = W.sph[j];
                     const float dx = ci.x - cj.x, dy = ci.y - cj.y, dz = ci.z - cj.z;
                     const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    const float R = ci.w + cj.w;
-                    const float T = R * R;
+                    const float Rs = ci.w + cj.w;
+                    const float T = Rs * Rs;
                     ++cnt.sp;
                     if (d2 <= T + M) {
                         if (d2 < T - M) hit = true;
-                        else {
-                            ++rechecks;
-                            if (arm_recheck_sph_sph(V, W, i, j)) hit = true;
-                        }
+                        else band = true;
                     }
                 }
+                arm_pend(W, band, 1, i, j, np);
                 if ((c & 3) == 3 && __any(hit)) return true;
+                if (np > ARM_PEND_MAX - 64 && arm_flush(V, W, np, rechecks)) return true;
             }
-            __syncthreads();
+            wave_sync();
             if (__any(hit)) return true;
         }
-        // 5'. links vs boxes, then the spheres of candidate (link, box) pairs: 8 candidates x 8
-        //     spheres per wave iteration
+        // 5'. links vs boxes (boxes in registers, lane b holds box b), then the spheres of
+        //     candidate (link, box) pairs: 8 candidates x 8 spheres per wave iteration
         for (int l = 0; l < Dact; ++l) {
             if (!((mask >> (l / dof)) & 1u)) continue;
-            const int f = V.link_first[l], ns = V.link_first[l + 1] - f;
+            const int fc = __builtin_amdgcn_readlane(R.linkFC, l);
+            const int f = fc & 0xffff, ns = fc >> 16;
             if (ns == 0) continue;
-            const float4 Lc = W.linkC[l];
+            float4 Lc;
+            Lc.x = readlane_f(myLink.x, l);
+            Lc.y = readlane_f(myLink.y, l);
+            Lc.z = readlane_f(myLink.z, l);
+            Lc.w = readlane_f(myLink.w, l);
             const float Tl = Lc.w * Lc.w;
             {   // the link's bounding sphere clears the box around all obstacles: no box can touch it
                 const float e0 = fminf(fmaxf(Lc.x, V.bb_lo[0]), V.bb_hi[0]) - Lc.x;
@@ -521,100 +699,94 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int
                 const int b = b0 + lane;
                 bool cand = false;
                 if (b < V.nb) {
-                    const float* bx = V.box32 + 8 * b;
-                    const float e0 = fminf(fmaxf(Lc.x, bx[0]), bx[4]) - Lc.x;
-                    const float e1 = fminf(fmaxf(Lc.y, bx[1]), bx[5]) - Lc.y;
-                    const float e2 = fminf(fmaxf(Lc.z, bx[2]), bx[6]) - Lc.z;
-                    cand = fmaf(e2, e2, fmaf(e1, e1, e0 * e0)) <= Tl + Mc;
+                    float4 lo, hi;
+                    if (b0 == 0) { lo = R.lo0; hi = R.hi0; }
+                    else if (b0 == 64) { lo = R.lo1; hi = R.hi1; }
+                    else { lo = W.box[2 * b]; hi = W.box[2 * b + 1]; }
+                    cand = box_d2(Lc, lo, hi) <= Tl + Mc;
                     ++cnt.lb;
                 }
                 const unsigned long long m = __ballot(cand);
                 if (m == 0ull) continue;
                 if (cand) W.cand[lanes_below(m)] = b;
-                __syncthreads();
+                wave_sync();
                 const int n = __popcll(m);
                 for (int c0 = 0; c0 < n; c0 += 8) {
                     const int k = c0 + (lane >> 3), j = lane & 7;
+                    bool band = false;
+                    int s = 0, bb = 0;
                     if (k < n && j < ns) {
-                        const int bb = W.cand[k];
-                        const int s = f + j;
+                        bb = W.cand[k];
+                        s = f + j;
                         const float4 c = W.sph[s];
-                        const float* bx = V.box32 + 8 * bb;
-                        const float e0 = fminf(fmaxf(c.x, bx[0]), bx[4]) - c.x;
-                        const float e1 = fminf(fmaxf(c.y, bx[1]), bx[5]) - c.y;
-                        const float e2 = fminf(fmaxf(c.z, bx[2]), bx[6]) - c.z;
-                        const float d2 = fmaf(e2, e2, fmaf(e1, e1, e0 * e0));
+                        const float d2 = box_d2(c, W.box[2 * bb], W.box[2 * bb + 1]);
                         const float T = c.w * c.w;
                         ++cnt.sb;
                         if (d2 <= T + M) {
                             if (d2 < T - M) hit = true;
-                            else {
-                                ++rechecks;
-                                if (arm_recheck_sph_box(V, W, s, bb)) hit = true;
-                            }
+                            else band = true;
                         }
                     }
+                    arm_pend(W, band, 0, s, bb, np);
+                    if (np > ARM_PEND_MAX - 64 && arm_flush(V, W, np, rechecks)) return true;
                 }
-                __syncthreads();
+                wave_sync();
                 if (__any(hit)) return true;
             }
         }
     } else {
-    // 4. spheres vs boxes
-    for (int base = 0; base < Sact; base += 64) {
-        const int s = base + lane;
-        const bool act = s < Sact && ((mask >> (V.sph_link[s < Sact ? s : 0] / dof)) & 1u);
-        const float4 c = W.sph[act ? s : 0];
-        const float T = c.w * c.w;
-        for (int b = 0; b < V.nb; ++b) {
-            const float* bx = V.box32 + 8 * b;
-            const float e0 = fminf(fmaxf(c.x, bx[0]), bx[4]) - c.x;
-            const float e1 = fminf(fmaxf(c.y, bx[1]), bx[5]) - c.y;
-            const float e2 = fminf(fmaxf(c.z, bx[2]), bx[6]) - c.z;
-            const float d2 = fmaf(e2, e2, fmaf(e1, e1, e0 * e0));
-            cnt.sb += act;
-            if (act && d2 <= T + M) {
-                if (d2 < T - M) hit = true;
-                else {
-                    ++rechecks;
-                    if (arm_recheck_sph_box(V, W, s, b)) hit = true;
+        // 4. spheres vs boxes (brute force)
+        for (int base = 0; base < S_act; base += 64) {
+            const int s = base + lane;
+            const bool act = s < S_act && ((mask >> (W.sphLink[s < S_act ? s : 0] / dof)) & 1u);
+            const float4 c = W.sph[act ? s : 0];
+            const float T = c.w * c.w;
+            for (int b = 0; b < V.nb; ++b) {
+                const float d2 = box_d2(c, W.box[2 * b], W.box[2 * b + 1]);
+                cnt.sb += act;
+                bool band = false;
+                if (act && d2 <= T + M) {
+                    if (d2 < T - M) hit = true;
+                    else band = true;
+                }
+                arm_pend(W, band, 0, s, b, np);
+                if (np > ARM_PEND_MAX - 64 && arm_flush(V, W, np, rechecks)) return true;
+            }
+            if (__any(hit)) return true;
+        }
+        // 5. sphere pairs: same arm links >= 2 apart, and different arms (brute force)
+        for (int base = 0; base < S_act; base += 64) {
+            const int i = base + lane;
+            const bool act = i < S_act && ((mask >> (W.sphLink[i < S_act ? i : 0] / dof)) & 1u);
+            const float4 ci = W.sph[act ? i : 0];
+            const int lo = act ? W.sphLo[i] : 0x7fffffff;
+            const int jstart = W.sphLo[base];
+            int nj = 0;
+            for (int j = jstart; j < S_act; ++j) {
+                const int aj = W.sphLink[j] / dof;
+                if (!((mask >> aj) & 1u)) continue;  // wave-uniform
+                const float4 cj = W.sph[j];
+                const float dx = ci.x - cj.x, dy = ci.y - cj.y, dz = ci.z - cj.z;
+                const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                const float Rs = ci.w + cj.w;
+                const float T = Rs * Rs;
+                cnt.sp += (j >= lo);
+                bool band = false;
+                if (j >= lo && d2 <= T + M) {
+                    if (d2 < T - M) hit = true;
+                    else band = true;
+                }
+                arm_pend(W, band, 1, i, j, np);
+                if (np > ARM_PEND_MAX - 64 && arm_flush(V, W, np, rechecks)) return true;
+                if (++nj == 16) {
+                    nj = 0;
+                    if (__any(hit)) return true;
                 }
             }
+            if (__any(hit)) return true;
         }
-        if (__any(hit)) return true;
     }
-    // 5. sphere pairs: same arm links >= 2 apart, and different arms
-    for (int base = 0; base < Sact; base += 64) {
-        const int i = base + lane;
-        const bool act = i < Sact && ((mask >> (V.sph_link[i < Sact ? i : 0] / dof)) & 1u);
-        const float4 ci = W.sph[act ? i : 0];
-        const int lo = act ? V.sph_lo[i] : 0x7fffffff;
-        int jstart = V.sph_lo[base];
-        int nj = 0;
-        for (int j = jstart; j < Sact; ++j) {
-            const int aj = V.sph_link[j] / dof;
-            if (!((mask >> aj) & 1u)) continue;  // wave-uniform
-            const float4 cj = W.sph[j];
-            const float dx = ci.x - cj.x, dy = ci.y - cj.y, dz = ci.z - cj.z;
-            const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            const float R = ci.w + cj.w;
-            const float T = R * R;
-            cnt.sp += (j >= lo);
-            if (j >= lo && d2 <= T + M) {
-                if (d2 < T - M) hit = true;
-                else {
-                    ++rechecks;
-                    if (arm_recheck_sph_sph(V, W, i, j)) hit = true;
-                }
-            }
-            if (++nj == 16) {
-                nj = 0;
-                if (__any(hit)) return true;
-            }
-        }
-        if (__any(hit)) return true;
-    }
-    }  // brute force
+    if (np && arm_flush(V, W, np, rechecks)) return true;
     // 6. held boxes
     if constexpr (!GRASP) return false;
     unsigned gmask = 0;
@@ -626,16 +798,19 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int
         for (int a = 0; a < na; ++a) {
             if (!((gmask >> a) & 1u)) continue;
             const float* G = W.G32 + 16 * a;
-            for (int b = lane; b < V.nb; b += 64) {
-                float A[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
-                const float* bc = V.boxch32 + 6 * b;
-                A[9] = bc[0]; A[10] = bc[1]; A[11] = bc[2];
-                int r = obb3_overlap_f32(A, bc + 3, G, G + 12, Mb);
-                if (r == 1) hit = true;
-                else if (r == 2) {
-                    ++rechecks;
-                    if (arm_recheck_obb_box(V, W, a, b, mode)) hit = true;
+            for (int b0 = 0; b0 < V.nb; b0 += 64) {
+                const int b = b0 + lane;
+                bool band = false;
+                if (b < V.nb) {
+                    float A[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+                    const float* bc = V.boxch32 + 6 * b;
+                    A[9] = bc[0]; A[10] = bc[1]; A[11] = bc[2];
+                    const int r = obb3_overlap_f32(A, bc + 3, G, G + 12, Mb);
+                    if (r == 1) hit = true;
+                    else if (r == 2) band = true;
                 }
+                arm_pend(W, band, 2, a, b, np);
+                if (np > ARM_PEND_MAX - 64 && arm_flush_held(V, W, mode, np, rechecks)) return true;
             }
         }
         if (__any(hit)) return true;
@@ -643,46 +818,47 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, int
         for (int a = 0; a < na; ++a) {
             if (!((gmask >> a) & 1u)) continue;
             const float* G = W.G32 + 16 * a;
-            for (int base = 0; base < Sact; base += 64) {
+            for (int base = 0; base < S_act; base += 64) {
                 const int s = base + lane;
-                const int as = s < Sact ? V.sph_link[s] / dof : a;
-                const bool act = s < Sact && as != a && ((mask >> as) & 1u);
-                if (!act) continue;
-                const float4 c = W.sph[s];
-                const float w0 = c.x - G[9], w1 = c.y - G[10], w2 = c.z - G[11];
-                float d2 = 0.0f;
+                const int as = s < S_act ? W.sphLink[s] / dof : a;
+                const bool act = s < S_act && as != a && ((mask >> as) & 1u);
+                bool band = false;
+                if (act) {
+                    const float4 c = W.sph[s];
+                    const float w0 = c.x - G[9], w1 = c.y - G[10], w2 = c.z - G[11];
+                    float d2 = 0.0f;
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    const float l = G[k] * w0 + G[3 + k] * w1 + G[6 + k] * w2;
-                    const float e = fminf(fmaxf(l, -G[12 + k]), G[12 + k]) - l;
-                    d2 = fmaf(e, e, d2);
-                }
-                const float T = c.w * c.w;
-                if (d2 <= T + Mb) {
-                    if (d2 < T - Mb) hit = true;
-                    else {
-                        ++rechecks;
-                        if (arm_recheck_sph_obb(V, W, s, a, mode)) hit = true;
+                    for (int k = 0; k < 3; ++k) {
+                        const float l = G[k] * w0 + G[3 + k] * w1 + G[6 + k] * w2;
+                        const float e = fminf(fmaxf(l, -G[12 + k]), G[12 + k]) - l;
+                        d2 = fmaf(e, e, d2);
+                    }
+                    const float T = c.w * c.w;
+                    if (d2 <= T + Mb) {
+                        if (d2 < T - Mb) hit = true;
+                        else band = true;
                     }
                 }
+                arm_pend(W, band, 3, s, a, np);
+                if (np > ARM_PEND_MAX - 64 && arm_flush_held(V, W, mode, np, rechecks)) return true;
             }
         }
         if (__any(hit)) return true;
-        // 6c. held box vs held box
-        for (int a = 0; a < na; ++a) {
-            if (!((gmask >> a) & 1u)) continue;
-            for (int b = a + 1 + lane; b < na; b += 64) {
-                if (!((gmask >> b) & 1u)) continue;
+        // 6c. held box vs held box (at most 8 x 7 / 2 pairs: one lane each)
+        {
+            const int a = lane >> 3, b = lane & 7;
+            bool band = false;
+            if (a < na && b < na && a < b && ((gmask >> a) & 1u) && ((gmask >> b) & 1u)) {
                 const float* Ga = W.G32 + 16 * a;
                 const float* Gb = W.G32 + 16 * b;
-                int r = obb3_overlap_f32(Ga, Ga + 12, Gb, Gb + 12, Mb);
+                const int r = obb3_overlap_f32(Ga, Ga + 12, Gb, Gb + 12, Mb);
                 if (r == 1) hit = true;
-                else if (r == 2) {
-                    ++rechecks;
-                    if (arm_recheck_obb_obb(V, W, a, b, mode)) hit = true;
-                }
+                else if (r == 2) band = true;
             }
+            arm_pend(W, band, 4, a, b, np);
         }
+        if (__any(hit)) return true;
+        if (np && arm_flush_held(V, W, mode, np, rechecks)) return true;
     }
     return __any(hit);
 }

@@ -64,36 +64,50 @@ __global__ void k_plan_init(PlanWork W) {
     if (t < 8 && W.stats) W.stats[t] = 0;
 }
 
-__global__ void k_plan_prep(EdgeBatchView B, PlanWork W, int D, int lgG, int want_dir) {
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= B.n) return;
-    const int N = edge_N_dev(B, e, D);
-    const int ns = B.N_start ? B.N_start[e] : 0;
-    const int nm = B.N_max ? B.N_max[e] : -1;
-    EdgeWin w = edge_window(N, ns, nm, B.incl);
-    int M = w.M;
-    long long S = ((long long)M + (1 << lgG) - 1) >> lgG;
-    if (w.err) atomicAdd((unsigned long long*)&W.meta[2], 1ull);
-    if (S > MRCC_MAX_PASSES) {
-        unsigned long long slot = atomicAdd((unsigned long long*)&W.meta[3], 1ull);
-        W.long_list[slot] = (int)e;
-        S = 0;
-    }
-    W.eN[e] = N;
-    W.eM[e] = M;
-    W.eS[e] = (int)S;
-    W.ew0[e] = w.w0;
-    W.efull[e] = (unsigned char)w.full;
-    B.out[e] = w.err ? 0 : 1;
-    if (want_dir && W.dir) {
-        const double* a = row1(B, e);
-        const double* b = row2(B, e);
-        for (int k = 0; k < D; ++k) {
-#pragma clang fp contract(off)
-            W.dir[e * D + k] = (b[k * B.ds] - a[k * B.ds]) / (double)(N - 1);
+// Pass counts of small edges are histogrammed in LDS first (one global atomic per block and
+// count): edges of a batch share a few pass counts, and one global atomic per edge on the same few
+// addresses serialises in L2.
+#define PLAN_LDS_BINS 1024
+
+__global__ void __launch_bounds__(256) k_plan_prep(EdgeBatchView B, PlanWork W, int D, int lgG,
+                                                   int want_dir) {
+    __shared__ int h[PLAN_LDS_BINS];
+    for (int i = threadIdx.x; i < PLAN_LDS_BINS; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < B.n;
+         e += (long long)gridDim.x * blockDim.x) {
+        const int N = edge_N_dev(B, e, D);
+        const int ns = B.N_start ? B.N_start[e] : 0;
+        const int nm = B.N_max ? B.N_max[e] : -1;
+        EdgeWin w = edge_window(N, ns, nm, B.incl);
+        int M = w.M;
+        long long S = ((long long)M + (1 << lgG) - 1) >> lgG;
+        if (w.err) atomicAdd((unsigned long long*)&W.meta[2], 1ull);
+        if (S > MRCC_MAX_PASSES) {
+            unsigned long long slot = atomicAdd((unsigned long long*)&W.meta[3], 1ull);
+            W.long_list[slot] = (int)e;
+            S = 0;
         }
+        W.eN[e] = N;
+        W.eM[e] = M;
+        W.eS[e] = (int)S;
+        W.ew0[e] = w.w0;
+        W.efull[e] = (unsigned char)w.full;
+        B.out[e] = w.err ? 0 : 1;
+        if (want_dir && W.dir) {
+            const double* a = row1(B, e);
+            const double* b = row2(B, e);
+            for (int k = 0; k < D; ++k) {
+#pragma clang fp contract(off)
+                W.dir[e * D + k] = (b[k * B.ds] - a[k * B.ds]) / (double)(N - 1);
+            }
+        }
+        if (S < PLAN_LDS_BINS) atomicAdd(&h[S], 1);
+        else atomicAdd(&W.hist[S], 1);
     }
-    atomicAdd(&W.hist[S], 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < PLAN_LDS_BINS; i += blockDim.x)
+        if (h[i]) atomicAdd(&W.hist[i], h[i]);
 }
 
 // one block of 1024 threads: suffix sums of the pass histogram and prefix sums of E_p
@@ -160,13 +174,37 @@ __global__ void __launch_bounds__(1024) k_plan_scan(PlanWork W, int lgG) {
     }
 }
 
-__global__ void k_plan_scatter(PlanWork W, long long n) {
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const int S = W.eS[e];
-    if (S <= 0) return;
-    const int pos = atomicAdd(&W.cursor[S], 1);
-    W.order[pos] = (int)e;
+// Tiles of 256 x SCATTER_EPT edges: edges of a small pass count are ranked within the tile in
+// LDS; the tile then reserves one range of the count's bucket with one global atomic.
+#define SCATTER_EPT 4
+
+__global__ void __launch_bounds__(256) k_plan_scatter(PlanWork W, long long n) {
+    __shared__ int cnt[PLAN_LDS_BINS];
+    __shared__ int base[PLAN_LDS_BINS];
+    const long long tile_edges = (long long)blockDim.x * SCATTER_EPT;
+    for (long long t0 = (long long)blockIdx.x * tile_edges; t0 < n; t0 += (long long)gridDim.x * tile_edges) {
+        for (int i = threadIdx.x; i < PLAN_LDS_BINS; i += blockDim.x) cnt[i] = 0;
+        __syncthreads();
+        int S[SCATTER_EPT], r[SCATTER_EPT];
+#pragma unroll
+        for (int k = 0; k < SCATTER_EPT; ++k) {
+            const long long e = t0 + (long long)k * blockDim.x + threadIdx.x;
+            S[k] = e < n ? W.eS[e] : 0;
+            r[k] = (S[k] > 0 && S[k] < PLAN_LDS_BINS) ? atomicAdd(&cnt[S[k]], 1) : 0;
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < PLAN_LDS_BINS; i += blockDim.x)
+            if (cnt[i]) base[i] = atomicAdd(&W.cursor[i], cnt[i]);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < SCATTER_EPT; ++k) {
+            const long long e = t0 + (long long)k * blockDim.x + threadIdx.x;
+            if (S[k] <= 0) continue;
+            const int pos = S[k] < PLAN_LDS_BINS ? base[S[k]] + r[k] : atomicAdd(&W.cursor[S[k]], 1);
+            W.order[pos] = (int)e;
+        }
+        __syncthreads();
+    }
 }
 
 // Work queue of the persistent kernels: the slots are cut into chunks; chunk c belongs to counter
@@ -391,86 +429,125 @@ __global__ void __launch_bounds__(256) k_lane_configs(View S, const double* qv, 
 }
 
 // ============================================================================================
-// arms: one state per block of 64 lanes
-__device__ inline void arm_load_state(const ArmView& V, const ArmLds& Wl, const EdgeBatchView& B,
-                                      long long e, int N, int i, int Dact) {
-    const int lane = threadIdx.x;
+// arms: one state per wavefront.  A workgroup of nw = blockDim.x / 64 waves stages the scene into
+// LDS once; each wave then works as an independent worker (worker id blockIdx.x * nw + wave)
+// with wave-level synchronisation only.
+__device__ inline void arm_load_state(const ArmLds& Wl, const EdgeBatchView& B, long long e, int N,
+                                      int i, int Dact) {
+    const int lane = wave_lane();
     if (lane < Dact) {
         const double* a = row1(B, e);
         const double* b = row2(B, e);
         Wl.q[lane] = interp_f64(a[lane * B.ds], b[lane * B.ds], N, i);
     }
-    __syncthreads();
+    wave_sync();
 }
 
+__device__ inline void arm_stats(unsigned long long* stats, unsigned rechecks, const ArmCount& cnt) {
+    if (!stats) return;
+    stat_add(stats, 1, rechecks);
+    stat_add(stats, 4, cnt.lb);
+    stat_add(stats, 5, cnt.sb);
+    stat_add(stats, 6, cnt.lp);
+    stat_add(stats, 7, cnt.sp);
+}
+
+// Persistent arm kernel.  A wave claims a chunk of `spt` (<= 64) consecutive slots; lane j
+// resolves slot j of the chunk (slot -> edge, state; edge already dead?) so the dependent loads of
+// all slots overlap; the wave then evaluates the live states one after another.
 template <bool GRASP>
-__global__ void __launch_bounds__(64) k_arm_persistent(ArmView V, EdgeBatchView B, PlanWork W,
-                                                       int lgG, int spt) {
+__global__ void __launch_bounds__(256) k_arm_persistent(ArmView V, EdgeBatchView B, PlanWork W,
+                                                        int lgG, int spt) {
     extern __shared__ double lds_arm[];
-    ArmLds Wl = arm_lds(lds_arm, V);
+    arm_stage_scene(lds_arm, V);
+    const ArmLds Wl = arm_lds(lds_arm, V);
+    const ArmRegs R = arm_regs(V, Wl);
+    const int lane = wave_lane();
     const long long T = W.meta[0];
     unsigned long long n_eval = 0, n_dead = 0, n_slots = 0;
     unsigned rechecks = 0;
     ArmCount cnt = {0, 0, 0, 0};
-    int p = 0;  // pass hint
+    int p = 0;  // pass hint (wave-uniform)
     int q = 0;  // queues exhausted so far
     const long long nchunks = (T + spt - 1) / spt;
     for (;;) {
         const long long c = claim_chunk(W, q, nchunks);
         if (c < 0) break;
         const long long s0 = c * spt;
-        for (int j = 0; j < spt; ++j) {
-            const long long s = s0 + j;
-            if (s >= T) break;
-            ++n_slots;
-            SlotRef ref = slot_ref(W, p, s, lgG, B.n);
-            if (ref.e < 0) continue;
-            if (B.out[ref.e] == 0) { ++n_dead; continue; }
-            EdgeWin w = win_of(W, ref.e, B.incl);
-            const int i = edge_state_index(w, ref.local, B.incl);
-            if (i < 0) continue;
-            const int mode = B.modes ? B.modes[ref.e] : 0;
-            const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
-            arm_load_state(V, Wl, B, ref.e, w.N, i, V.m_active[md] * V.dof);
-            ++n_eval;
-            if (arm_state_collides<GRASP>(V, Wl, mode, B.mask, rechecks, cnt)) {
-                if (threadIdx.x == 0) B.out[ref.e] = 0;
+        const long long rem = T - s0;
+        const int n_here = (int)(rem < spt ? rem : spt);  // >= 1: c < nchunks
+        // lane-parallel resolution of the chunk's slots
+        int e = -1, i = -1, N = 0;
+        int pl = p;
+        bool dead = false;
+        if (lane < n_here) {
+            const SlotRef ref = slot_ref(W, pl, s0 + lane, lgG, B.n);
+            if (ref.e >= 0) {
+                if (B.out[ref.e] == 0) {
+                    dead = true;
+                } else {
+                    const EdgeWin w = win_of(W, ref.e, B.incl);
+                    i = edge_state_index(w, ref.local, B.incl);
+                    if (i >= 0) {
+                        e = ref.e;
+                        N = w.N;
+                    }
+                }
             }
-            __syncthreads();
+        }
+        p = __builtin_amdgcn_readlane(pl, n_here - 1);  // the chunk's last slot has the largest pass
+        n_slots += n_here;
+        n_dead += __popcll(__ballot(dead));
+        unsigned long long live = __ballot(e >= 0);
+        while (live) {
+            const int j = __builtin_ctzll(live);
+            live &= live - 1;
+            const int ej = __builtin_amdgcn_readlane(e, j);
+            const int ij = __builtin_amdgcn_readlane(i, j);
+            const int Nj = __builtin_amdgcn_readlane(N, j);
+            if (B.out[ej] == 0) {  // killed by another wave since the chunk was resolved
+                ++n_dead;
+                continue;
+            }
+            const int mode = B.modes ? B.modes[ej] : 0;
+            const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
+            arm_load_state(Wl, B, ej, Nj, ij, V.m_active[md] * V.dof);
+            ++n_eval;
+            if (arm_state_collides<GRASP>(V, Wl, R, mode, B.mask, rechecks, cnt)) {
+                if (lane == 0) B.out[ej] = 0;
+            }
         }
     }
-    if (W.stats && threadIdx.x == 0) {
+    if (W.stats && lane == 0) {
         stat_add(W.stats, 0, n_eval);
         stat_add(W.stats, 2, n_dead);
         stat_add(W.stats, 3, n_slots);
     }
-    if (W.stats) {
-        stat_add(W.stats, 1, rechecks);
-        stat_add(W.stats, 4, cnt.lb);
-        stat_add(W.stats, 5, cnt.sb);
-        stat_add(W.stats, 6, cnt.lp);
-        stat_add(W.stats, 7, cnt.sp);
-    }
+    arm_stats(W.stats, rechecks, cnt);
 }
 
+// Work item k of n_list * wpe: edge list[k / wpe] (or k / wpe), states k % wpe, k % wpe + wpe, ...
+// With init_out (wpe == 1 only) the wave owns its edge: it writes the verdict without reading it.
 template <bool GRASP>
-__global__ void __launch_bounds__(64) k_arm_direct(ArmView V, EdgeBatchView B, const int* list,
-                                                   long long n_list, const long long* n_dev,
-                                                   int wpe, int init_out,
-                                                   unsigned long long* stats) {
+__global__ void __launch_bounds__(256) k_arm_direct(ArmView V, EdgeBatchView B, const int* list,
+                                                    long long n_list, const long long* n_dev,
+                                                    int wpe, int init_out,
+                                                    unsigned long long* stats) {
     extern __shared__ double lds_arm[];
-    ArmLds Wl = arm_lds(lds_arm, V);
+    arm_stage_scene(lds_arm, V);
+    const ArmLds Wl = arm_lds(lds_arm, V);
+    const ArmRegs R = arm_regs(V, Wl);
+    const int nw = blockDim.x >> 6;
     unsigned long long n_eval = 0;
     unsigned rechecks = 0;
     ArmCount cnt = {0, 0, 0, 0};
     if (n_dev) n_list = *n_dev;
-    const long long nblocks = n_list * wpe;
-    for (long long blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
-        const long long li = blk / wpe;
-        const int u = (int)(blk % wpe);
+    const long long nitems = n_list * wpe;
+    const long long w0 = (long long)blockIdx.x * nw + (threadIdx.x >> 6);
+    for (long long it = w0; it < nitems; it += (long long)gridDim.x * nw) {
+        const long long li = it / wpe;
+        const int u = (int)(it % wpe);
         const long long e = list ? (long long)list[li] : li;
-        if (init_out && threadIdx.x == 0) B.out[e] = 1;  // only with wpe == 1
-        __syncthreads();
         const int mode = B.modes ? B.modes[e] : 0;
         const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
         const int N = edge_N_dev(B, e, V.D);
@@ -478,51 +555,51 @@ __global__ void __launch_bounds__(64) k_arm_direct(ArmView V, EdgeBatchView B, c
         const int nm = B.N_max ? B.N_max[e] : -1;
         EdgeWin w = edge_window(N, ns, nm, B.incl);
         if (w.err) {
-            if (threadIdx.x == 0) B.out[e] = 0;
+            if (wave_lane() == 0) B.out[e] = 0;
             continue;
         }
+        bool hit = false;
         for (int local = u; local < w.M; local += wpe) {
-            if (B.out[e] == 0) break;
+            if (!init_out && B.out[e] == 0) break;
             const int i = edge_state_index(w, local, B.incl);
             if (i < 0) continue;
-            arm_load_state(V, Wl, B, e, N, i, V.m_active[md] * V.dof);
+            arm_load_state(Wl, B, e, N, i, V.m_active[md] * V.dof);
             ++n_eval;
-            const bool c = arm_state_collides<GRASP>(V, Wl, mode, B.mask, rechecks, cnt);
-            __syncthreads();
-            if (c) {
-                if (threadIdx.x == 0) B.out[e] = 0;
+            if (arm_state_collides<GRASP>(V, Wl, R, mode, B.mask, rechecks, cnt)) {
+                hit = true;
                 break;
             }
         }
-        __syncthreads();
+        if (wave_lane() == 0) {
+            if (hit) B.out[e] = 0;
+            else if (init_out) B.out[e] = 1;
+        }
     }
-    if (stats && threadIdx.x == 0) stat_add(stats, 0, n_eval);
-    if (stats) {
-        stat_add(stats, 1, rechecks);
-        stat_add(stats, 4, cnt.lb);
-        stat_add(stats, 5, cnt.sb);
-        stat_add(stats, 6, cnt.lp);
-        stat_add(stats, 7, cnt.sp);
-    }
+    if (stats && wave_lane() == 0) stat_add(stats, 0, n_eval);
+    arm_stats(stats, rechecks, cnt);
 }
 
 template <bool GRASP>
-__global__ void __launch_bounds__(64) k_arm_configs(ArmView V, const double* qv, long long n,
-                                                    long long rs, long long ds, const int* modes,
-                                                    unsigned mask, unsigned char* out) {
+__global__ void __launch_bounds__(256) k_arm_configs(ArmView V, const double* qv, long long n,
+                                                     long long rs, long long ds, const int* modes,
+                                                     unsigned mask, unsigned char* out) {
     extern __shared__ double lds_arm[];
-    ArmLds Wl = arm_lds(lds_arm, V);
+    arm_stage_scene(lds_arm, V);
+    const ArmLds Wl = arm_lds(lds_arm, V);
+    const ArmRegs R = arm_regs(V, Wl);
+    const int nw = blockDim.x >> 6;
+    const int lane = wave_lane();
     unsigned rechecks = 0;
     ArmCount cnt = {0, 0, 0, 0};
-    for (long long r = blockIdx.x; r < n; r += gridDim.x) {
+    for (long long r = (long long)blockIdx.x * nw + (threadIdx.x >> 6); r < n;
+         r += (long long)gridDim.x * nw) {
         const int mode = modes ? modes[r] : 0;
         const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
         const int Dact = V.m_active[md] * V.dof;
-        if ((int)threadIdx.x < Dact) Wl.q[threadIdx.x] = qv[r * rs + (long long)threadIdx.x * ds];
-        __syncthreads();
-        const bool c = arm_state_collides<GRASP>(V, Wl, mode, mask, rechecks, cnt);
-        if (threadIdx.x == 0) out[r] = c ? 0 : 1;
-        __syncthreads();
+        if (lane < Dact) Wl.q[lane] = qv[r * rs + (long long)lane * ds];
+        wave_sync();
+        const bool c = arm_state_collides<GRASP>(V, Wl, R, mode, mask, rechecks, cnt);
+        if (lane == 0) out[r] = c ? 0 : 1;
     }
 }
 
@@ -564,9 +641,14 @@ void launch_plan_build(const SceneRec& sc, const EdgeBatchView& B, const PlanWor
                        hipStream_t st) {
     const int want_dir = sc.kind != KIND_ARMS;
     k_plan_init<<<64, 256, 0, st>>>(W);
-    k_plan_prep<<<grid_for(B.n, 256), 256, 0, st>>>(B, W, sc.D, lgG, want_dir);
+    // a few edges per thread, so each block's LDS histogram covers many edges
+    int gp = grid_for(B.n, 4 * 256);
+    if (gp > 1024) gp = 1024;
+    int gs = grid_for(B.n, 256 * SCATTER_EPT);
+    if (gs > 1024) gs = 1024;
+    k_plan_prep<<<gp, 256, 0, st>>>(B, W, sc.D, lgG, want_dir);
     k_plan_scan<<<1, 1024, 0, st>>>(W, lgG);
-    k_plan_scatter<<<grid_for(B.n, 256), 256, 0, st>>>(W, B.n);
+    k_plan_scatter<<<gs, 256, 0, st>>>(W, B.n);
 }
 
 void launch_plan_run(const SceneRec& sc, const EdgeBatchView& B, const PlanWork& W, int lgG,
@@ -584,12 +666,16 @@ void launch_plan_run(const SceneRec& sc, const EdgeBatchView& B, const PlanWork&
             else
                 k_lane_persistent<PlanarView, 0, 0><<<grid, 256, 0, st>>>(sc.planar, B, W, lgG, spt);
             break;
-        case KIND_ARMS:
+        case KIND_ARMS: {
+            // grid counts waves; workgroups of arm_nw waves
+            const int nw = sc.arm_nw;
+            const int blocks = (grid + nw - 1) / nw;
             if (sc.arm_has_grasp)
-                k_arm_persistent<true><<<grid, 64, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
+                k_arm_persistent<true><<<blocks, 64 * nw, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
             else
-                k_arm_persistent<false><<<grid, 64, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
+                k_arm_persistent<false><<<blocks, 64 * nw, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
             break;
+        }
     }
 }
 
@@ -621,12 +707,18 @@ void launch_direct(const SceneRec& sc, const EdgeBatchView& B, const int* list, 
             else
                 k_lane_direct<PlanarView, 0, 0><<<grid, 64, 0, st>>>(sc.planar, B, list, n_list, n_dev, wpe, init_out, stats);
             break;
-        case KIND_ARMS:
+        case KIND_ARMS: {
+            // `grid` waves (at most one per work item), in workgroups of arm_nw waves, or of
+            // one wave for a launch of fewer waves than that
+            const int nw = grid < sc.arm_nw ? 1 : sc.arm_nw;
+            const size_t lds = nw == 1 ? sc.arm_lds1 : sc.arm_lds;
+            const int blocks = (grid + nw - 1) / nw;
             if (sc.arm_has_grasp)
-                k_arm_direct<true><<<grid, 64, sc.arm_lds, st>>>(sc.arm, B, list, n_list, n_dev, wpe, init_out, stats);
+                k_arm_direct<true><<<blocks, 64 * nw, lds, st>>>(sc.arm, B, list, n_list, n_dev, wpe, init_out, stats);
             else
-                k_arm_direct<false><<<grid, 64, sc.arm_lds, st>>>(sc.arm, B, list, n_list, n_dev, wpe, init_out, stats);
+                k_arm_direct<false><<<blocks, 64 * nw, lds, st>>>(sc.arm, B, list, n_list, n_dev, wpe, init_out, stats);
             break;
+        }
     }
 }
 
@@ -653,12 +745,16 @@ void launch_configs(const SceneRec& sc, const double* q, long long n, long long 
             break;
         }
         case KIND_ARMS: {
+            // one wave per configuration up to 4 * max_grid waves
             long long g = n < (long long)max_grid * 4 ? n : (long long)max_grid * 4;
             if (g < 1) g = 1;
+            const int nw = g < sc.arm_nw ? 1 : sc.arm_nw;
+            const size_t lds = nw == 1 ? sc.arm_lds1 : sc.arm_lds;
+            const int blocks = (int)((g + nw - 1) / nw);
             if (sc.arm_has_grasp)
-                k_arm_configs<true><<<(int)g, 64, sc.arm_lds, st>>>(sc.arm, q, n, rs, ds, modes, mask, out);
+                k_arm_configs<true><<<blocks, 64 * nw, lds, st>>>(sc.arm, q, n, rs, ds, modes, mask, out);
             else
-                k_arm_configs<false><<<(int)g, 64, sc.arm_lds, st>>>(sc.arm, q, n, rs, ds, modes, mask, out);
+                k_arm_configs<false><<<blocks, 64 * nw, lds, st>>>(sc.arm, q, n, rs, ds, modes, mask, out);
             break;
         }
     }

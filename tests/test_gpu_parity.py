@@ -275,6 +275,60 @@ def test_c3_edges(c3, B, bp):
     sc.set_broadphase(True)
 
 
+@pytest.mark.parametrize("spt,g,w", [(1, 1, 1), (16, 1, 12), (64, 4, 16), (7, 2, 3)])
+def test_c3_queue_shapes(c3, B, spt, g, w):
+    _say(f"C3 persistent queue shapes: slots/ticket {spt}, slots/pass {g}, waves/CU {w}; stats invariants")
+    desc, sc, o, rng = c3
+    a = near_home(rng, desc, 1500, sigma=0.7)
+    b = np.clip(a + rng.normal(0, 0.4, a.shape), desc.limits[0], desc.limits[1])
+    ref = o.check_edges(a, b, 0.05)
+    t = B.make_tune(slots_per_pass=g, slots_per_ticket=spt, waves_per_cu=w, use_graph=0, collect_stats=1)
+    assert np.array_equal(gpu_edges(sc, a, b, 0.05, tune=t), ref)
+
+
+def test_c3_windows(c3):
+    _say("C3 windowed edges (N_start / N_max / explicit N) through the arm queue")
+    desc, sc, o, rng = c3
+    n = 1500
+    a = near_home(rng, desc, n, sigma=0.6)
+    b = np.clip(a + rng.normal(0, 0.3, a.shape), desc.limits[0], desc.limits[1])
+    N = np.maximum(2, (np.max(np.abs(b - a), axis=1) / 0.05).astype(np.int64) + 1)
+    N = np.where(rng.uniform(size=n) < 0.3, N + rng.integers(0, 5, n), N).astype(np.int32)
+    S = (rng.uniform(size=n) * (N + 1)).astype(np.int32)
+    S = np.where(rng.uniform(size=n) < 0.3, 0, S)
+    M = np.where(rng.uniform(size=n) < 0.2, -1, S + rng.integers(0, 10, n)).astype(np.int32)
+    for incl in (False, True):
+        ref = o.check_edges(a, b, 0.05, N=N, N_start=S, N_max=M, include_endpoints=incl)
+        assert np.array_equal(gpu_edges(sc, a, b, 0.05, N=N, N_start=S, N_max=M, incl=incl), ref), incl
+
+
+def test_plan_timing_and_stats(c3, B):
+    _say("plan timing events (time_kernels) and stats invariants; arm slots_per_ticket limit")
+    desc, sc, o, rng = c3
+    a = near_home(rng, desc, 2000, sigma=0.7)
+    b = np.clip(a + rng.normal(0, 0.4, a.shape), desc.limits[0], desc.limits[1])
+    ref = o.check_edges(a, b, 0.05)
+    ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    out = torch.empty(len(a), dtype=torch.uint8, device="cuda")
+    batch = sc.edge_batch(ta, tb, out, 0.05)
+    plan = B.EdgePlan(sc, batch, B.make_tune(use_graph=0, time_kernels=1, collect_stats=1))
+    plan.run()
+    t = plan.timing()
+    assert 0.0 < t["kernel_ms"] <= t["run_ms"] and 0.0 <= t["build_ms"] <= t["run_ms"]
+    assert np.array_equal(out.cpu().numpy().astype(bool), ref)
+    st = plan.stats()
+    N = np.maximum(2, (np.max(np.abs(b - a), axis=1) / 0.05).astype(np.int64) + 1)
+    assert st["slots"] == int(np.sum(N - 2))
+    assert st["states"] + st["dead_skips"] <= st["slots"]
+    # every state of a free edge is evaluated; a colliding edge needs at least one
+    assert st["states"] >= int(np.sum(N[ref] - 2)) + int(np.sum(~ref))
+    gplan = B.EdgePlan(sc, batch, B.make_tune(use_graph=1))
+    with pytest.raises(pkg.MrccError):
+        gplan.timing()
+    with pytest.raises(pkg.MrccError):
+        B.EdgePlan(sc, batch, B.make_tune(slots_per_ticket=65))
+
+
 def test_c3_masks(c3):
     _say("C3 per-arm masks")
     desc, sc, o, rng = c3
