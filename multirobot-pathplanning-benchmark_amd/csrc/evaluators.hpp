@@ -194,27 +194,32 @@ __device__ __attribute__((noinline)) bool obb2_overlap_f64(const double* A, cons
     return s0 <= 0.0 && s1 <= 0.0 && s2 <= 0.0 && s3 <= 0.0;
 }
 
-// binary64 robot box [x y ux uy vx vy h0 h1] of robot i at q
-__device__ inline void planar_robot_box_f64(const PlanarView& S, const double* q, int i, double* A) {
-    SinCos t = sincos_f64(q[3 * i + 2]);
-    A[0] = q[3 * i];
-    A[1] = q[3 * i + 1];
+// binary64 robot box [x y ux uy vx vy h0 h1] of robot i at pose (x, y, theta)
+__device__ inline void planar_robot_box_f64(const PlanarView& S, double x, double y, double th, int i,
+                                            double* A) {
+    SinCos t = sincos_f64(th);
+    A[0] = x;
+    A[1] = y;
     A[2] = t.c; A[3] = t.s;
     A[4] = -t.s; A[5] = t.c;
     A[6] = S.rh64[2 * i];
     A[7] = S.rh64[2 * i + 1];
 }
 
-__device__ __attribute__((noinline)) bool planar_rr_f64(const PlanarView& S, const double* q, int i, int j) {
+// The rechecks take the poses by value: a pointer to the caller's state vector would keep that
+// vector in scratch memory for the whole kernel.
+__device__ __attribute__((noinline)) bool planar_rr_f64(const PlanarView& S, double xi, double yi, double ti,
+                                                        int i, double xj, double yj, double tj, int j) {
     double A[8], B[8];
-    planar_robot_box_f64(S, q, i, A);
-    planar_robot_box_f64(S, q, j, B);
+    planar_robot_box_f64(S, xi, yi, ti, i, A);
+    planar_robot_box_f64(S, xj, yj, tj, j, B);
     return obb2_overlap_f64(A, B);
 }
 
-__device__ __attribute__((noinline)) bool planar_ro_f64(const PlanarView& S, const double* q, int i, int o) {
+__device__ __attribute__((noinline)) bool planar_ro_f64(const PlanarView& S, double x, double y, double th,
+                                                        int i, int o) {
     double A[8];
-    planar_robot_box_f64(S, q, i, A);
+    planar_robot_box_f64(S, x, y, th, i, A);
     return obb2_overlap_f64(A, S.ob64 + 8 * o);
 }
 
@@ -276,7 +281,9 @@ __device__ inline bool planar_collides(const PlanarView& S, const double* q, uns
             if (v <= M) {
                 if (v < -M) return true;
                 ++rechecks;
-                if (planar_rr_f64(S, q, i, j)) return true;
+                if (planar_rr_f64(S, q[3 * i], q[3 * i + 1], q[3 * i + 2], i, q[3 * j], q[3 * j + 1],
+                                  q[3 * j + 2], j))
+                    return true;
             }
         }
     }
@@ -291,7 +298,7 @@ __device__ inline bool planar_collides(const PlanarView& S, const double* q, uns
             if (v <= M) {
                 if (v < -M) return true;
                 ++rechecks;
-                if (planar_ro_f64(S, q, i, o)) return true;
+                if (planar_ro_f64(S, q[3 * i], q[3 * i + 1], q[3 * i + 2], i, o)) return true;
             }
         }
     }
