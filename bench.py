@@ -306,7 +306,10 @@ def main():
                    "edges": args.edges, "edge_states": n_states, "resolution": res,
                    "spheres": pair_info["spheres"], "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": measured_traffic(),
+                     "traffic_source": TRAFFIC_FILE + " (rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, "
+                                                      "scaled by a 256 MiB copy)",
+                     "algorithmic_bytes_per_launch": args.edges * (2 * desc.dim * 8 + 4 + 1),
                      "kernel": "k_arm_persistent, all-free variant (no early exit), link broad phase on",
                      "flops_per_launch": rf["exec_flops"], "states_per_launch": rf["f_states"],
                      "ms_per_launch": k_ms, "run_split_ms": rf["f_split"],
@@ -327,6 +330,21 @@ def main():
         "secondary": secondary,
     }
     print(json.dumps(line), flush=True)
+
+
+TRAFFIC_FILE = "profiles/k_arm_persistent_c3_allfree_traffic.json"
+
+
+def measured_traffic():
+    """HBM bytes per launch of the roofline kernel, from the committed rocprofv3 counter summary
+    of this same workload (FETCH_SIZE and WRITE_SIZE, each scaled by what it reads for a copy of
+    known size); None when the summary is absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_FILE)
+    try:
+        with open(path) as f:
+            return float(json.load(f)["traffic_bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def copy_probe(dev):

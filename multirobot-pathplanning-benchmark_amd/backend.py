@@ -291,6 +291,19 @@ class MrccEnvironment(BaseProblem):
         self.scene.check_configs_device(qd, out, md, mask)
         return out.cpu().numpy().astype(bool)
 
+    def _batch_is_collision_free(self, qs, mode=None) -> bool:
+        """True iff every configuration of `qs` is collision free (abstract_env.py:278-299).
+
+        The reference's batched form (unused by its planners) evaluates its distances with
+        array reductions, whose rounding can differ from the per-configuration predicates by an
+        ulp at exact contact; this form applies the per-configuration predicates of
+        ``is_collision_free`` to every configuration, so it agrees with a loop over that method.
+        """
+        if len(qs) == 0:
+            return True
+        modes = None if mode is None or not isinstance(mode, (list, tuple)) else mode
+        return bool(np.all(self.batch_is_collision_free(qs, modes)))
+
     def batch_is_edge_collision_free(self, q1s, q2s, modes=None, resolution=None, include_endpoints=False,
                                      N_start=None, N_max=None, N=None, robots=None,
                                      tune: Optional[Tune] = None) -> np.ndarray:

@@ -271,12 +271,14 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     V.lpair_count = at<int>(d, o_lpc);
     sc->D = V.D;
     sc->n_robots = na;
+    V.grasp = 0;
+    for (unsigned char g : H.m_grasp) V.grasp |= (g != 0);
     // waves per workgroup: the most resident waves per CU (160 KiB of LDS) within 64 KiB per
     // workgroup; the scene is staged once per workgroup
     sc->arm_nw = 0;
     int best = 0;
     for (int nw = 4; nw >= 1; nw >>= 1) {
-        const size_t b = arm_lds_bytes(V.D, L, na, nb, S, V.NP, nw);
+        const size_t b = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.grasp, nw);
         if (b > 65536) continue;
         const int resident = (int)(163840 / b) * nw;
         if (resident > best) {
@@ -285,14 +287,13 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
         }
     }
     if (sc->arm_nw == 0) return fail(MRCC_ERR_INVALID, "arm scene needs more than 64 KiB of LDS per wave");
-    sc->arm_lds = arm_lds_bytes(V.D, L, na, nb, S, V.NP, sc->arm_nw);
-    sc->arm_lds1 = arm_lds_bytes(V.D, L, na, nb, S, V.NP, 1);
+    sc->arm_lds = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.grasp, sc->arm_nw);
+    sc->arm_lds1 = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.grasp, 1);
     sc->arm_na = na;
     sc->arm_dof = dof;
     sc->arm_S = S;
     sc->arm_nm = H.nm;
-    sc->arm_has_grasp = 0;
-    for (unsigned char g : H.m_grasp) sc->arm_has_grasp |= (g != 0);
+    sc->arm_has_grasp = V.grasp;
     return MRCC_OK;
 }
 

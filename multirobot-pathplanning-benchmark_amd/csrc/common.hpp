@@ -377,6 +377,7 @@ struct ArmView {
     float Mcull;             // binary32 margin of the bounding-sphere tests
     const double* link_bc64; // [L*3] bounding-sphere centre in the link frame
     const float* link_R32;   // [L] bounding radius (covers every sphere of the link, inflated)
+    int grasp;               // 1: some mode holds a box (the wave LDS region has held-box space)
     int NP;                  // link pairs in the pair list
     const int2* lpairs2;     // [NP] (l << 8 | m, fl | nl << 9 | fm << 13 | nm << 22): link pair
                              // l < m and the sphere ranges [fl, fl+nl), [fm, fm+nm); sorted by arm of m

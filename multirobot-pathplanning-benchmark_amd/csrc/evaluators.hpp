@@ -341,27 +341,30 @@ __host__ __device__ inline ArmStaticLayout arm_static_layout(int L, int na, int 
     return o;
 }
 
-// doubles of a wave region before the float4 sphere table, rounded up to an even count
-__host__ __device__ inline int arm_lds_doubles(int D, int L, int na) {
-    return (3 * D + 12 * L + 12 * na + 1) & ~1;
+#define ARM_PEND_MAX 64  // band pairs awaiting their binary64 recheck, per wave
+
+// Wave region: X [12 L] doubles, held boxes G64 [12 na] doubles (grasp scenes only), a union of
+// the joint cos / sin [2 D] doubles (kinematics phase) and the band-pair queue (narrow phase),
+// sphere table [S] and link spheres [L] float4, candidates [64] int, held boxes G32 [16 na] float
+// (grasp scenes only).  Every part is a multiple of 16 bytes.
+__host__ __device__ inline int arm_union_bytes(int D) {
+    return 16 * D > 8 * ARM_PEND_MAX ? 16 * D : 8 * ARM_PEND_MAX;
 }
 
-#define ARM_PEND_MAX 128  // band pairs awaiting their binary64 recheck, per wave
-
-__host__ __device__ inline int arm_wave_bytes(int D, int L, int na, int S) {
-    return align16(8 * arm_lds_doubles(D, L, na) + 16 * (S + L) + 8 * ARM_PEND_MAX + 4 * 64 + 4 * 16 * na);
+__host__ __device__ inline int arm_wave_bytes(int D, int L, int na, int S, int grasp) {
+    return 96 * L + (grasp ? 96 * na : 0) + arm_union_bytes(D) + 16 * (S + L) + 4 * 64 + (grasp ? 64 * na : 0);
 }
 
 // dynamic LDS of an arm kernel launched with `nw` waves per workgroup
-__host__ __device__ inline size_t arm_lds_bytes(int D, int L, int na, int nb, int S, int NP, int nw) {
-    return (size_t)arm_static_layout(L, na, nb, S, NP).bytes + (size_t)nw * arm_wave_bytes(D, L, na, S);
+__host__ __device__ inline size_t arm_lds_bytes(int D, int L, int na, int nb, int S, int NP, int grasp,
+                                                int nw) {
+    return (size_t)arm_static_layout(L, na, nb, S, NP).bytes + (size_t)nw * arm_wave_bytes(D, L, na, S, grasp);
 }
 
 struct ArmLds {
     // per wave
-    double* q;
-    double* cs;
-    double* sn;
+    double* cs;     // [D] joint cosines (kinematics phase; shares memory with pend)
+    double* sn;     // [D]
     double* X;
     double* G64;
     float4* sph;
@@ -403,16 +406,18 @@ __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
     W.linkR = (const float*)(b + o.o_linkR);
     W.linkFC = (const int*)(b + o.o_linkFC);
     W.pairs = V.NP <= ARM_PAIRS_LDS_MAX ? (const int2*)(b + o.o_pairs) : V.lpairs2;
-    double* d = (double*)(b + o.bytes + (threadIdx.x >> 6) * arm_wave_bytes(V.D, V.L, V.na, V.S));
-    W.q = d;
-    W.cs = d + V.D;
-    W.sn = d + 2 * V.D;
-    W.X = d + 3 * V.D;
-    W.G64 = W.X + 12 * V.L;
-    W.sph = (float4*)(d + arm_lds_doubles(V.D, V.L, V.na));
+    unsigned char* w = b + o.bytes + (threadIdx.x >> 6) * arm_wave_bytes(V.D, V.L, V.na, V.S, V.grasp);
+    W.X = (double*)w;
+    w += 96 * V.L;
+    W.G64 = (double*)w;
+    if (V.grasp) w += 96 * V.na;
+    W.cs = (double*)w;
+    W.sn = W.cs + V.D;
+    W.pend = (int2*)w;
+    w += arm_union_bytes(V.D);
+    W.sph = (float4*)w;
     W.linkC = W.sph + V.S;
-    W.pend = (int2*)(W.linkC + V.L);
-    W.cand = (int*)(W.pend + ARM_PEND_MAX);
+    W.cand = (int*)(W.linkC + V.L);
     W.G32 = (float*)(W.cand + 64);
     return W;
 }
@@ -561,11 +566,12 @@ __device__ inline float box_d2(float4 c, float4 lo, float4 hi) {
     return fmaf(e2, e2, fmaf(e1, e1, e0 * e0));
 }
 
-// Evaluates the state whose binary64 joint vector is in W.q (written by lanes < D before the
-// call).  All 64 lanes of the wave must call.  Returns the wave-uniform result.
+// Evaluates the state whose binary64 joint vector is distributed over the lanes (lane k < D
+// holds coordinate k in `qv`).  All 64 lanes of the wave must call.  Returns the wave-uniform
+// result.
 // GRASP = false instantiates the evaluator for scenes without held boxes (fewer registers).
 template <bool GRASP>
-__device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, const ArmRegs& R,
+__device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, const ArmRegs& R, double qv,
                                           int mode, unsigned mask, unsigned& rechecks, ArmCount& cnt) {
     const int lane = wave_lane();
     if (mode < 0 || mode >= V.nm) mode = 0;
@@ -575,7 +581,6 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
     // 1. joint sines / cosines (binary64), finiteness
     bool bad = false;
     if (lane < Dact) {
-        const double qv = W.q[lane];
         bad = !__builtin_isfinite(qv);
         SinCos t = sincos_f64(qv);
         W.cs[lane] = t.c;
@@ -684,24 +689,26 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             if (__any(hit)) return true;
         }
         // 5'. links vs boxes (boxes in registers, lane b holds box b), then the spheres of
-        //     candidate (link, box) pairs: 8 candidates x 8 spheres per wave iteration
-        for (int l = 0; l < Dact; ++l) {
-            if (!((mask >> (l / dof)) & 1u)) continue;
+        //     candidate (link, box) pairs: 8 candidates x 8 spheres per wave iteration.  A link
+        //     whose bounding sphere clears the box around all obstacles cannot touch any box:
+        //     lane l tests link l against that box first, and only the surviving links are walked.
+        bool lsurv = false;
+        if (lane < Dact && ((mask >> (lane / dof)) & 1u) && (R.linkFC >> 16) > 0) {
+            const float e0 = fminf(fmaxf(myLink.x, V.bb_lo[0]), V.bb_hi[0]) - myLink.x;
+            const float e1 = fminf(fmaxf(myLink.y, V.bb_lo[1]), V.bb_hi[1]) - myLink.y;
+            const float e2 = fminf(fmaxf(myLink.z, V.bb_lo[2]), V.bb_hi[2]) - myLink.z;
+            lsurv = fmaf(e2, e2, fmaf(e1, e1, e0 * e0)) <= myLink.w * myLink.w + Mc;
+        }
+        for (unsigned long long lm = __ballot(lsurv); lm; lm &= lm - 1) {
+            const int l = __builtin_ctzll(lm);
             const int fc = __builtin_amdgcn_readlane(R.linkFC, l);
             const int f = fc & 0xffff, ns = fc >> 16;
-            if (ns == 0) continue;
             float4 Lc;
             Lc.x = readlane_f(myLink.x, l);
             Lc.y = readlane_f(myLink.y, l);
             Lc.z = readlane_f(myLink.z, l);
             Lc.w = readlane_f(myLink.w, l);
             const float Tl = Lc.w * Lc.w;
-            {   // the link's bounding sphere clears the box around all obstacles: no box can touch it
-                const float e0 = fminf(fmaxf(Lc.x, V.bb_lo[0]), V.bb_hi[0]) - Lc.x;
-                const float e1 = fminf(fmaxf(Lc.y, V.bb_lo[1]), V.bb_hi[1]) - Lc.y;
-                const float e2 = fminf(fmaxf(Lc.z, V.bb_lo[2]), V.bb_hi[2]) - Lc.z;
-                if (fmaf(e2, e2, fmaf(e1, e1, e0 * e0)) > Tl + Mc) continue;
-            }
             for (int b0 = 0; b0 < V.nb; b0 += 64) {
                 const int b = b0 + lane;
                 bool cand = false;
@@ -748,8 +755,9 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             const bool act = s < S_act && ((mask >> (W.sphLink[s < S_act ? s : 0] / dof)) & 1u);
             const float4 c = W.sph[act ? s : 0];
             const float T = c.w * c.w;
+            const float4* gbox = (const float4*)V.box32;  // uniform index: scalar loads
             for (int b = 0; b < V.nb; ++b) {
-                const float d2 = box_d2(c, W.box[2 * b], W.box[2 * b + 1]);
+                const float d2 = box_d2(c, gbox[2 * b], gbox[2 * b + 1]);
                 cnt.sb += act;
                 bool band = false;
                 if (act && d2 <= T + M) {

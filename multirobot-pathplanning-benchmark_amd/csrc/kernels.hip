@@ -35,7 +35,8 @@ __device__ inline int edge_N_dev(const EdgeBatchView& B, long long e, int D) {
     const double* a = row1(B, e);
     const double* b = row2(B, e);
     double m = __builtin_fabs(a[0] - b[0]);
-    for (int k = 1; k < D; ++k) {
+#pragma unroll 8
+    for (int k = 1; k < D; ++k) {  // unrolled: the row loads of several coordinates in flight
         double v = __builtin_fabs(a[k * B.ds] - b[k * B.ds]);
         m = v > m ? v : m;
     }
@@ -432,15 +433,16 @@ __global__ void __launch_bounds__(256) k_lane_configs(View S, const double* qv, 
 // arms: one state per wavefront.  A workgroup of nw = blockDim.x / 64 waves stages the scene into
 // LDS once; each wave then works as an independent worker (worker id blockIdx.x * nw + wave)
 // with wave-level synchronisation only.
-__device__ inline void arm_load_state(const ArmLds& Wl, const EdgeBatchView& B, long long e, int N,
-                                      int i, int Dact) {
+// coordinate `lane` of state i of edge e (lanes >= Dact: 0)
+__device__ inline double arm_load_state(const EdgeBatchView& B, long long e, int N, int i, int Dact) {
     const int lane = wave_lane();
+    double qv = 0.0;
     if (lane < Dact) {
         const double* a = row1(B, e);
         const double* b = row2(B, e);
-        Wl.q[lane] = interp_f64(a[lane * B.ds], b[lane * B.ds], N, i);
+        qv = interp_f64(a[lane * B.ds], b[lane * B.ds], N, i);
     }
-    wave_sync();
+    return qv;
 }
 
 __device__ inline void arm_stats(unsigned long long* stats, unsigned rechecks, const ArmCount& cnt) {
@@ -511,9 +513,9 @@ __global__ void __launch_bounds__(256) k_arm_persistent(ArmView V, EdgeBatchView
             }
             const int mode = B.modes ? B.modes[ej] : 0;
             const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
-            arm_load_state(Wl, B, ej, Nj, ij, V.m_active[md] * V.dof);
+            const double qv = arm_load_state(B, ej, Nj, ij, V.m_active[md] * V.dof);
             ++n_eval;
-            if (arm_state_collides<GRASP>(V, Wl, R, mode, B.mask, rechecks, cnt)) {
+            if (arm_state_collides<GRASP>(V, Wl, R, qv, mode, B.mask, rechecks, cnt)) {
                 if (lane == 0) B.out[ej] = 0;
             }
         }
@@ -563,9 +565,9 @@ __global__ void __launch_bounds__(256) k_arm_direct(ArmView V, EdgeBatchView B, 
             if (!init_out && B.out[e] == 0) break;
             const int i = edge_state_index(w, local, B.incl);
             if (i < 0) continue;
-            arm_load_state(Wl, B, e, N, i, V.m_active[md] * V.dof);
+            const double qv = arm_load_state(B, e, N, i, V.m_active[md] * V.dof);
             ++n_eval;
-            if (arm_state_collides<GRASP>(V, Wl, R, mode, B.mask, rechecks, cnt)) {
+            if (arm_state_collides<GRASP>(V, Wl, R, qv, mode, B.mask, rechecks, cnt)) {
                 hit = true;
                 break;
             }
@@ -596,9 +598,8 @@ __global__ void __launch_bounds__(256) k_arm_configs(ArmView V, const double* qv
         const int mode = modes ? modes[r] : 0;
         const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
         const int Dact = V.m_active[md] * V.dof;
-        if (lane < Dact) Wl.q[lane] = qv[r * rs + (long long)lane * ds];
-        wave_sync();
-        const bool c = arm_state_collides<GRASP>(V, Wl, R, mode, mask, rechecks, cnt);
+        const double q = lane < Dact ? qv[r * rs + (long long)lane * ds] : 0.0;
+        const bool c = arm_state_collides<GRASP>(V, Wl, R, q, mode, mask, rechecks, cnt);
         if (lane == 0) out[r] = c ? 0 : 1;
     }
 }

@@ -184,6 +184,21 @@ def test_abstract_test_kat(B):
         assert env.is_collision_free_for_robot("a1", a, None) == o.config_free(a, mask=1)
 
 
+def test_abstract_batch_forms(B):
+    _say("abstract.test batch forms: batch_is_collision_free, _batch_is_collision_free")
+    env = pkg.get_env_by_name("abstract.test")
+    o = O.Scene(env.desc)
+    rng = np.random.default_rng(18)
+    Q = rng.uniform(-2, 2, size=(500, 4))
+    ref = o.check_configs(Q)
+    assert np.array_equal(env.batch_is_collision_free(Q), ref)
+    free = [env.start_pos.from_flat(q) for q in Q[ref][:40]]
+    hit = [env.start_pos.from_flat(q) for q in Q[~ref][:1]]
+    assert env._batch_is_collision_free(free, None) is True
+    assert env._batch_is_collision_free(free + hit, None) is False
+    assert env._batch_is_collision_free([], None) is True
+
+
 def test_path_schedules_match_sequential(B):
     _say("batched is_path_collision_free == reference sequential schedules")
     env = pkg.get_env_by_name("abstract.center_rect_2d")
