@@ -22,7 +22,7 @@ CACHE = os.environ.get("MRCC_AUTOTUNE_CACHE", os.path.join(os.path.expanduser("~
 LANE_GRID = [dict(slots_per_pass=g, slots_per_ticket=t, waves_per_cu=w)
              for g in (4, 8, 16, 32) for t in (1, 2) for w in (8, 16, 32)]
 ARM_GRID = [dict(slots_per_pass=g, slots_per_ticket=t, waves_per_cu=w)
-            for g in (1, 2, 4) for t in (8, 16, 32) for w in (8, 12, 16)]
+            for g in (1, 2, 4, 8) for t in (8, 16, 32) for w in (12, 16)]
 
 
 def _p2(x: float) -> int:

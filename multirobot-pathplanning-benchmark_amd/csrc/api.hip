@@ -206,7 +206,7 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
                 if (al == bm && m - l < 2) continue;
                 if (link_first[l] == link_first[l + 1] || link_first[m] == link_first[m + 1]) continue;
                 lpairs.push_back((l << 8) | m);
-                lpairs2.push_back((l << 8) | m);
+                lpairs2.push_back(m | (l << 8) | (bm << 16) | (al << 20));
                 lpairs2.push_back(link_first[l] | (H.counts[l] << 9) | (link_first[m] << 13) | (H.counts[m] << 22));
             }
     for (int na_ = 1; na_ <= na; ++na_) {
@@ -657,7 +657,7 @@ int mrcc_edge_plan_create(const mrcc_scene* scene, const mrcc_edge_batch* batch,
     memset(&t, 0, sizeof(t));
     if (tune) t = *tune;
     const bool arms = sc->kind == KIND_ARMS;
-    if (t.slots_per_pass <= 0) t.slots_per_pass = arms ? 1 : 8;
+    if (t.slots_per_pass <= 0) t.slots_per_pass = arms ? 4 : 8;
     if (t.slots_per_ticket <= 0) t.slots_per_ticket = arms ? 16 : 4;
     if (t.waves_per_cu <= 0) t.waves_per_cu = arms ? 16 : 16;
     if (!tune) {

@@ -379,8 +379,9 @@ struct ArmView {
     const float* link_R32;   // [L] bounding radius (covers every sphere of the link, inflated)
     int grasp;               // 1: some mode holds a box (the wave LDS region has held-box space)
     int NP;                  // link pairs in the pair list
-    const int2* lpairs2;     // [NP] (l << 8 | m, fl | nl << 9 | fm << 13 | nm << 22): link pair
-                             // l < m and the sphere ranges [fl, fl+nl), [fm, fm+nm); sorted by arm of m
+    const int2* lpairs2;     // [NP] (m | l << 8 | arm(m) << 16 | arm(l) << 20,
+                             //  fl | nl << 9 | fm << 13 | nm << 22): link pair l < m and the sphere
+                             // ranges [fl, fl+nl), [fm, fm+nm); sorted by arm of m
     const int* lpair_count;  // [na+1] pairs whose arms are all < n_active
     float bb_lo[3], bb_hi[3];   // box containing every obstacle box (binary32, rounded outward)
 };

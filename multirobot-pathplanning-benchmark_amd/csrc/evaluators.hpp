@@ -379,7 +379,7 @@ struct ArmLds {
     const double* linkBC;  // [L*3] bounding-sphere centres in the link frame
     const float4* box;     // [nb*2] lo, hi
     const float* sphR;     // [S]
-    const int* sphLink;    // [S]
+    const int* sphLink;    // [S] link | arm << 8
     const int* sphLo;      // [S]
     const float* linkR;    // [L]
     const int* linkFC;     // [L] first sphere | count << 16
@@ -437,7 +437,7 @@ __device__ inline void arm_stage_scene(void* base, const ArmView& V) {
         sphL[3 * i + 1] = V.sph64[4 * i + 1];
         sphL[3 * i + 2] = V.sph64[4 * i + 2];
         ((float*)(b + o.o_sphR))[i] = V.sph_r32[i];
-        ((int*)(b + o.o_sphLink))[i] = V.sph_link[i];
+        ((int*)(b + o.o_sphLink))[i] = V.sph_link[i] | ((V.sph_link[i] / V.dof) << 8);  // link | arm << 8
         ((int*)(b + o.o_sphLo))[i] = V.sph_lo[i];
     }
     double* bc = (double*)(b + o.o_linkBC);
@@ -494,12 +494,12 @@ __device__ inline bool arm_flush(const ArmView& V, const ArmLds& W, int& np, uns
         const int2 e = W.pend[k];
         const int i = e.y >> 16, j = e.y & 0xffff;
         double a[3];
-        xform_point_f64(W.X + 12 * W.sphLink[i], W.sphL[3 * i], W.sphL[3 * i + 1], W.sphL[3 * i + 2], a);
+        xform_point_f64(W.X + 12 * (W.sphLink[i] & 255), W.sphL[3 * i], W.sphL[3 * i + 1], W.sphL[3 * i + 2], a);
         if (e.x == 0) {
             hit |= sph_box_f64(a, V.sph64[4 * i + 3], V.box64 + 6 * j, V.box64 + 6 * j + 3);
         } else {
             double b[3];
-            xform_point_f64(W.X + 12 * W.sphLink[j], W.sphL[3 * j], W.sphL[3 * j + 1], W.sphL[3 * j + 2], b);
+            xform_point_f64(W.X + 12 * (W.sphLink[j] & 255), W.sphL[3 * j], W.sphL[3 * j + 1], W.sphL[3 * j + 2], b);
             hit |= sph_sph_f64(a, V.sph64[4 * i + 3], b, V.sph64[4 * j + 3]);
         }
     }
@@ -527,7 +527,7 @@ __device__ inline bool arm_flush_held(const ArmView& V, const ArmLds& W, int mod
             hit |= obb3_overlap_f64(A, bc + 3, W.G64 + 12 * i, V.m_half + 3 * (mode * V.na + i));
         } else if (e.x == 3) {
             double c[3];
-            xform_point_f64(W.X + 12 * W.sphLink[i], W.sphL[3 * i], W.sphL[3 * i + 1], W.sphL[3 * i + 2], c);
+            xform_point_f64(W.X + 12 * (W.sphLink[i] & 255), W.sphL[3 * i], W.sphL[3 * i + 1], W.sphL[3 * i + 2], c);
             hit |= sph_obb_f64(c, V.sph64[4 * i + 3], W.G64 + 12 * j, V.m_half + 3 * (mode * V.na + j));
         } else {
             hit |= obb3_overlap_f64(W.G64 + 12 * i, V.m_half + 3 * (mode * V.na + i), W.G64 + 12 * j,
@@ -623,7 +623,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
     for (int s = lane; s < S_act; s += 64) {
         double c[3];
         const double* l = W.sphL + 3 * s;
-        xform_point_f64(W.X + 12 * W.sphLink[s], l[0], l[1], l[2], c);
+        xform_point_f64(W.X + 12 * (W.sphLink[s] & 255), l[0], l[1], l[2], c);
         W.sph[s] = make_float4((float)c[0], (float)c[1], (float)c[2], W.sphR[s]);
     }
     float4 myLink = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -638,6 +638,8 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
     const float M = V.Msph;
     bool hit = false;
     int np = 0;  // queued band pairs
+    const unsigned amask = na >= 32 ? ~0u : ((1u << na) - 1u);
+    const bool all_arms = (mask & amask) == amask;  // no pair is masked out
     if (V.broadphase) {
         const float Mc = V.Mcull;
         // 4'. link pairs of the pair set, then the sphere pairs of candidate link pairs (8 x 8);
@@ -646,12 +648,12 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
         for (int p0 = 0; p0 < NP; p0 += 64) {
             const int pi = p0 + lane;
             bool cand = false;
-            int v = 0;
+            int y = 0;
             if (pi < NP) {
                 const int2 pr = W.pairs[pi];
-                const int l = pr.x >> 8, mm = pr.x & 255;
-                v = pi;
-                if (((mask >> (l / dof)) & 1u) && ((mask >> (mm / dof)) & 1u)) {
+                const int l = (pr.x >> 8) & 255, mm = pr.x & 255;
+                y = pr.y;
+                if (all_arms || (((mask >> ((pr.x >> 20) & 15)) & 1u) && ((mask >> ((pr.x >> 16) & 15)) & 1u))) {
                     const float4 A = W.linkC[l], Bc = W.linkC[mm];
                     const float dx = A.x - Bc.x, dy = A.y - Bc.y, dz = A.z - Bc.z;
                     const float Rs = A.w + Bc.w;
@@ -661,12 +663,12 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             }
             const unsigned long long m = __ballot(cand);
             if (m == 0ull) continue;
-            if (cand) W.cand[lanes_below(m)] = v;
+            if (cand) W.cand[lanes_below(m)] = y;  // the pair's sphere ranges
             wave_sync();
             const int n = __popcll(m);
             for (int c = 0; c < n; ++c) {
-                const int2 pr = W.pairs[W.cand[c]];
-                const int fl = pr.y & 511, nl = (pr.y >> 9) & 15, fm = (pr.y >> 13) & 511, nm = (pr.y >> 22) & 15;
+                const int yc = W.cand[c];
+                const int fl = yc & 511, nl = (yc >> 9) & 15, fm = (yc >> 13) & 511, nm = (yc >> 22) & 15;
                 const int i = fl + (lane >> 3), j = fm + (lane & 7);
                 bool band = false;
                 if ((lane >> 3) < nl && (lane & 7) < nm) {
@@ -693,7 +695,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
         //     whose bounding sphere clears the box around all obstacles cannot touch any box:
         //     lane l tests link l against that box first, and only the surviving links are walked.
         bool lsurv = false;
-        if (lane < Dact && ((mask >> (lane / dof)) & 1u) && (R.linkFC >> 16) > 0) {
+        if (lane < Dact && (all_arms || ((mask >> (lane / dof)) & 1u)) && (R.linkFC >> 16) > 0) {
             const float e0 = fminf(fmaxf(myLink.x, V.bb_lo[0]), V.bb_hi[0]) - myLink.x;
             const float e1 = fminf(fmaxf(myLink.y, V.bb_lo[1]), V.bb_hi[1]) - myLink.y;
             const float e2 = fminf(fmaxf(myLink.z, V.bb_lo[2]), V.bb_hi[2]) - myLink.z;
@@ -752,7 +754,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
         // 4. spheres vs boxes (brute force)
         for (int base = 0; base < S_act; base += 64) {
             const int s = base + lane;
-            const bool act = s < S_act && ((mask >> (W.sphLink[s < S_act ? s : 0] / dof)) & 1u);
+            const bool act = s < S_act && ((mask >> (W.sphLink[s < S_act ? s : 0] >> 8)) & 1u);
             const float4 c = W.sph[act ? s : 0];
             const float T = c.w * c.w;
             const float4* gbox = (const float4*)V.box32;  // uniform index: scalar loads
@@ -772,13 +774,13 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
         // 5. sphere pairs: same arm links >= 2 apart, and different arms (brute force)
         for (int base = 0; base < S_act; base += 64) {
             const int i = base + lane;
-            const bool act = i < S_act && ((mask >> (W.sphLink[i < S_act ? i : 0] / dof)) & 1u);
+            const bool act = i < S_act && ((mask >> (W.sphLink[i < S_act ? i : 0] >> 8)) & 1u);
             const float4 ci = W.sph[act ? i : 0];
             const int lo = act ? W.sphLo[i] : 0x7fffffff;
             const int jstart = W.sphLo[base];
             int nj = 0;
             for (int j = jstart; j < S_act; ++j) {
-                const int aj = W.sphLink[j] / dof;
+                const int aj = W.sphLink[j] >> 8;
                 if (!((mask >> aj) & 1u)) continue;  // wave-uniform
                 const float4 cj = W.sph[j];
                 const float dx = ci.x - cj.x, dy = ci.y - cj.y, dz = ci.z - cj.z;
@@ -835,7 +837,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             const float* G = W.G32 + 16 * a;
             for (int base = 0; base < S_act; base += 64) {
                 const int s = base + lane;
-                const int as = s < S_act ? W.sphLink[s] / dof : a;
+                const int as = s < S_act ? W.sphLink[s] >> 8 : a;
                 const bool act = s < S_act && as != a && ((mask >> as) & 1u);
                 bool band = false;
                 if (act) {

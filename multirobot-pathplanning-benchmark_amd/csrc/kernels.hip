@@ -455,10 +455,12 @@ __device__ inline void arm_stats(unsigned long long* stats, unsigned rechecks, c
 }
 
 // Persistent arm kernel.  A wave claims a chunk of `spt` (<= 64) consecutive slots; lane j
-// resolves slot j of the chunk (slot -> edge, state; edge already dead?) so the dependent loads of
-// all slots overlap; the wave then evaluates the live states one after another.
+// resolves slot j of the chunk (slot -> edge, state, mode; edge already dead?) so the dependent
+// loads of all slots overlap; the wave then evaluates the live states one after another.  The
+// ticket of the next chunk is requested before those evaluations and read after them, so the
+// atomic's round trip overlaps the work.
 template <bool GRASP>
-__global__ void __launch_bounds__(256) k_arm_persistent(ArmView V, EdgeBatchView B, PlanWork W,
+__global__ void __launch_bounds__(256, 3) k_arm_persistent(ArmView V, EdgeBatchView B, PlanWork W,
                                                         int lgG, int spt) {
     extern __shared__ double lds_arm[];
     arm_stage_scene(lds_arm, V);
@@ -472,14 +474,14 @@ __global__ void __launch_bounds__(256) k_arm_persistent(ArmView V, EdgeBatchView
     int p = 0;  // pass hint (wave-uniform)
     int q = 0;  // queues exhausted so far
     const long long nchunks = (T + spt - 1) / spt;
-    for (;;) {
-        const long long c = claim_chunk(W, q, nchunks);
-        if (c < 0) break;
+    const int home = blockIdx.x & 7;
+    long long c = claim_chunk(W, q, nchunks);
+    while (c >= 0) {
         const long long s0 = c * spt;
         const long long rem = T - s0;
         const int n_here = (int)(rem < spt ? rem : spt);  // >= 1: c < nchunks
         // lane-parallel resolution of the chunk's slots
-        int e = -1, i = -1, N = 0;
+        int e = -1, i = -1, N = 0, mode = 0, Dact = 0;
         int pl = p;
         bool dead = false;
         if (lane < n_here) {
@@ -493,6 +495,9 @@ __global__ void __launch_bounds__(256) k_arm_persistent(ArmView V, EdgeBatchView
                     if (i >= 0) {
                         e = ref.e;
                         N = w.N;
+                        mode = B.modes ? B.modes[e] : 0;
+                        const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
+                        Dact = V.m_active[md] * V.dof;
                     }
                 }
             }
@@ -500,24 +505,35 @@ __global__ void __launch_bounds__(256) k_arm_persistent(ArmView V, EdgeBatchView
         p = __builtin_amdgcn_readlane(pl, n_here - 1);  // the chunk's last slot has the largest pass
         n_slots += n_here;
         n_dead += __popcll(__ballot(dead));
-        unsigned long long live = __ballot(e >= 0);
-        while (live) {
+        // request the next chunk's ticket now; read it after this chunk's evaluations
+        const int x = (home + q) & 7;
+        unsigned long long tk = 0;
+        if (lane == 0) tk = atomicAdd(W.ticket + 32 * x, 1ull);
+        int last_dead = -1;  // edge found in collision by this wave in this chunk
+        for (unsigned long long live = __ballot(e >= 0); live; live &= live - 1) {
             const int j = __builtin_ctzll(live);
-            live &= live - 1;
             const int ej = __builtin_amdgcn_readlane(e, j);
-            const int ij = __builtin_amdgcn_readlane(i, j);
-            const int Nj = __builtin_amdgcn_readlane(N, j);
-            if (B.out[ej] == 0) {  // killed by another wave since the chunk was resolved
+            if (ej == last_dead) {  // a later state of an edge this chunk already rejected
                 ++n_dead;
                 continue;
             }
-            const int mode = B.modes ? B.modes[ej] : 0;
-            const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
-            const double qv = arm_load_state(B, ej, Nj, ij, V.m_active[md] * V.dof);
+            const int ij = __builtin_amdgcn_readlane(i, j);
+            const int Nj = __builtin_amdgcn_readlane(N, j);
+            const int mj = __builtin_amdgcn_readlane(mode, j);
+            const int Dj = __builtin_amdgcn_readlane(Dact, j);
+            const double qv = arm_load_state(B, ej, Nj, ij, Dj);
             ++n_eval;
-            if (arm_state_collides<GRASP>(V, Wl, R, qv, mode, B.mask, rechecks, cnt)) {
+            if (arm_state_collides<GRASP>(V, Wl, R, qv, mj, B.mask, rechecks, cnt)) {
                 if (lane == 0) B.out[ej] = 0;
+                last_dead = ej;
             }
+        }
+        const unsigned long long k = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(tk >> 32)) << 32) |
+                                     (unsigned)__builtin_amdgcn_readfirstlane((unsigned)tk);
+        c = (long long)x + 8 * (long long)k;
+        if (c >= nchunks) {  // counter x is exhausted: continue on the others
+            ++q;
+            c = claim_chunk(W, q, nchunks);
         }
     }
     if (W.stats && lane == 0) {
@@ -582,7 +598,7 @@ __global__ void __launch_bounds__(256) k_arm_direct(ArmView V, EdgeBatchView B, 
 }
 
 template <bool GRASP>
-__global__ void __launch_bounds__(256) k_arm_configs(ArmView V, const double* qv, long long n,
+__global__ void __launch_bounds__(256, 3) k_arm_configs(ArmView V, const double* qv, long long n,
                                                      long long rs, long long ds, const int* modes,
                                                      unsigned mask, unsigned char* out) {
     extern __shared__ double lds_arm[];

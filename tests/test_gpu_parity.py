@@ -326,14 +326,16 @@ def test_plan_timing_and_stats(c3, B):
     ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
     out = torch.empty(len(a), dtype=torch.uint8, device="cuda")
     batch = sc.edge_batch(ta, tb, out, 0.05)
-    plan = B.EdgePlan(sc, batch, B.make_tune(use_graph=0, time_kernels=1, collect_stats=1))
+    G = 2
+    plan = B.EdgePlan(sc, batch, B.make_tune(slots_per_pass=G, use_graph=0, time_kernels=1, collect_stats=1))
     plan.run()
     t = plan.timing()
     assert 0.0 < t["kernel_ms"] <= t["run_ms"] and 0.0 <= t["build_ms"] <= t["run_ms"]
     assert np.array_equal(out.cpu().numpy().astype(bool), ref)
     st = plan.stats()
     N = np.maximum(2, (np.max(np.abs(b - a), axis=1) / 0.05).astype(np.int64) + 1)
-    assert st["slots"] == int(np.sum(N - 2))
+    # an edge of M = N - 2 states takes ceil(M / G) passes of G slots each
+    assert st["slots"] == int(np.sum(G * ((N - 2 + G - 1) // G)))
     assert st["states"] + st["dead_skips"] <= st["slots"]
     # every state of a free edge is evaluated; a colliding edge needs at least one
     assert st["states"] >= int(np.sum(N[ref] - 2)) + int(np.sum(~ref))
