@@ -594,9 +594,20 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
         const int a = lane / 3, r = lane - 3 * a;
         double R0 = W.base[12 * a + 3 * r], R1 = W.base[12 * a + 3 * r + 1], R2 = W.base[12 * a + 3 * r + 2];
         double t = W.base[12 * a + 9 + r];
+        // joint k's parameters are loaded one step ahead, off the chain's critical path
+        const int l0 = a * dof;
+        double p0 = W.dh[4 * l0], p1 = W.dh[4 * l0 + 1], p2 = W.dh[4 * l0 + 2], p3 = W.dh[4 * l0 + 3];
+        double cn = W.cs[l0], sn = W.sn[l0];
         for (int k = 0; k < dof; ++k) {
-            const int l = a * dof + k;
-            dh_row_f64(R0, R1, R2, t, W.dh + 4 * l, W.cs[l], W.sn[l]);
+            const int l = l0 + k;
+            const double pc[4] = {p0, p1, p2, p3};
+            const double c0 = cn, s0 = sn;
+            if (k + 1 < dof) {
+                p0 = W.dh[4 * l + 4]; p1 = W.dh[4 * l + 5]; p2 = W.dh[4 * l + 6]; p3 = W.dh[4 * l + 7];
+                cn = W.cs[l + 1];
+                sn = W.sn[l + 1];
+            }
+            dh_row_f64(R0, R1, R2, t, pc, c0, s0);
             double* X = W.X + 12 * l;
             X[3 * r] = R0;
             X[3 * r + 1] = R1;

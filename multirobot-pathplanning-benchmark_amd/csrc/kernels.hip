@@ -70,14 +70,54 @@ __global__ void k_plan_init(PlanWork W) {
 // addresses serialises in L2.
 #define PLAN_LDS_BINS 1024
 
+// PREP_LPE lanes share an edge: lane `sub` reads coordinates sub, sub + PREP_LPE, ... of both
+// rows (PREP_LPE consecutive doubles per row and step instead of one), the maximum is reduced
+// across the group, and lane sub == 0 writes the edge's queue entries.
+#define PREP_LPE 4
+
 __global__ void __launch_bounds__(256) k_plan_prep(EdgeBatchView B, PlanWork W, int D, int lgG,
                                                    int want_dir) {
     __shared__ int h[PLAN_LDS_BINS];
     for (int i = threadIdx.x; i < PLAN_LDS_BINS; i += blockDim.x) h[i] = 0;
     __syncthreads();
-    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < B.n;
-         e += (long long)gridDim.x * blockDim.x) {
-        const int N = edge_N_dev(B, e, D);
+    const int sub = threadIdx.x & (PREP_LPE - 1);
+    const long long per_grid = (long long)gridDim.x * (blockDim.x / PREP_LPE);
+    // the lanes of a group share e, so they run the same iterations and their shuffles stay within
+    // active lanes
+    for (long long e = (long long)blockIdx.x * (blockDim.x / PREP_LPE) + threadIdx.x / PREP_LPE; e < B.n;
+         e += per_grid) {
+        int N;
+        if (B.N) {
+            N = B.N[e];
+        } else {
+            // the sequential rule (edge_N_dev): m = |a0 - b0|, then m = v > m ? v : m, so a NaN
+            // in coordinate 0 sticks and a NaN elsewhere is skipped.  Lane 0 starts from
+            // coordinate 0, the other lanes from 0.0; lane 0's reduction keeps the rule.
+            const double* a = row1(B, e);
+            const double* b = row2(B, e);
+            double m = sub == 0 ? __builtin_fabs(a[0] - b[0]) : 0.0;
+#pragma unroll 4
+            for (int k = sub == 0 ? PREP_LPE : sub; k < D; k += PREP_LPE) {
+                const double v = __builtin_fabs(a[k * B.ds] - b[k * B.ds]);
+                m = v > m ? v : m;
+            }
+#pragma unroll
+            for (int o = 1; o < PREP_LPE; o <<= 1) {
+                const double v = __shfl_xor(m, o, PREP_LPE);
+                m = v > m ? v : m;
+            }
+            m = __shfl(m, 0, PREP_LPE);
+            N = edge_N_from_max(m, B.res);
+        }
+        if (want_dir && W.dir) {
+            const double* a = row1(B, e);
+            const double* b = row2(B, e);
+            for (int k = sub; k < D; k += PREP_LPE) {
+#pragma clang fp contract(off)
+                W.dir[e * D + k] = (b[k * B.ds] - a[k * B.ds]) / (double)(N - 1);
+            }
+        }
+        if (sub != 0) continue;
         const int ns = B.N_start ? B.N_start[e] : 0;
         const int nm = B.N_max ? B.N_max[e] : -1;
         EdgeWin w = edge_window(N, ns, nm, B.incl);
@@ -95,14 +135,6 @@ __global__ void __launch_bounds__(256) k_plan_prep(EdgeBatchView B, PlanWork W, 
         W.ew0[e] = w.w0;
         W.efull[e] = (unsigned char)w.full;
         B.out[e] = w.err ? 0 : 1;
-        if (want_dir && W.dir) {
-            const double* a = row1(B, e);
-            const double* b = row2(B, e);
-            for (int k = 0; k < D; ++k) {
-#pragma clang fp contract(off)
-                W.dir[e * D + k] = (b[k * B.ds] - a[k * B.ds]) / (double)(N - 1);
-            }
-        }
         if (S < PLAN_LDS_BINS) atomicAdd(&h[S], 1);
         else atomicAdd(&W.hist[S], 1);
     }
@@ -658,8 +690,9 @@ void launch_plan_build(const SceneRec& sc, const EdgeBatchView& B, const PlanWor
                        hipStream_t st) {
     const int want_dir = sc.kind != KIND_ARMS;
     k_plan_init<<<64, 256, 0, st>>>(W);
-    // a few edges per thread, so each block's LDS histogram covers many edges
-    int gp = grid_for(B.n, 4 * 256);
+    // PREP_LPE lanes per edge and a few edges per group, so each block's LDS histogram covers
+    // many edges
+    int gp = grid_for(B.n, 4 * (256 / PREP_LPE));
     if (gp > 1024) gp = 1024;
     int gs = grid_for(B.n, 256 * SCATTER_EPT);
     if (gs > 1024) gs = 1024;

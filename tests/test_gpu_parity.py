@@ -346,6 +346,28 @@ def test_plan_timing_and_stats(c3, B):
         B.EdgePlan(sc, batch, B.make_tune(slots_per_ticket=65))
 
 
+def test_c3_non_finite_edges(c3, B):
+    _say("C3 edges with NaN / inf away from coordinate 0 (N rule skips them; states collide)")
+    desc, sc, o, rng = c3
+    Q = near_home(rng, desc, 4000, sigma=0.4)
+    a = Q[o.check_configs(Q)]
+    b = np.clip(a + rng.normal(0, 0.05, a.shape), desc.limits[0], desc.limits[1])
+    keep = o.check_edges(a, b, 0.05)
+    a, b = a[keep][:60], b[keep][:60]  # free edges: the non-finite states decide the outcome
+    assert len(a) >= 12
+    a[::3, 5] = np.nan
+    b[1::3, 17] = np.nan
+    ref = o.check_edges(a, b, 0.05)
+    assert np.array_equal(gpu_edges(sc, a, b, 0.05), ref)
+    # a NaN in coordinate 0, or an infinite length, makes N unbounded: the batch is rejected, as
+    # int(nan) / int(inf) raise in the reference's N rule (abstract_env.py:321-323)
+    for k, v in ((0, np.nan), (9, np.inf)):
+        a2 = a.copy()
+        a2[2, k] = v
+        with pytest.raises(pkg.MrccError):
+            gpu_edges(sc, a2, b, 0.05)
+
+
 def test_c3_masks(c3):
     _say("C3 per-arm masks")
     desc, sc, o, rng = c3
