@@ -278,7 +278,7 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     sc->arm_nw = 0;
     int best = 0;
     for (int nw = 4; nw >= 1; nw >>= 1) {
-        const size_t b = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.grasp, nw);
+        const size_t b = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.grasp, nw);
         if (b > 65536) continue;
         const int resident = (int)(163840 / b) * nw;
         if (resident > best) {
@@ -287,8 +287,8 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
         }
     }
     if (sc->arm_nw == 0) return fail(MRCC_ERR_INVALID, "arm scene needs more than 64 KiB of LDS per wave");
-    sc->arm_lds = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.grasp, sc->arm_nw);
-    sc->arm_lds1 = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.grasp, 1);
+    sc->arm_lds = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.grasp, sc->arm_nw);
+    sc->arm_lds1 = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.grasp, 1);
     sc->arm_na = na;
     sc->arm_dof = dof;
     sc->arm_S = S;

@@ -315,13 +315,15 @@ __device__ inline bool planar_collides(const PlanarView& S, const double* q, uns
 #define ARM_BOXES_LDS_MAX 256   // obstacle box tables up to this length are staged in LDS
 
 struct ArmStaticLayout {
-    int o_dh, o_base, o_sphL, o_linkBC, o_box, o_sphR, o_sphLink, o_sphLo, o_linkR, o_linkFC, o_pairs;
+    int o_dh, o_base, o_sphL, o_linkBC, o_box, o_sphR, o_sphLink, o_sphLo, o_linkR, o_linkFC, o_mact, o_pairs;
     int bytes;
 };
 
+#define ARM_MODES_LDS_MAX 1024  // per-mode active arm counts up to this many modes are staged in LDS
+
 __host__ __device__ inline int align16(int x) { return (x + 15) & ~15; }
 
-__host__ __device__ inline ArmStaticLayout arm_static_layout(int L, int na, int nb, int S, int NP) {
+__host__ __device__ inline ArmStaticLayout arm_static_layout(int L, int na, int nb, int S, int NP, int nm) {
     ArmStaticLayout o;
     int b = 0;
     o.o_dh = b;      b += 8 * 4 * L;
@@ -335,6 +337,7 @@ __host__ __device__ inline ArmStaticLayout arm_static_layout(int L, int na, int 
     o.o_sphLo = b;   b += 4 * S;
     o.o_linkR = b;   b += 4 * L;
     o.o_linkFC = b;  b += 4 * L;
+    o.o_mact = b;    b += (nm <= ARM_MODES_LDS_MAX ? 4 * nm : 0);
     b = align16(b);
     o.o_pairs = b;   b += (NP <= ARM_PAIRS_LDS_MAX ? 8 * NP : 0);
     o.bytes = align16(b);
@@ -356,9 +359,9 @@ __host__ __device__ inline int arm_wave_bytes(int D, int L, int na, int S, int g
 }
 
 // dynamic LDS of an arm kernel launched with `nw` waves per workgroup
-__host__ __device__ inline size_t arm_lds_bytes(int D, int L, int na, int nb, int S, int NP, int grasp,
-                                                int nw) {
-    return (size_t)arm_static_layout(L, na, nb, S, NP).bytes + (size_t)nw * arm_wave_bytes(D, L, na, S, grasp);
+__host__ __device__ inline size_t arm_lds_bytes(int D, int L, int na, int nb, int S, int NP, int nm,
+                                                int grasp, int nw) {
+    return (size_t)arm_static_layout(L, na, nb, S, NP, nm).bytes + (size_t)nw * arm_wave_bytes(D, L, na, S, grasp);
 }
 
 struct ArmLds {
@@ -384,6 +387,7 @@ struct ArmLds {
     const float* linkR;    // [L]
     const int* linkFC;     // [L] first sphere | count << 16
     const int2* pairs;     // [NP] link pairs with their sphere ranges (LDS or global)
+    const int* mact;       // [nm] active arms of each mode (LDS or global)
 };
 
 // tests executed by one lane (broad phase link-box / sphere-box / link-pair / sphere-pair)
@@ -394,7 +398,7 @@ struct ArmCount {
 __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
     ArmLds W;
     unsigned char* b = (unsigned char*)base;
-    const ArmStaticLayout o = arm_static_layout(V.L, V.na, V.nb, V.S, V.NP);
+    const ArmStaticLayout o = arm_static_layout(V.L, V.na, V.nb, V.S, V.NP, V.nm);
     W.dh = (const double*)(b + o.o_dh);
     W.base = (const double*)(b + o.o_base);
     W.sphL = (const double*)(b + o.o_sphL);
@@ -406,6 +410,7 @@ __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
     W.linkR = (const float*)(b + o.o_linkR);
     W.linkFC = (const int*)(b + o.o_linkFC);
     W.pairs = V.NP <= ARM_PAIRS_LDS_MAX ? (const int2*)(b + o.o_pairs) : V.lpairs2;
+    W.mact = V.nm <= ARM_MODES_LDS_MAX ? (const int*)(b + o.o_mact) : V.m_active;
     unsigned char* w = b + o.bytes + (threadIdx.x >> 6) * arm_wave_bytes(V.D, V.L, V.na, V.S, V.grasp);
     W.X = (double*)w;
     w += 96 * V.L;
@@ -425,7 +430,7 @@ __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
 // Stages the scene into the static LDS region (all threads of the workgroup, then a barrier).
 __device__ inline void arm_stage_scene(void* base, const ArmView& V) {
     unsigned char* b = (unsigned char*)base;
-    const ArmStaticLayout o = arm_static_layout(V.L, V.na, V.nb, V.S, V.NP);
+    const ArmStaticLayout o = arm_static_layout(V.L, V.na, V.nb, V.S, V.NP, V.nm);
     const int t = threadIdx.x, nt = blockDim.x;
     double* dh = (double*)(b + o.o_dh);
     for (int i = t; i < 4 * V.L; i += nt) dh[i] = V.dh[i];
@@ -450,6 +455,10 @@ __device__ inline void arm_stage_scene(void* base, const ArmView& V) {
         float4* box = (float4*)(b + o.o_box);
         const float4* src = (const float4*)V.box32;
         for (int i = t; i < 2 * V.nb; i += nt) box[i] = src[i];
+    }
+    if (V.nm <= ARM_MODES_LDS_MAX) {
+        int* ma = (int*)(b + o.o_mact);
+        for (int i = t; i < V.nm; i += nt) ma[i] = V.m_active[i];
     }
     if (V.NP <= ARM_PAIRS_LDS_MAX) {
         int2* pr = (int2*)(b + o.o_pairs);
@@ -575,7 +584,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                                           int mode, unsigned mask, unsigned& rechecks, ArmCount& cnt) {
     const int lane = wave_lane();
     if (mode < 0 || mode >= V.nm) mode = 0;
-    const int na = V.m_active[mode];
+    const int na = W.mact[mode];
     const int dof = V.dof;
     const int Dact = na * dof;
     // 1. joint sines / cosines (binary64), finiteness
@@ -695,7 +704,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                     }
                 }
                 arm_pend(W, band, 1, i, j, np);
-                if ((c & 3) == 3 && __any(hit)) return true;
+                if (__any(hit)) return true;
                 if (np > ARM_PEND_MAX - 64 && arm_flush(V, W, np, rechecks)) return true;
             }
             wave_sync();

@@ -288,6 +288,29 @@ __device__ inline SlotRef slot_ref(const PlanWork& W, int& p, long long s, int l
     return ref;
 }
 
+// slot_ref with the wave's current pass [lo, hi) = [pass_start[p], pass_start[p + 1]) held in
+// scalar registers: a slot inside it needs no pass_start load.
+__device__ inline SlotRef slot_ref_hint(const PlanWork& W, int& p, long long lo, long long hi, long long s,
+                                        int lgG, long long n) {
+    if (s < lo || s >= hi) return slot_ref(W, p, s, lgG, n);
+    SlotRef ref;
+    ref.e = -1;
+    ref.local = 0;
+    const long long off = s - lo;
+    const long long r = off >> lgG;
+    const int k = (int)(off & ((1 << lgG) - 1));
+    if (r >= n) return ref;  // unreachable for a consistent queue
+    ref.e = W.order[r];
+    if (ref.e < 0 || ref.e >= n) {
+        ref.e = -1;
+        return ref;
+    }
+    const int S = W.eS[ref.e];
+    ref.local = k * S + p;
+    if (ref.local >= W.eM[ref.e]) ref.e = -1;
+    return ref;
+}
+
 __device__ inline EdgeWin win_of(const PlanWork& W, int e, int incl) {
     EdgeWin w;
     w.N = W.eN[e];
@@ -492,8 +515,8 @@ __device__ inline void arm_stats(unsigned long long* stats, unsigned rechecks, c
 // ticket of the next chunk is requested before those evaluations and read after them, so the
 // atomic's round trip overlaps the work.
 template <bool GRASP>
-__global__ void __launch_bounds__(256, 3) k_arm_persistent(ArmView V, EdgeBatchView B, PlanWork W,
-                                                        int lgG, int spt) {
+__device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView& B, const PlanWork& W,
+                                           int lgG, int spt) {
     extern __shared__ double lds_arm[];
     arm_stage_scene(lds_arm, V);
     const ArmLds Wl = arm_lds(lds_arm, V);
@@ -507,6 +530,7 @@ __global__ void __launch_bounds__(256, 3) k_arm_persistent(ArmView V, EdgeBatchV
     int q = 0;  // queues exhausted so far
     const long long nchunks = (T + spt - 1) / spt;
     const int home = blockIdx.x & 7;
+    long long ps_lo = W.pass_start[0], ps_hi = W.pass_start[1];  // pass p's slots (scalar)
     long long c = claim_chunk(W, q, nchunks);
     while (c >= 0) {
         const long long s0 = c * spt;
@@ -517,7 +541,7 @@ __global__ void __launch_bounds__(256, 3) k_arm_persistent(ArmView V, EdgeBatchV
         int pl = p;
         bool dead = false;
         if (lane < n_here) {
-            const SlotRef ref = slot_ref(W, pl, s0 + lane, lgG, B.n);
+            const SlotRef ref = slot_ref_hint(W, pl, ps_lo, ps_hi, s0 + lane, lgG, B.n);
             if (ref.e >= 0) {
                 if (B.out[ref.e] == 0) {
                     dead = true;
@@ -529,12 +553,19 @@ __global__ void __launch_bounds__(256, 3) k_arm_persistent(ArmView V, EdgeBatchV
                         N = w.N;
                         mode = B.modes ? B.modes[e] : 0;
                         const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
-                        Dact = V.m_active[md] * V.dof;
+                        Dact = Wl.mact[md] * V.dof;
                     }
                 }
             }
         }
-        p = __builtin_amdgcn_readlane(pl, n_here - 1);  // the chunk's last slot has the largest pass
+        {
+            const int pn = __builtin_amdgcn_readlane(pl, n_here - 1);  // the chunk's largest pass
+            if (pn != p) {
+                p = pn;
+                ps_lo = W.pass_start[p];
+                ps_hi = W.pass_start[p + 1];
+            }
+        }
         n_slots += n_here;
         n_dead += __popcll(__ballot(dead));
         // request the next chunk's ticket now; read it after this chunk's evaluations
@@ -576,6 +607,17 @@ __global__ void __launch_bounds__(256, 3) k_arm_persistent(ArmView V, EdgeBatchV
     arm_stats(W.stats, rechecks, cnt);
 }
 
+// Scenes without held boxes fit 4 waves per SIMD (<= 128 VGPRs); the grasp variant 3.
+__global__ void __launch_bounds__(256, 4) k_arm_persistent(ArmView V, EdgeBatchView B, PlanWork W, int lgG,
+                                                          int spt) {
+    arm_persistent_body<false>(V, B, W, lgG, spt);
+}
+
+__global__ void __launch_bounds__(256, 3) k_arm_persistent_grasp(ArmView V, EdgeBatchView B, PlanWork W,
+                                                                int lgG, int spt) {
+    arm_persistent_body<true>(V, B, W, lgG, spt);
+}
+
 // Work item k of n_list * wpe: edge list[k / wpe] (or k / wpe), states k % wpe, k % wpe + wpe, ...
 // With init_out (wpe == 1 only) the wave owns its edge: it writes the verdict without reading it.
 template <bool GRASP>
@@ -613,7 +655,7 @@ __global__ void __launch_bounds__(256) k_arm_direct(ArmView V, EdgeBatchView B, 
             if (!init_out && B.out[e] == 0) break;
             const int i = edge_state_index(w, local, B.incl);
             if (i < 0) continue;
-            const double qv = arm_load_state(B, e, N, i, V.m_active[md] * V.dof);
+            const double qv = arm_load_state(B, e, N, i, Wl.mact[md] * V.dof);
             ++n_eval;
             if (arm_state_collides<GRASP>(V, Wl, R, qv, mode, B.mask, rechecks, cnt)) {
                 hit = true;
@@ -645,7 +687,7 @@ __global__ void __launch_bounds__(256, 3) k_arm_configs(ArmView V, const double*
          r += (long long)gridDim.x * nw) {
         const int mode = modes ? modes[r] : 0;
         const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
-        const int Dact = V.m_active[md] * V.dof;
+        const int Dact = Wl.mact[md] * V.dof;
         const double q = lane < Dact ? qv[r * rs + (long long)lane * ds] : 0.0;
         const bool c = arm_state_collides<GRASP>(V, Wl, R, q, mode, mask, rechecks, cnt);
         if (lane == 0) out[r] = c ? 0 : 1;
@@ -721,9 +763,9 @@ void launch_plan_run(const SceneRec& sc, const EdgeBatchView& B, const PlanWork&
             const int nw = sc.arm_nw;
             const int blocks = (grid + nw - 1) / nw;
             if (sc.arm_has_grasp)
-                k_arm_persistent<true><<<blocks, 64 * nw, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
+                k_arm_persistent_grasp<<<blocks, 64 * nw, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
             else
-                k_arm_persistent<false><<<blocks, 64 * nw, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
+                k_arm_persistent<<<blocks, 64 * nw, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
             break;
         }
     }
