@@ -148,12 +148,50 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     }
     B += lmax + rmax;
     const int nb = (int)(H.boxes.size() / 6);
+    // Obstacle boxes are grouped into clusters of at most 8 for the two-level link-box test:
+    // median splits of the box centres along their widest axis.  The device arrays list the
+    // boxes cluster by cluster (the order of boxes does not change any result).
+    std::vector<int> perm(nb);
+    for (int b = 0; b < nb; ++b) perm[b] = b;
+    std::vector<std::pair<int, int>> leaves;
+    {
+        std::vector<std::pair<int, int>> todo;
+        if (nb > 0) todo.push_back({0, nb});
+        while (!todo.empty()) {
+            std::pair<int, int> r = todo.back();
+            todo.pop_back();
+            if (r.second - r.first <= 8) {
+                leaves.push_back(r);
+                continue;
+            }
+            double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (int i = r.first; i < r.second; ++i)
+                for (int k = 0; k < 3; ++k) {
+                    lo[k] = std::min(lo[k], H.boxes[6 * perm[i] + k]);
+                    hi[k] = std::max(hi[k], H.boxes[6 * perm[i] + k]);
+                }
+            int ax = 0;
+            for (int k = 1; k < 3; ++k)
+                if (hi[k] - lo[k] > hi[ax] - lo[ax]) ax = k;
+            const int mid = (r.first + r.second) / 2;
+            std::stable_sort(perm.begin() + r.first, perm.begin() + r.second, [&](int a, int b) {
+                return H.boxes[6 * a + ax] < H.boxes[6 * b + ax];
+            });
+            todo.push_back({mid, r.second});
+            todo.push_back({r.first, mid});
+        }
+        std::sort(leaves.begin(), leaves.end());
+    }
+    const int nclu = (int)leaves.size() <= 16 ? (int)leaves.size() : 0;  // more: one-level test
+    std::vector<double> bx(6 * (size_t)nb);
+    for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < 6; ++k) bx[6 * b + k] = H.boxes[6 * perm[b] + k];
     std::vector<double> box64(6 * (size_t)nb);
     std::vector<float> box32(8 * (size_t)nb), boxch32(6 * (size_t)nb);
     double hmax = 0.0;
     for (int b = 0; b < nb; ++b) {
         for (int k = 0; k < 3; ++k) {
-            const double c = H.boxes[6 * b + k], h = H.boxes[6 * b + 3 + k];
+            const double c = bx[6 * b + k], h = bx[6 * b + 3 + k];
             box64[6 * b + k] = c - h;
             box64[6 * b + 3 + k] = c + h;
             box32[8 * b + k] = (float)(c - h);
@@ -161,7 +199,27 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
             B = std::max(B, fabs(c) + fabs(h));
             hmax = std::max(hmax, fabs(h));
         }
-        for (int k = 0; k < 6; ++k) boxch32[6 * b + k] = (float)H.boxes[6 * b + k];
+        for (int k = 0; k < 6; ++k) boxch32[6 * b + k] = (float)bx[6 * b + k];
+    }
+    // cluster bounds in binary32, rounded outward: lo xyz, first box, hi xyz, box count
+    std::vector<float> clu32(8 * (size_t)(nclu > 0 ? nclu : 1), 0.0f);
+    for (int c = 0; c < nclu; ++c) {
+        const int f = leaves[c].first, e = leaves[c].second;
+        for (int k = 0; k < 3; ++k) {
+            double lo = INFINITY, hi = -INFINITY;
+            for (int b = f; b < e; ++b) {
+                lo = std::min(lo, box64[6 * b + k]);
+                hi = std::max(hi, box64[6 * b + 3 + k]);
+            }
+            float l32 = (float)lo, h32 = (float)hi;
+            if ((double)l32 > lo) l32 = nextafterf(l32, -INFINITY);
+            if ((double)h32 < hi) h32 = nextafterf(h32, INFINITY);
+            clu32[8 * c + k] = l32;
+            clu32[8 * c + 4 + k] = h32;
+        }
+        int fi = f, ni = e - f;
+        memcpy(&clu32[8 * c + 3], &fi, 4);
+        memcpy(&clu32[8 * c + 7], &ni, 4);
     }
     // held boxes: reach of the end-effector + grasp offset + box extent
     double grasp_reach = 0.0;
@@ -219,7 +277,7 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     size_t o_lbc = P.add(link_bc), o_lR = P.add(link_R), o_lp = P.add(lpairs2), o_lpc = P.add(lpair_count);
     size_t o_base = P.add(H.base), o_dh = P.add(dh4), o_lf = P.add(link_first), o_sl = P.add(sph_link),
            o_slo = P.add(sph_lo), o_s64 = P.add(H.sph), o_sr = P.add(sph_r32), o_b32 = P.add(box32),
-           o_b64 = P.add(box64), o_bch64 = P.add(H.boxes), o_bch32 = P.add(boxch32),
+           o_b64 = P.add(box64), o_bch64 = P.add(bx), o_bch32 = P.add(boxch32), o_clu = P.add(clu32),
            o_ma = P.add(H.m_active), o_mg = P.add(H.m_grasp), o_mr = P.add(H.m_rel),
            o_mh = P.add(H.m_half);
     const int bp = sc->dev_mem ? sc->arm.broadphase : 1;  // keep the setting across mode updates
@@ -264,6 +322,8 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
         V.bb_lo[k] = l32;
         V.bb_hi[k] = h32;
     }
+    V.nclu = nclu;
+    V.clu32 = at<float>(d, o_clu);
     V.link_bc64 = at<double>(d, o_lbc);
     V.link_R32 = at<float>(d, o_lR);
     V.NP = (int)lpairs.size();
@@ -278,7 +338,7 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     sc->arm_nw = 0;
     int best = 0;
     for (int nw = 4; nw >= 1; nw >>= 1) {
-        const size_t b = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.grasp, nw);
+        const size_t b = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, nw);
         if (b > 65536) continue;
         const int resident = (int)(163840 / b) * nw;
         if (resident > best) {
@@ -287,8 +347,8 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
         }
     }
     if (sc->arm_nw == 0) return fail(MRCC_ERR_INVALID, "arm scene needs more than 64 KiB of LDS per wave");
-    sc->arm_lds = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.grasp, sc->arm_nw);
-    sc->arm_lds1 = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.grasp, 1);
+    sc->arm_lds = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, sc->arm_nw);
+    sc->arm_lds1 = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, 1);
     sc->arm_na = na;
     sc->arm_dof = dof;
     sc->arm_S = S;

@@ -384,6 +384,8 @@ struct ArmView {
                              // ranges [fl, fl+nl), [fm, fm+nm); sorted by arm of m
     const int* lpair_count;  // [na+1] pairs whose arms are all < n_active
     float bb_lo[3], bb_hi[3];   // box containing every obstacle box (binary32, rounded outward)
+    int nclu;                // obstacle box clusters (0: the link-box test has one level)
+    const float* clu32;      // [nclu*8] cluster bounds: lo xyz, first box (int bits), hi xyz, count
 };
 
 // batch of edges as seen by kernels

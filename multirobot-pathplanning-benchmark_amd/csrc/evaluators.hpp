@@ -315,7 +315,7 @@ __device__ inline bool planar_collides(const PlanarView& S, const double* q, uns
 #define ARM_BOXES_LDS_MAX 256   // obstacle box tables up to this length are staged in LDS
 
 struct ArmStaticLayout {
-    int o_dh, o_base, o_sphL, o_linkBC, o_box, o_sphR, o_sphLink, o_sphLo, o_linkR, o_linkFC, o_mact, o_pairs;
+    int o_dh, o_base, o_sphL, o_linkBC, o_box, o_clu, o_sphR, o_sphLink, o_linkR, o_linkFC, o_mact, o_pairs;
     int bytes;
 };
 
@@ -323,7 +323,8 @@ struct ArmStaticLayout {
 
 __host__ __device__ inline int align16(int x) { return (x + 15) & ~15; }
 
-__host__ __device__ inline ArmStaticLayout arm_static_layout(int L, int na, int nb, int S, int NP, int nm) {
+__host__ __device__ inline ArmStaticLayout arm_static_layout(int L, int na, int nb, int S, int NP, int nm,
+                                                           int nclu) {
     ArmStaticLayout o;
     int b = 0;
     o.o_dh = b;      b += 8 * 4 * L;
@@ -332,9 +333,9 @@ __host__ __device__ inline ArmStaticLayout arm_static_layout(int L, int na, int 
     o.o_linkBC = b;  b += 8 * 3 * L;
     b = align16(b);
     o.o_box = b;     b += (nb <= ARM_BOXES_LDS_MAX ? 32 * nb : 0);  // lo xyz pad, hi xyz pad
+    o.o_clu = b;     b += 32 * nclu;
     o.o_sphR = b;    b += 4 * S;
     o.o_sphLink = b; b += 4 * S;
-    o.o_sphLo = b;   b += 4 * S;
     o.o_linkR = b;   b += 4 * L;
     o.o_linkFC = b;  b += 4 * L;
     o.o_mact = b;    b += (nm <= ARM_MODES_LDS_MAX ? 4 * nm : 0);
@@ -355,13 +356,13 @@ __host__ __device__ inline int arm_union_bytes(int D) {
 }
 
 __host__ __device__ inline int arm_wave_bytes(int D, int L, int na, int S, int grasp) {
-    return 96 * L + (grasp ? 96 * na : 0) + arm_union_bytes(D) + 16 * (S + L) + 4 * 64 + (grasp ? 64 * na : 0);
+    return 96 * L + (grasp ? 96 * na : 0) + arm_union_bytes(D) + 16 * (S + L) + 3 * 4 * 64 + (grasp ? 64 * na : 0);
 }
 
 // dynamic LDS of an arm kernel launched with `nw` waves per workgroup
 __host__ __device__ inline size_t arm_lds_bytes(int D, int L, int na, int nb, int S, int NP, int nm,
-                                                int grasp, int nw) {
-    return (size_t)arm_static_layout(L, na, nb, S, NP, nm).bytes + (size_t)nw * arm_wave_bytes(D, L, na, S, grasp);
+                                                int nclu, int grasp, int nw) {
+    return (size_t)arm_static_layout(L, na, nb, S, NP, nm, nclu).bytes + (size_t)nw * arm_wave_bytes(D, L, na, S, grasp);
 }
 
 struct ArmLds {
@@ -374,6 +375,8 @@ struct ArmLds {
     float4* linkC;  // [L] bounding sphere (x, y, z, R) of each link
     int2* pend;     // [ARM_PEND_MAX] (kind, i << 16 | j): kind 0 sphere i - box j, 1 sphere pair
     int* cand;      // [64] compacted candidates of one broad-phase round
+    int* lst;       // [64] links that pass the cull against the box around all obstacles
+    int* lbc;       // [64] (link, box) candidates of the two-level link-box test
     float* G32;
     // per workgroup (the scene)
     const double* dh;      // [L*4] d a cos(alpha) sin(alpha)
@@ -383,7 +386,7 @@ struct ArmLds {
     const float4* box;     // [nb*2] lo, hi
     const float* sphR;     // [S]
     const int* sphLink;    // [S] link | arm << 8
-    const int* sphLo;      // [S]
+    const float4* clu;     // [nclu*2] box clusters: lo xyz | first box, hi xyz | box count
     const float* linkR;    // [L]
     const int* linkFC;     // [L] first sphere | count << 16
     const int2* pairs;     // [NP] link pairs with their sphere ranges (LDS or global)
@@ -398,7 +401,7 @@ struct ArmCount {
 __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
     ArmLds W;
     unsigned char* b = (unsigned char*)base;
-    const ArmStaticLayout o = arm_static_layout(V.L, V.na, V.nb, V.S, V.NP, V.nm);
+    const ArmStaticLayout o = arm_static_layout(V.L, V.na, V.nb, V.S, V.NP, V.nm, V.nclu);
     W.dh = (const double*)(b + o.o_dh);
     W.base = (const double*)(b + o.o_base);
     W.sphL = (const double*)(b + o.o_sphL);
@@ -406,7 +409,7 @@ __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
     W.box = V.nb <= ARM_BOXES_LDS_MAX ? (const float4*)(b + o.o_box) : (const float4*)V.box32;
     W.sphR = (const float*)(b + o.o_sphR);
     W.sphLink = (const int*)(b + o.o_sphLink);
-    W.sphLo = (const int*)(b + o.o_sphLo);
+    W.clu = (const float4*)(b + o.o_clu);
     W.linkR = (const float*)(b + o.o_linkR);
     W.linkFC = (const int*)(b + o.o_linkFC);
     W.pairs = V.NP <= ARM_PAIRS_LDS_MAX ? (const int2*)(b + o.o_pairs) : V.lpairs2;
@@ -423,14 +426,16 @@ __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
     W.sph = (float4*)w;
     W.linkC = W.sph + V.S;
     W.cand = (int*)(W.linkC + V.L);
-    W.G32 = (float*)(W.cand + 64);
+    W.lst = W.cand + 64;
+    W.lbc = W.lst + 64;
+    W.G32 = (float*)(W.lbc + 64);
     return W;
 }
 
 // Stages the scene into the static LDS region (all threads of the workgroup, then a barrier).
 __device__ inline void arm_stage_scene(void* base, const ArmView& V) {
     unsigned char* b = (unsigned char*)base;
-    const ArmStaticLayout o = arm_static_layout(V.L, V.na, V.nb, V.S, V.NP, V.nm);
+    const ArmStaticLayout o = arm_static_layout(V.L, V.na, V.nb, V.S, V.NP, V.nm, V.nclu);
     const int t = threadIdx.x, nt = blockDim.x;
     double* dh = (double*)(b + o.o_dh);
     for (int i = t; i < 4 * V.L; i += nt) dh[i] = V.dh[i];
@@ -443,7 +448,6 @@ __device__ inline void arm_stage_scene(void* base, const ArmView& V) {
         sphL[3 * i + 2] = V.sph64[4 * i + 2];
         ((float*)(b + o.o_sphR))[i] = V.sph_r32[i];
         ((int*)(b + o.o_sphLink))[i] = V.sph_link[i] | ((V.sph_link[i] / V.dof) << 8);  // link | arm << 8
-        ((int*)(b + o.o_sphLo))[i] = V.sph_lo[i];
     }
     double* bc = (double*)(b + o.o_linkBC);
     for (int i = t; i < 3 * V.L; i += nt) bc[i] = V.link_bc64[i];
@@ -455,6 +459,11 @@ __device__ inline void arm_stage_scene(void* base, const ArmView& V) {
         float4* box = (float4*)(b + o.o_box);
         const float4* src = (const float4*)V.box32;
         for (int i = t; i < 2 * V.nb; i += nt) box[i] = src[i];
+    }
+    {
+        float4* cl = (float4*)(b + o.o_clu);
+        const float4* src = (const float4*)V.clu32;
+        for (int i = t; i < 2 * V.nclu; i += nt) cl[i] = src[i];
     }
     if (V.nm <= ARM_MODES_LDS_MAX) {
         int* ma = (int*)(b + o.o_mact);
@@ -549,21 +558,14 @@ __device__ inline bool arm_flush_held(const ArmView& V, const ArmLds& W, int mod
     return __any(hit);
 }
 
-// Per-lane registers loaded once per kernel: obstacle boxes lane and lane + 64 (up to 128 boxes
-// in registers) and the lane's link (first sphere, count).
+// Per-lane register loaded once per kernel: the lane's link (first sphere, sphere count).
 struct ArmRegs {
-    float4 lo0, hi0, lo1, hi1;
     int linkFC;
 };
 
 __device__ inline ArmRegs arm_regs(const ArmView& V, const ArmLds& W) {
     ArmRegs R;
     const int lane = wave_lane();
-    const float4 far = make_float4(3.0e38f, 3.0e38f, 3.0e38f, 0.f);
-    R.lo0 = lane < V.nb ? W.box[2 * lane] : far;
-    R.hi0 = lane < V.nb ? W.box[2 * lane + 1] : far;
-    R.lo1 = lane + 64 < V.nb ? W.box[2 * (lane + 64)] : far;
-    R.hi1 = lane + 64 < V.nb ? W.box[2 * (lane + 64) + 1] : far;
     R.linkFC = lane < V.L ? W.linkFC[lane] : 0;
     return R;
 }
@@ -721,7 +723,84 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             const float e2 = fminf(fmaxf(myLink.z, V.bb_lo[2]), V.bb_hi[2]) - myLink.z;
             lsurv = fmaf(e2, e2, fmaf(e1, e1, e0 * e0)) <= myLink.w * myLink.w + Mc;
         }
-        for (unsigned long long lm = __ballot(lsurv); lm; lm &= lm - 1) {
+        const unsigned long long smask = __ballot(lsurv);
+        if (V.nclu > 0 && smask) {
+            // two levels: (surviving link, box cluster) pairs lane-parallel, then the boxes of the
+            // candidate clusters (8 lanes per pair), then the spheres of the candidate (link, box)
+            // pairs (8 lanes per pair)
+            if (lsurv) W.lst[lanes_below(smask)] = lane;  // lane l carries link l
+            wave_sync();
+            const int nsv = __popcll(smask);
+            const int K = V.nclu;
+            const int lgKG = K <= 8 ? 3 : 4;  // lanes per link: 8 or 16
+            for (int k0 = 0; k0 < nsv; k0 += 64 >> lgKG) {
+                const int k = k0 + (lane >> lgKG), cc = lane & ((1 << lgKG) - 1);
+                bool pc = false;
+                int code = 0;
+                if (k < nsv && cc < K) {
+                    const int l = W.lst[k];
+                    const float4 Lc = W.linkC[l];
+                    pc = box_d2(Lc, W.clu[2 * cc], W.clu[2 * cc + 1]) <= Lc.w * Lc.w + Mc;
+                    code = l | (cc << 8);
+                    ++cnt.lb;
+                }
+                const unsigned long long pm = __ballot(pc);
+                if (pm == 0ull) continue;
+                if (pc) W.cand[lanes_below(pm)] = code;
+                wave_sync();
+                const int nA = __popcll(pm);
+                for (int a0 = 0; a0 < nA; a0 += 8) {
+                    const int g = lane >> 3, j = lane & 7;
+                    bool bcand = false;
+                    int lb = 0;
+                    if (a0 + g < nA) {
+                        const int code2 = W.cand[a0 + g];
+                        const int l = code2 & 255, c2 = code2 >> 8;
+                        const int first = __float_as_int(W.clu[2 * c2].w), count = __float_as_int(W.clu[2 * c2 + 1].w);
+                        if (j < count) {
+                            const int b = first + j;
+                            const float4 Lc = W.linkC[l];
+                            bcand = box_d2(Lc, W.box[2 * b], W.box[2 * b + 1]) <= Lc.w * Lc.w + Mc;
+                            lb = l | (b << 8);
+                            ++cnt.lb;
+                        }
+                    }
+                    const unsigned long long bm = __ballot(bcand);
+                    if (bm == 0ull) continue;
+                    if (bcand) W.lbc[lanes_below(bm)] = lb;
+                    wave_sync();
+                    const int nB = __popcll(bm);
+                    for (int c0 = 0; c0 < nB; c0 += 8) {
+                        const int g2 = lane >> 3, j2 = lane & 7;
+                        bool band = false;
+                        int sph = 0, bb = 0;
+                        if (c0 + g2 < nB) {
+                            const int v = W.lbc[c0 + g2];
+                            const int l = v & 255;
+                            const int fc = W.linkFC[l];
+                            if (j2 < (fc >> 16)) {
+                                bb = v >> 8;
+                                sph = (fc & 0xffff) + j2;
+                                const float4 c = W.sph[sph];
+                                const float d2 = box_d2(c, W.box[2 * bb], W.box[2 * bb + 1]);
+                                const float T = c.w * c.w;
+                                ++cnt.sb;
+                                if (d2 <= T + M) {
+                                    if (d2 < T - M) hit = true;
+                                    else band = true;
+                                }
+                            }
+                        }
+                        arm_pend(W, band, 0, sph, bb, np);
+                        if (np > ARM_PEND_MAX - 64 && arm_flush(V, W, np, rechecks)) return true;
+                    }
+                    wave_sync();  // W.lbc is rewritten by the next group
+                    if (__any(hit)) return true;
+                }
+                wave_sync();  // W.cand is rewritten by the next round
+            }
+        }
+        for (unsigned long long lm = V.nclu > 0 ? 0ull : smask; lm; lm &= lm - 1) {
             const int l = __builtin_ctzll(lm);
             const int fc = __builtin_amdgcn_readlane(R.linkFC, l);
             const int f = fc & 0xffff, ns = fc >> 16;
@@ -736,9 +815,8 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                 bool cand = false;
                 if (b < V.nb) {
                     float4 lo, hi;
-                    if (b0 == 0) { lo = R.lo0; hi = R.hi0; }
-                    else if (b0 == 64) { lo = R.lo1; hi = R.hi1; }
-                    else { lo = W.box[2 * b]; hi = W.box[2 * b + 1]; }
+                    lo = W.box[2 * b];
+                    hi = W.box[2 * b + 1];
                     cand = box_d2(Lc, lo, hi) <= Tl + Mc;
                     ++cnt.lb;
                 }
@@ -796,8 +874,8 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             const int i = base + lane;
             const bool act = i < S_act && ((mask >> (W.sphLink[i < S_act ? i : 0] >> 8)) & 1u);
             const float4 ci = W.sph[act ? i : 0];
-            const int lo = act ? W.sphLo[i] : 0x7fffffff;
-            const int jstart = W.sphLo[base];
+            const int lo = act ? V.sph_lo[i] : 0x7fffffff;
+            const int jstart = V.sph_lo[base];
             int nj = 0;
             for (int j = jstart; j < S_act; ++j) {
                 const int aj = W.sphLink[j] >> 8;
