@@ -311,15 +311,15 @@ __device__ inline bool planar_collides(const PlanarView& S, const double* q, uns
 // LDS of a workgroup = one static region (the scene, staged once per kernel by the whole
 // workgroup) + one dynamic region per wave (the state being evaluated).  Waves advance
 // independently: inside the evaluator they synchronise with wave-level barriers only.
-#define ARM_PAIRS_LDS_MAX 1024  // link-pair lists up to this length are staged in LDS
-#define ARM_BOXES_LDS_MAX 256   // obstacle box tables up to this length are staged in LDS
+// LDS holds the obstacle boxes when they form clusters (at most 16 x 8 boxes), the whole link-pair
+// list, and the per-mode arm counts.  Pointers that could point to
+// either LDS or global memory would compile to flat accesses, so each array has one home.
 
 struct ArmStaticLayout {
     int o_dh, o_base, o_sphL, o_linkBC, o_box, o_clu, o_sphR, o_sphLink, o_linkR, o_linkFC, o_mact, o_pairs;
     int bytes;
 };
 
-#define ARM_MODES_LDS_MAX 1024  // per-mode active arm counts up to this many modes are staged in LDS
 
 __host__ __device__ inline int align16(int x) { return (x + 15) & ~15; }
 
@@ -332,15 +332,15 @@ __host__ __device__ inline ArmStaticLayout arm_static_layout(int L, int na, int 
     o.o_sphL = b;    b += 8 * 3 * S;
     o.o_linkBC = b;  b += 8 * 3 * L;
     b = align16(b);
-    o.o_box = b;     b += (nb <= ARM_BOXES_LDS_MAX ? 32 * nb : 0);  // lo xyz pad, hi xyz pad
+    o.o_box = b;     b += (nclu > 0 ? 32 * nb : 0);  // lo xyz pad, hi xyz pad
     o.o_clu = b;     b += 32 * nclu;
     o.o_sphR = b;    b += 4 * S;
     o.o_sphLink = b; b += 4 * S;
     o.o_linkR = b;   b += 4 * L;
     o.o_linkFC = b;  b += 4 * L;
-    o.o_mact = b;    b += (nm <= ARM_MODES_LDS_MAX ? 4 * nm : 0);
+    o.o_mact = b;    b += 4 * nm;
     b = align16(b);
-    o.o_pairs = b;   b += (NP <= ARM_PAIRS_LDS_MAX ? 8 * NP : 0);
+    o.o_pairs = b;   b += 8 * NP;
     o.bytes = align16(b);
     return o;
 }
@@ -356,7 +356,8 @@ __host__ __device__ inline int arm_union_bytes(int D) {
 }
 
 __host__ __device__ inline int arm_wave_bytes(int D, int L, int na, int S, int grasp) {
-    return 96 * L + (grasp ? 96 * na : 0) + arm_union_bytes(D) + 16 * (S + L) + 3 * 4 * 64 + (grasp ? 64 * na : 0);
+    return 96 * L + (grasp ? 96 * na : 0) + arm_union_bytes(D) + 16 * (S + L) + 3 * 4 * 64 + 16 +
+           (grasp ? 64 * na : 0);
 }
 
 // dynamic LDS of an arm kernel launched with `nw` waves per workgroup
@@ -377,20 +378,22 @@ struct ArmLds {
     int* cand;      // [64] compacted candidates of one broad-phase round
     int* lst;       // [64] links that pass the cull against the box around all obstacles
     int* lbc;       // [64] (link, box) candidates of the two-level link-box test
+    unsigned long long* need;  // links whose spheres the current candidates read
     float* G32;
     // per workgroup (the scene)
     const double* dh;      // [L*4] d a cos(alpha) sin(alpha)
     const double* base;    // [na*12]
     const double* sphL;    // [S*3] sphere centres in the link frame
     const double* linkBC;  // [L*3] bounding-sphere centres in the link frame
-    const float4* box;     // [nb*2] lo, hi
+    const float4* box;     // [nb*2] lo, hi (LDS; scenes with box clusters only)
+    const float4* boxG;    // [nb*2] lo, hi (global)
     const float* sphR;     // [S]
     const int* sphLink;    // [S] link | arm << 8
     const float4* clu;     // [nclu*2] box clusters: lo xyz | first box, hi xyz | box count
     const float* linkR;    // [L]
     const int* linkFC;     // [L] first sphere | count << 16
-    const int2* pairs;     // [NP] link pairs with their sphere ranges (LDS or global)
-    const int* mact;       // [nm] active arms of each mode (LDS or global)
+    const int2* pairs;     // [NP] link pairs with their sphere ranges
+    const int* mact;       // [nm] active arms of each mode
 };
 
 // tests executed by one lane (broad phase link-box / sphere-box / link-pair / sphere-pair)
@@ -406,14 +409,15 @@ __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
     W.base = (const double*)(b + o.o_base);
     W.sphL = (const double*)(b + o.o_sphL);
     W.linkBC = (const double*)(b + o.o_linkBC);
-    W.box = V.nb <= ARM_BOXES_LDS_MAX ? (const float4*)(b + o.o_box) : (const float4*)V.box32;
+    W.box = (const float4*)(b + o.o_box);
+    W.boxG = (const float4*)V.box32;
     W.sphR = (const float*)(b + o.o_sphR);
     W.sphLink = (const int*)(b + o.o_sphLink);
     W.clu = (const float4*)(b + o.o_clu);
     W.linkR = (const float*)(b + o.o_linkR);
     W.linkFC = (const int*)(b + o.o_linkFC);
-    W.pairs = V.NP <= ARM_PAIRS_LDS_MAX ? (const int2*)(b + o.o_pairs) : V.lpairs2;
-    W.mact = V.nm <= ARM_MODES_LDS_MAX ? (const int*)(b + o.o_mact) : V.m_active;
+    W.pairs = (const int2*)(b + o.o_pairs);
+    W.mact = (const int*)(b + o.o_mact);
     unsigned char* w = b + o.bytes + (threadIdx.x >> 6) * arm_wave_bytes(V.D, V.L, V.na, V.S, V.grasp);
     W.X = (double*)w;
     w += 96 * V.L;
@@ -428,7 +432,8 @@ __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
     W.cand = (int*)(W.linkC + V.L);
     W.lst = W.cand + 64;
     W.lbc = W.lst + 64;
-    W.G32 = (float*)(W.lbc + 64);
+    W.need = (unsigned long long*)(W.lbc + 64);
+    W.G32 = (float*)(W.need + 2);
     return W;
 }
 
@@ -455,7 +460,7 @@ __device__ inline void arm_stage_scene(void* base, const ArmView& V) {
         ((float*)(b + o.o_linkR))[i] = V.link_R32[i];
         ((int*)(b + o.o_linkFC))[i] = V.link_first[i] | ((V.link_first[i + 1] - V.link_first[i]) << 16);
     }
-    if (V.nb <= ARM_BOXES_LDS_MAX) {
+    if (V.nclu > 0) {
         float4* box = (float4*)(b + o.o_box);
         const float4* src = (const float4*)V.box32;
         for (int i = t; i < 2 * V.nb; i += nt) box[i] = src[i];
@@ -465,11 +470,11 @@ __device__ inline void arm_stage_scene(void* base, const ArmView& V) {
         const float4* src = (const float4*)V.clu32;
         for (int i = t; i < 2 * V.nclu; i += nt) cl[i] = src[i];
     }
-    if (V.nm <= ARM_MODES_LDS_MAX) {
+    {
         int* ma = (int*)(b + o.o_mact);
         for (int i = t; i < V.nm; i += nt) ma[i] = V.m_active[i];
     }
-    if (V.NP <= ARM_PAIRS_LDS_MAX) {
+    {
         int2* pr = (int2*)(b + o.o_pairs);
         for (int i = t; i < V.NP; i += nt) pr[i] = V.lpairs2[i];
     }
@@ -558,6 +563,48 @@ __device__ inline bool arm_flush_held(const ArmView& V, const ArmLds& W, int mod
     return __any(hit);
 }
 
+// active arms of mode md
+__device__ inline int arm_mact(const ArmView& V, const ArmLds& W, int md) { return W.mact[md]; }
+
+// Places (binary64, rounded to binary32) the spheres of the links in `need` that `placed` lacks:
+// 8 links per round, lane group k = lane >> 3 taking the k-th such link and lane j = lane & 7 its
+// sphere j.  The broad phase places only the links its candidates read.
+__device__ inline void arm_place(const ArmLds& W, unsigned long long need, unsigned long long& placed) {
+    unsigned long long todo = need & ~placed;
+    if (todo == 0ull) return;
+    placed |= todo;
+    const int lane = wave_lane(), k = lane >> 3, j = lane & 7;
+    while (todo) {
+        int l = -1;
+        for (int t = 0; t < 8 && todo; ++t) {
+            const int bit = __builtin_ctzll(todo);
+            todo &= todo - 1;
+            if (t == k) l = bit;
+        }
+        if (l >= 0) {
+            const int fc = W.linkFC[l];
+            if (j < (fc >> 16)) {
+                const int s = (fc & 0xffff) + j;
+                double c[3];
+                const double* sl = W.sphL + 3 * s;
+                xform_point_f64(W.X + 12 * l, sl[0], sl[1], sl[2], c);
+                W.sph[s] = make_float4((float)c[0], (float)c[1], (float)c[2], W.sphR[s]);
+            }
+        }
+    }
+    wave_sync();
+}
+
+// The wave-uniform union of the link bits the lanes added to *W.need since the last call (the
+// lanes' LDS atomics precede a wave_sync); clears it.
+__device__ inline unsigned long long arm_take_need(const ArmLds& W) {
+    const unsigned long long v = *W.need;
+    const unsigned long long u = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(v >> 32)) << 32) |
+                                 (unsigned)__builtin_amdgcn_readfirstlane((unsigned)v);
+    if (wave_lane() == 0) *W.need = 0ull;
+    return u;
+}
+
 // Per-lane register loaded once per kernel: the lane's link (first sphere, sphere count).
 struct ArmRegs {
     int linkFC;
@@ -586,7 +633,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                                           int mode, unsigned mask, unsigned& rechecks, ArmCount& cnt) {
     const int lane = wave_lane();
     if (mode < 0 || mode >= V.nm) mode = 0;
-    const int na = W.mact[mode];
+    const int na = arm_mact(V, W, mode);
     const int dof = V.dof;
     const int Dact = na * dof;
     // 1. joint sines / cosines (binary64), finiteness
@@ -639,15 +686,13 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
         }
     }
     wave_sync();
-    // 3. sphere centres and link bounding spheres: binary64, rounded to binary32
+    // 3. link bounding spheres (binary64, rounded to binary32); sphere centres are placed when a
+    //    test first reads them (arm_place)
     const int lastFC = __builtin_amdgcn_readlane(R.linkFC, Dact - 1);
     const int S_act = (lastFC & 0xffff) + (lastFC >> 16);
-    for (int s = lane; s < S_act; s += 64) {
-        double c[3];
-        const double* l = W.sphL + 3 * s;
-        xform_point_f64(W.X + 12 * (W.sphLink[s] & 255), l[0], l[1], l[2], c);
-        W.sph[s] = make_float4((float)c[0], (float)c[1], (float)c[2], W.sphR[s]);
-    }
+    const unsigned long long all_links = Dact >= 64 ? ~0ull : ((1ull << Dact) - 1ull);
+    unsigned long long placed = 0ull;
+    if (lane == 0) *W.need = 0ull;
     float4 myLink = make_float4(0.f, 0.f, 0.f, 0.f);
     if (V.broadphase && lane < Dact) {
         const double* b = W.linkBC + 3 * lane;
@@ -681,12 +726,14 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                     const float Rs = A.w + Bc.w;
                     cand = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) <= Rs * Rs + Mc;
                     ++cnt.lp;
+                    if (cand) atomicOr(W.need, (1ull << l) | (1ull << mm));
                 }
             }
             const unsigned long long m = __ballot(cand);
             if (m == 0ull) continue;
             if (cand) W.cand[lanes_below(m)] = y;  // the pair's sphere ranges
             wave_sync();
+            arm_place(W, arm_take_need(W), placed);
             const int n = __popcll(m);
             for (int c = 0; c < n; ++c) {
                 const int yc = W.cand[c];
@@ -763,12 +810,14 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                             bcand = box_d2(Lc, W.box[2 * b], W.box[2 * b + 1]) <= Lc.w * Lc.w + Mc;
                             lb = l | (b << 8);
                             ++cnt.lb;
+                            if (bcand) atomicOr(W.need, 1ull << l);
                         }
                     }
                     const unsigned long long bm = __ballot(bcand);
                     if (bm == 0ull) continue;
                     if (bcand) W.lbc[lanes_below(bm)] = lb;
                     wave_sync();
+                    arm_place(W, arm_take_need(W), placed);
                     const int nB = __popcll(bm);
                     for (int c0 = 0; c0 < nB; c0 += 8) {
                         const int g2 = lane >> 3, j2 = lane & 7;
@@ -815,8 +864,8 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                 bool cand = false;
                 if (b < V.nb) {
                     float4 lo, hi;
-                    lo = W.box[2 * b];
-                    hi = W.box[2 * b + 1];
+                    lo = W.boxG[2 * b];
+                    hi = W.boxG[2 * b + 1];
                     cand = box_d2(Lc, lo, hi) <= Tl + Mc;
                     ++cnt.lb;
                 }
@@ -824,6 +873,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                 if (m == 0ull) continue;
                 if (cand) W.cand[lanes_below(m)] = b;
                 wave_sync();
+                arm_place(W, 1ull << l, placed);
                 const int n = __popcll(m);
                 for (int c0 = 0; c0 < n; c0 += 8) {
                     const int k = c0 + (lane >> 3), j = lane & 7;
@@ -833,7 +883,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                         bb = W.cand[k];
                         s = f + j;
                         const float4 c = W.sph[s];
-                        const float d2 = box_d2(c, W.box[2 * bb], W.box[2 * bb + 1]);
+                        const float d2 = box_d2(c, W.boxG[2 * bb], W.boxG[2 * bb + 1]);
                         const float T = c.w * c.w;
                         ++cnt.sb;
                         if (d2 <= T + M) {
@@ -849,6 +899,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             }
         }
     } else {
+        arm_place(W, all_links, placed);
         // 4. spheres vs boxes (brute force)
         for (int base = 0; base < S_act; base += 64) {
             const int s = base + lane;
@@ -929,6 +980,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             }
         }
         if (__any(hit)) return true;
+        arm_place(W, all_links, placed);
         // 6b. held box vs spheres of the other arms (in the mask)
         for (int a = 0; a < na; ++a) {
             if (!((gmask >> a) & 1u)) continue;

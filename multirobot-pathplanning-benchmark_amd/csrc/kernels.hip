@@ -245,12 +245,29 @@ __global__ void __launch_bounds__(256) k_plan_scatter(PlanWork W, long long n) {
 // chunks in increasing order, so all counters advance through the passes together; when its
 // counter is exhausted it takes chunks from the others.  Returns the chunk, or -1 when every
 // counter is exhausted.  Must be called by all lanes of the wave (wave-uniform result).
+//
+// Before an atomic, lanes 0..7 read the 8 counters (one round trip, relaxed loads at device
+// scope) and counters already past the end are skipped: counters only grow, so a value read as
+// exhausted is exhausted.  A wave that finds the queue drained leaves after one read instead of
+// one atomic per counter.
 __device__ inline long long claim_chunk(const PlanWork& W, int& q, long long nchunks) {
     const int home = blockIdx.x & 7;
+    const int lane = threadIdx.x & 63;
     while (q < 8) {
+        unsigned long long v = 0;
+        if (lane < 8)
+            v = __hip_atomic_load(W.ticket + 32 * ((home + lane) & 7), __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+        const bool open = lane >= q && lane < 8 && ((long long)((home + lane) & 7) + 8 * (long long)v) < nchunks;
+        const unsigned long long om = __ballot(open);
+        if (om == 0ull) {
+            q = 8;
+            break;
+        }
+        q = __builtin_ctzll(om);
         const int x = (home + q) & 7;
         unsigned long long k = 0;
-        if ((threadIdx.x & 63) == 0) k = atomicAdd(W.ticket + 32 * x, 1ull);
+        if (lane == 0) k = atomicAdd(W.ticket + 32 * x, 1ull);
         k = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(k >> 32)) << 32) |
             (unsigned)__builtin_amdgcn_readfirstlane((unsigned)k);
         const long long c = (long long)x + 8 * (long long)k;
@@ -277,29 +294,6 @@ __device__ inline SlotRef slot_ref(const PlanWork& W, int& p, long long s, int l
     const long long r = off >> lgG;
     const int k = (int)(off & ((1 << lgG) - 1));
     if (off < 0 || r >= n) return ref;  // unreachable for a consistent queue
-    ref.e = W.order[r];
-    if (ref.e < 0 || ref.e >= n) {
-        ref.e = -1;
-        return ref;
-    }
-    const int S = W.eS[ref.e];
-    ref.local = k * S + p;
-    if (ref.local >= W.eM[ref.e]) ref.e = -1;
-    return ref;
-}
-
-// slot_ref with the wave's current pass [lo, hi) = [pass_start[p], pass_start[p + 1]) held in
-// scalar registers: a slot inside it needs no pass_start load.
-__device__ inline SlotRef slot_ref_hint(const PlanWork& W, int& p, long long lo, long long hi, long long s,
-                                        int lgG, long long n) {
-    if (s < lo || s >= hi) return slot_ref(W, p, s, lgG, n);
-    SlotRef ref;
-    ref.e = -1;
-    ref.local = 0;
-    const long long off = s - lo;
-    const long long r = off >> lgG;
-    const int k = (int)(off & ((1 << lgG) - 1));
-    if (r >= n) return ref;  // unreachable for a consistent queue
     ref.e = W.order[r];
     if (ref.e < 0 || ref.e >= n) {
         ref.e = -1;
@@ -536,24 +530,49 @@ __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView
         const long long s0 = c * spt;
         const long long rem = T - s0;
         const int n_here = (int)(rem < spt ? rem : spt);  // >= 1: c < nchunks
-        // lane-parallel resolution of the chunk's slots
+        // lane-parallel resolution of the chunk's slots: slot -> (pass, rank, k) by arithmetic
+        // (the current pass's bounds are in scalar registers), rank -> edge (one load), then every
+        // per-edge word in one round of loads
         int e = -1, i = -1, N = 0, mode = 0, Dact = 0;
         int pl = p;
         bool dead = false;
         if (lane < n_here) {
-            const SlotRef ref = slot_ref_hint(W, pl, ps_lo, ps_hi, s0 + lane, lgG, B.n);
-            if (ref.e >= 0) {
-                if (B.out[ref.e] == 0) {
-                    dead = true;
-                } else {
-                    const EdgeWin w = win_of(W, ref.e, B.incl);
-                    i = edge_state_index(w, ref.local, B.incl);
-                    if (i >= 0) {
-                        e = ref.e;
-                        N = w.N;
-                        mode = B.modes ? B.modes[e] : 0;
-                        const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
-                        Dact = Wl.mact[md] * V.dof;
+            const long long sl = s0 + lane;
+            long long lo = ps_lo;
+            if (sl < ps_lo || sl >= ps_hi) {  // another pass: walk from the hint
+                if (W.pass_start[pl] > sl) pl = 0;
+                while (pl < MRCC_MAX_PASSES && W.pass_start[pl + 1] <= sl) ++pl;
+                lo = W.pass_start[pl];
+            }
+            const long long off = sl - lo;
+            const long long r = off >> lgG;
+            const int k = (int)(off & ((1 << lgG) - 1));
+            const int ee = (off >= 0 && r < B.n) ? W.order[r] : -1;
+            if (ee >= 0 && ee < B.n) {
+                const int S = W.eS[ee], Mv = W.eM[ee], Nv = W.eN[ee], w0 = W.ew0[ee];
+                const int full = W.efull[ee];
+                const unsigned char o = B.out[ee];
+                const int md0 = B.modes ? B.modes[ee] : 0;
+                const int local = k * S + pl;
+                if (local < Mv) {
+                    if (o == 0) {
+                        dead = true;
+                    } else {
+                        EdgeWin w;
+                        w.N = Nv;
+                        w.w0 = w0;
+                        w.M = Mv;
+                        w.full = full;
+                        w.first = B.incl ? 0 : 1;
+                        w.err = 0;
+                        i = edge_state_index(w, local, B.incl);
+                        if (i >= 0) {
+                            e = ee;
+                            N = Nv;
+                            mode = md0;
+                            const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
+                            Dact = arm_mact(V, Wl, md) * V.dof;
+                        }
                     }
                 }
             }
@@ -655,7 +674,7 @@ __global__ void __launch_bounds__(256) k_arm_direct(ArmView V, EdgeBatchView B, 
             if (!init_out && B.out[e] == 0) break;
             const int i = edge_state_index(w, local, B.incl);
             if (i < 0) continue;
-            const double qv = arm_load_state(B, e, N, i, Wl.mact[md] * V.dof);
+            const double qv = arm_load_state(B, e, N, i, arm_mact(V, Wl, md) * V.dof);
             ++n_eval;
             if (arm_state_collides<GRASP>(V, Wl, R, qv, mode, B.mask, rechecks, cnt)) {
                 hit = true;
@@ -687,7 +706,7 @@ __global__ void __launch_bounds__(256, 3) k_arm_configs(ArmView V, const double*
          r += (long long)gridDim.x * nw) {
         const int mode = modes ? modes[r] : 0;
         const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
-        const int Dact = Wl.mact[md] * V.dof;
+        const int Dact = arm_mact(V, Wl, md) * V.dof;
         const double q = lane < Dact ? qv[r * rs + (long long)lane * ds] : 0.0;
         const bool c = arm_state_collides<GRASP>(V, Wl, R, q, mode, mask, rechecks, cnt);
         if (lane == 0) out[r] = c ? 0 : 1;
