@@ -647,8 +647,7 @@ int mrcc_scene_set_modes(mrcc_scene* scene, int n_modes, const int32_t* n_active
     if (!scene) return fail(MRCC_ERR_INVALID, "scene is null");
     SceneBox* box = (SceneBox*)scene;
     if (box->rec.kind != KIND_ARMS) return fail(MRCC_ERR_UNSUPPORTED, "modes need an arm scene");
-    if (n_modes < 1 || n_modes > MRCC_MAX_MODES)
-        return fail(MRCC_ERR_INVALID, "n_modes %d not in 1..%d", n_modes, MRCC_MAX_MODES);
+    if (n_modes < 1) return fail(MRCC_ERR_INVALID, "n_modes < 1");
     ArmHost& H = box->arm_host;
     const size_t nma = (size_t)n_modes * H.na;
     for (int m = 0; m < n_modes; ++m)

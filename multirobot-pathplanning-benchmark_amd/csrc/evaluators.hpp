@@ -312,7 +312,7 @@ __device__ inline bool planar_collides(const PlanarView& S, const double* q, uns
 // workgroup) + one dynamic region per wave (the state being evaluated).  Waves advance
 // independently: inside the evaluator they synchronise with wave-level barriers only.
 // LDS holds the obstacle boxes when they form clusters (at most 16 x 8 boxes), the whole link-pair
-// list, and the per-mode arm counts.  Pointers that could point to
+// list, and the per-mode arm counts of up to MRCC_MODES_LDS_MAX modes.  Pointers that could point to
 // either LDS or global memory would compile to flat accesses, so each array has one home.
 
 struct ArmStaticLayout {
@@ -338,7 +338,7 @@ __host__ __device__ inline ArmStaticLayout arm_static_layout(int L, int na, int 
     o.o_sphLink = b; b += 4 * S;
     o.o_linkR = b;   b += 4 * L;
     o.o_linkFC = b;  b += 4 * L;
-    o.o_mact = b;    b += 4 * nm;
+    o.o_mact = b;    b += (nm <= MRCC_MODES_LDS_MAX ? 4 * nm : 0);
     b = align16(b);
     o.o_pairs = b;   b += 8 * NP;
     o.bytes = align16(b);
@@ -393,7 +393,7 @@ struct ArmLds {
     const float* linkR;    // [L]
     const int* linkFC;     // [L] first sphere | count << 16
     const int2* pairs;     // [NP] link pairs with their sphere ranges
-    const int* mact;       // [nm] active arms of each mode
+    const int* mact;       // [nm] active arms of each mode (nm <= MRCC_MODES_LDS_MAX)
 };
 
 // tests executed by one lane (broad phase link-box / sphere-box / link-pair / sphere-pair)
@@ -470,7 +470,7 @@ __device__ inline void arm_stage_scene(void* base, const ArmView& V) {
         const float4* src = (const float4*)V.clu32;
         for (int i = t; i < 2 * V.nclu; i += nt) cl[i] = src[i];
     }
-    {
+    if (V.nm <= MRCC_MODES_LDS_MAX) {
         int* ma = (int*)(b + o.o_mact);
         for (int i = t; i < V.nm; i += nt) ma[i] = V.m_active[i];
     }
@@ -563,8 +563,13 @@ __device__ inline bool arm_flush_held(const ArmView& V, const ArmLds& W, int mod
     return __any(hit);
 }
 
-// active arms of mode md
-__device__ inline int arm_mact(const ArmView& V, const ArmLds& W, int md) { return W.mact[md]; }
+// active arms of mode md.  The LDS read goes through an LDS-qualified pointer so that the two
+// loads cannot be merged into one flat load through a selected pointer.
+typedef __attribute__((address_space(3))) const int lds_cint;
+__device__ inline int arm_mact(const ArmView& V, const ArmLds& W, int md) {
+    if (V.nm <= MRCC_MODES_LDS_MAX) return ((lds_cint*)W.mact)[md];
+    return V.m_active[md];
+}
 
 // Places (binary64, rounded to binary32) the spheres of the links in `need` that `placed` lacks:
 // 8 links per round, lane group k = lane >> 3 taking the k-th such link and lane j = lane & 7 its

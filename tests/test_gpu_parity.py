@@ -423,13 +423,13 @@ def test_c5_modes(B, bp):
     q2 = np.clip(Q + rng.normal(0, 0.2, Q.shape), desc.limits[0], desc.limits[1])
     refe = o.check_edges(Q[:3000], q2[:3000], 0.05, modes=modes[:3000])
     assert np.array_equal(gpu_edges(sc, Q[:3000], q2[:3000], 0.05, modes=modes[:3000]), refe)
-    # more than 1024 modes is rejected, and the scene keeps its modes
-    na = desc.n_arms
-    big = {"n_active": np.full(1025, na, dtype=np.int32), "has_grasp": np.zeros((1025, na), dtype=np.uint8),
-           "grasp_rel": np.zeros((1025, na, 12)), "grasp_half": np.zeros((1025, na, 3))}
-    with pytest.raises(pkg.MrccError):
-        sc.set_modes(big)
-    assert np.array_equal(gpu_configs(sc, Q[:2000], modes=modes[:2000]), ref[:2000])
+    # more modes than the LDS table holds: the per-mode arm counts are read from global memory
+    desc2 = scenes.mixed_arms(rng=np.random.default_rng(19), n_modes=3000)
+    sc2, o2 = B.Scene(desc2), O.Scene(desc2)
+    sc2.set_broadphase(bp)
+    m2 = rng.integers(0, 3000, 3000).astype(np.int32)
+    Q2 = near_home(rng, desc2, 3000, sigma=0.5)
+    assert np.array_equal(gpu_configs(sc2, Q2, modes=m2), o2.check_configs(Q2, modes=m2))
 
 
 # ---------------------------------------------------------------------------------------------
