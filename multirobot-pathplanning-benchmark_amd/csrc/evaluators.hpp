@@ -356,8 +356,7 @@ __host__ __device__ inline int arm_union_bytes(int D) {
 }
 
 __host__ __device__ inline int arm_wave_bytes(int D, int L, int na, int S, int grasp) {
-    return 96 * L + (grasp ? 96 * na : 0) + arm_union_bytes(D) + 16 * (S + L) + 3 * 4 * 64 + 16 +
-           (grasp ? 64 * na : 0);
+    return 96 * L + (grasp ? 96 * na : 0) + arm_union_bytes(D) + 16 * (S + L) + 3 * 4 * 64 + (grasp ? 64 * na : 0);
 }
 
 // dynamic LDS of an arm kernel launched with `nw` waves per workgroup
@@ -378,7 +377,6 @@ struct ArmLds {
     int* cand;      // [64] compacted candidates of one broad-phase round
     int* lst;       // [64] links that pass the cull against the box around all obstacles
     int* lbc;       // [64] (link, box) candidates of the two-level link-box test
-    unsigned long long* need;  // links whose spheres the current candidates read
     float* G32;
     // per workgroup (the scene)
     const double* dh;      // [L*4] d a cos(alpha) sin(alpha)
@@ -432,8 +430,7 @@ __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
     W.cand = (int*)(W.linkC + V.L);
     W.lst = W.cand + 64;
     W.lbc = W.lst + 64;
-    W.need = (unsigned long long*)(W.lbc + 64);
-    W.G32 = (float*)(W.need + 2);
+    W.G32 = (float*)(W.lbc + 64);
     return W;
 }
 
@@ -571,45 +568,6 @@ __device__ inline int arm_mact(const ArmView& V, const ArmLds& W, int md) {
     return V.m_active[md];
 }
 
-// Places (binary64, rounded to binary32) the spheres of the links in `need` that `placed` lacks:
-// 8 links per round, lane group k = lane >> 3 taking the k-th such link and lane j = lane & 7 its
-// sphere j.  The broad phase places only the links its candidates read.
-__device__ inline void arm_place(const ArmLds& W, unsigned long long need, unsigned long long& placed) {
-    unsigned long long todo = need & ~placed;
-    if (todo == 0ull) return;
-    placed |= todo;
-    const int lane = wave_lane(), k = lane >> 3, j = lane & 7;
-    while (todo) {
-        int l = -1;
-        for (int t = 0; t < 8 && todo; ++t) {
-            const int bit = __builtin_ctzll(todo);
-            todo &= todo - 1;
-            if (t == k) l = bit;
-        }
-        if (l >= 0) {
-            const int fc = W.linkFC[l];
-            if (j < (fc >> 16)) {
-                const int s = (fc & 0xffff) + j;
-                double c[3];
-                const double* sl = W.sphL + 3 * s;
-                xform_point_f64(W.X + 12 * l, sl[0], sl[1], sl[2], c);
-                W.sph[s] = make_float4((float)c[0], (float)c[1], (float)c[2], W.sphR[s]);
-            }
-        }
-    }
-    wave_sync();
-}
-
-// The wave-uniform union of the link bits the lanes added to *W.need since the last call (the
-// lanes' LDS atomics precede a wave_sync); clears it.
-__device__ inline unsigned long long arm_take_need(const ArmLds& W) {
-    const unsigned long long v = *W.need;
-    const unsigned long long u = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(v >> 32)) << 32) |
-                                 (unsigned)__builtin_amdgcn_readfirstlane((unsigned)v);
-    if (wave_lane() == 0) *W.need = 0ull;
-    return u;
-}
-
 // Per-lane register loaded once per kernel: the lane's link (first sphere, sphere count).
 struct ArmRegs {
     int linkFC;
@@ -691,13 +649,15 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
         }
     }
     wave_sync();
-    // 3. link bounding spheres (binary64, rounded to binary32); sphere centres are placed when a
-    //    test first reads them (arm_place)
+    // 3. sphere centres and link bounding spheres: binary64, rounded to binary32
     const int lastFC = __builtin_amdgcn_readlane(R.linkFC, Dact - 1);
     const int S_act = (lastFC & 0xffff) + (lastFC >> 16);
-    const unsigned long long all_links = Dact >= 64 ? ~0ull : ((1ull << Dact) - 1ull);
-    unsigned long long placed = 0ull;
-    if (lane == 0) *W.need = 0ull;
+    for (int s = lane; s < S_act; s += 64) {
+        double c[3];
+        const double* l = W.sphL + 3 * s;
+        xform_point_f64(W.X + 12 * (W.sphLink[s] & 255), l[0], l[1], l[2], c);
+        W.sph[s] = make_float4((float)c[0], (float)c[1], (float)c[2], W.sphR[s]);
+    }
     float4 myLink = make_float4(0.f, 0.f, 0.f, 0.f);
     if (V.broadphase && lane < Dact) {
         const double* b = W.linkBC + 3 * lane;
@@ -731,14 +691,12 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                     const float Rs = A.w + Bc.w;
                     cand = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) <= Rs * Rs + Mc;
                     ++cnt.lp;
-                    if (cand) atomicOr(W.need, (1ull << l) | (1ull << mm));
                 }
             }
             const unsigned long long m = __ballot(cand);
             if (m == 0ull) continue;
             if (cand) W.cand[lanes_below(m)] = y;  // the pair's sphere ranges
             wave_sync();
-            arm_place(W, arm_take_need(W), placed);
             const int n = __popcll(m);
             for (int c = 0; c < n; ++c) {
                 const int yc = W.cand[c];
@@ -815,14 +773,12 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                             bcand = box_d2(Lc, W.box[2 * b], W.box[2 * b + 1]) <= Lc.w * Lc.w + Mc;
                             lb = l | (b << 8);
                             ++cnt.lb;
-                            if (bcand) atomicOr(W.need, 1ull << l);
                         }
                     }
                     const unsigned long long bm = __ballot(bcand);
                     if (bm == 0ull) continue;
                     if (bcand) W.lbc[lanes_below(bm)] = lb;
                     wave_sync();
-                    arm_place(W, arm_take_need(W), placed);
                     const int nB = __popcll(bm);
                     for (int c0 = 0; c0 < nB; c0 += 8) {
                         const int g2 = lane >> 3, j2 = lane & 7;
@@ -878,7 +834,6 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                 if (m == 0ull) continue;
                 if (cand) W.cand[lanes_below(m)] = b;
                 wave_sync();
-                arm_place(W, 1ull << l, placed);
                 const int n = __popcll(m);
                 for (int c0 = 0; c0 < n; c0 += 8) {
                     const int k = c0 + (lane >> 3), j = lane & 7;
@@ -904,7 +859,6 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             }
         }
     } else {
-        arm_place(W, all_links, placed);
         // 4. spheres vs boxes (brute force)
         for (int base = 0; base < S_act; base += 64) {
             const int s = base + lane;
@@ -985,7 +939,6 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             }
         }
         if (__any(hit)) return true;
-        arm_place(W, all_links, placed);
         // 6b. held box vs spheres of the other arms (in the mask)
         for (int a = 0; a < na; ++a) {
             if (!((gmask >> a) & 1u)) continue;

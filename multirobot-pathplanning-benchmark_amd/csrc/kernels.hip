@@ -245,29 +245,12 @@ __global__ void __launch_bounds__(256) k_plan_scatter(PlanWork W, long long n) {
 // chunks in increasing order, so all counters advance through the passes together; when its
 // counter is exhausted it takes chunks from the others.  Returns the chunk, or -1 when every
 // counter is exhausted.  Must be called by all lanes of the wave (wave-uniform result).
-//
-// Before an atomic, lanes 0..7 read the 8 counters (one round trip, relaxed loads at device
-// scope) and counters already past the end are skipped: counters only grow, so a value read as
-// exhausted is exhausted.  A wave that finds the queue drained leaves after one read instead of
-// one atomic per counter.
 __device__ inline long long claim_chunk(const PlanWork& W, int& q, long long nchunks) {
     const int home = blockIdx.x & 7;
-    const int lane = threadIdx.x & 63;
     while (q < 8) {
-        unsigned long long v = 0;
-        if (lane < 8)
-            v = __hip_atomic_load(W.ticket + 32 * ((home + lane) & 7), __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT);
-        const bool open = lane >= q && lane < 8 && ((long long)((home + lane) & 7) + 8 * (long long)v) < nchunks;
-        const unsigned long long om = __ballot(open);
-        if (om == 0ull) {
-            q = 8;
-            break;
-        }
-        q = __builtin_ctzll(om);
         const int x = (home + q) & 7;
         unsigned long long k = 0;
-        if (lane == 0) k = atomicAdd(W.ticket + 32 * x, 1ull);
+        if ((threadIdx.x & 63) == 0) k = atomicAdd(W.ticket + 32 * x, 1ull);
         k = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(k >> 32)) << 32) |
             (unsigned)__builtin_amdgcn_readfirstlane((unsigned)k);
         const long long c = (long long)x + 8 * (long long)k;
