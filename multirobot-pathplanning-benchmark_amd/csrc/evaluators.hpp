@@ -311,9 +311,12 @@ __device__ inline bool planar_collides(const PlanarView& S, const double* q, uns
 // LDS of a workgroup = one static region (the scene, staged once per kernel by the whole
 // workgroup) + one dynamic region per wave (the state being evaluated).  Waves advance
 // independently: inside the evaluator they synchronise with wave-level barriers only.
-// LDS holds the obstacle boxes when they form clusters (at most 16 x 8 boxes), the whole link-pair
-// list, and the per-mode arm counts of up to MRCC_MODES_LDS_MAX modes.  Pointers that could point to
+// LDS holds the obstacle boxes when they form clusters (at most 16 x 8 boxes), the link-pair list
+// of up to ARM_PAIRS_LDS_MAX pairs, and the per-mode arm counts of up to MRCC_MODES_LDS_MAX modes.
+// Larger tables stay in global memory, read through separately typed pointers.  Pointers that could point to
 // either LDS or global memory would compile to flat accesses, so each array has one home.
+
+#define ARM_PAIRS_LDS_MAX 1024  // link-pair lists up to this length are staged in LDS
 
 struct ArmStaticLayout {
     int o_dh, o_base, o_sphL, o_linkBC, o_box, o_clu, o_sphR, o_sphLink, o_linkR, o_linkFC, o_mact, o_pairs;
@@ -340,7 +343,7 @@ __host__ __device__ inline ArmStaticLayout arm_static_layout(int L, int na, int 
     o.o_linkFC = b;  b += 4 * L;
     o.o_mact = b;    b += (nm <= MRCC_MODES_LDS_MAX ? 4 * nm : 0);
     b = align16(b);
-    o.o_pairs = b;   b += 8 * NP;
+    o.o_pairs = b;   b += (NP <= ARM_PAIRS_LDS_MAX ? 8 * NP : 0);
     o.bytes = align16(b);
     return o;
 }
@@ -390,7 +393,7 @@ struct ArmLds {
     const float4* clu;     // [nclu*2] box clusters: lo xyz | first box, hi xyz | box count
     const float* linkR;    // [L]
     const int* linkFC;     // [L] first sphere | count << 16
-    const int2* pairs;     // [NP] link pairs with their sphere ranges
+    const int2* pairs;     // [NP] link pairs with their sphere ranges (NP <= ARM_PAIRS_LDS_MAX)
     const int* mact;       // [nm] active arms of each mode (nm <= MRCC_MODES_LDS_MAX)
 };
 
@@ -471,7 +474,7 @@ __device__ inline void arm_stage_scene(void* base, const ArmView& V) {
         int* ma = (int*)(b + o.o_mact);
         for (int i = t; i < V.nm; i += nt) ma[i] = V.m_active[i];
     }
-    {
+    if (V.NP <= ARM_PAIRS_LDS_MAX) {
         int2* pr = (int2*)(b + o.o_pairs);
         for (int i = t; i < V.NP; i += nt) pr[i] = V.lpairs2[i];
     }
@@ -563,6 +566,16 @@ __device__ inline bool arm_flush_held(const ArmView& V, const ArmLds& W, int mod
 // active arms of mode md.  The LDS read goes through an LDS-qualified pointer so that the two
 // loads cannot be merged into one flat load through a selected pointer.
 typedef __attribute__((address_space(3))) const int lds_cint;
+typedef __attribute__((address_space(3))) const unsigned long long lds_cu64;
+
+// link pair pi (LDS table for up to ARM_PAIRS_LDS_MAX pairs)
+__device__ inline int2 arm_pair(const ArmView& V, const ArmLds& W, int pi) {
+    if (V.NP <= ARM_PAIRS_LDS_MAX) {
+        const unsigned long long v = ((lds_cu64*)W.pairs)[pi];
+        return make_int2((int)(unsigned)v, (int)(unsigned)(v >> 32));
+    }
+    return V.lpairs2[pi];
+}
 __device__ inline int arm_mact(const ArmView& V, const ArmLds& W, int md) {
     if (V.nm <= MRCC_MODES_LDS_MAX) return ((lds_cint*)W.mact)[md];
     return V.m_active[md];
@@ -682,7 +695,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             bool cand = false;
             int y = 0;
             if (pi < NP) {
-                const int2 pr = W.pairs[pi];
+                const int2 pr = arm_pair(V, W, pi);
                 const int l = (pr.x >> 8) & 255, mm = pr.x & 255;
                 y = pr.y;
                 if (all_arms || (((mask >> ((pr.x >> 20) & 15)) & 1u) && ((mask >> ((pr.x >> 16) & 15)) & 1u))) {
