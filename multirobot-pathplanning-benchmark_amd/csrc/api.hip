@@ -745,7 +745,7 @@ int mrcc_edge_plan_create(const mrcc_scene* scene, const mrcc_edge_batch* batch,
     size_t o_eN = sz(4 * n), o_eM = sz(4 * n), o_eS = sz(4 * n), o_w0 = sz(4 * n), o_full = sz(n),
            o_order = sz(4 * n), o_hist = sz(4 * (MRCC_MAX_PASSES + 1)), o_cur = sz(4 * (MRCC_MAX_PASSES + 1)),
            o_ps = sz(8 * (MRCC_MAX_PASSES + 1)), o_meta = sz(8 * 8), o_long = sz(4 * n), o_tick = sz(8 * 32 * 8),
-           o_stats = sz(8 * 8), o_dir = arms ? 0 : sz(8 * (size_t)n * D);
+           o_stats = sz(8 * 8), o_dir = sz(8 * (size_t)n * D);
     if (hipMalloc(&p->work_mem, P.buf.size()) != hipSuccess) {
         delete p;
         return fail(MRCC_ERR_NOMEM, "workspace of %zu bytes", P.buf.size());
@@ -764,7 +764,7 @@ int mrcc_edge_plan_create(const mrcc_scene* scene, const mrcc_edge_batch* batch,
     p->W.long_list = (int*)(w + o_long);
     p->W.ticket = (unsigned long long*)(w + o_tick);
     p->W.stats = t.collect_stats ? (unsigned long long*)(w + o_stats) : nullptr;
-    p->W.dir = arms ? nullptr : (double*)(w + o_dir);
+    p->W.dir = (double*)(w + o_dir);
     if (t.collect_stats) HIP_OK(hipMemset(w + o_stats, 0, 64));
     HIP_OK(hipMemset(w + o_meta, 0, 64));
     if (t.time_kernels && !t.use_graph)

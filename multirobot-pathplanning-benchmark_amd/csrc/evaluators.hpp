@@ -482,10 +482,12 @@ __device__ inline void arm_stage_scene(void* base, const ArmView& V) {
 }
 
 // LDS written by some lanes of a wave is read by other lanes of the same wave after this point
+// (LDS-only fences: outstanding global loads, such as the next state's prefetched inputs, are
+// not waited for)
 __device__ inline void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 __device__ inline int wave_lane() { return threadIdx.x & 63; }

@@ -570,23 +570,50 @@ __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView
         }
         n_slots += n_here;
         n_dead += __popcll(__ballot(dead));
-        // request the next chunk's ticket now; read it after this chunk's evaluations
+        // The live slots are evaluated in turn; the inputs of the next one (its edge's row and
+        // step, binary64) are loaded before the current one is evaluated, and the next chunk's
+        // ticket is requested after the first slot's loads, so none of these round trips sits on
+        // the evaluation path.
+        unsigned long long live = __ballot(e >= 0);
+        int en = -1, in_ = 0, mn = 0;
+        double an = 0.0, dn = 0.0;
+        if (live) {
+            const int j = __builtin_ctzll(live);
+            en = __builtin_amdgcn_readlane(e, j);
+            in_ = __builtin_amdgcn_readlane(i, j);
+            mn = __builtin_amdgcn_readlane(mode, j);
+            const int Dn = __builtin_amdgcn_readlane(Dact, j);
+            if (lane < Dn) {
+                an = row1(B, en)[lane * B.ds];
+                dn = W.dir[(long long)en * V.D + lane];
+            }
+        }
         const int x = (home + q) & 7;
         unsigned long long tk = 0;
         if (lane == 0) tk = atomicAdd(W.ticket + 32 * x, 1ull);
         int last_dead = -1;  // edge found in collision by this wave in this chunk
-        for (unsigned long long live = __ballot(e >= 0); live; live &= live - 1) {
-            const int j = __builtin_ctzll(live);
-            const int ej = __builtin_amdgcn_readlane(e, j);
+        while (live) {
+            const int ej = en, ij = in_, mj = mn;
+            const double aj = an, dj = dn;
+            live &= live - 1;
+            if (live) {
+                const int j = __builtin_ctzll(live);
+                en = __builtin_amdgcn_readlane(e, j);
+                in_ = __builtin_amdgcn_readlane(i, j);
+                mn = __builtin_amdgcn_readlane(mode, j);
+                const int Dn = __builtin_amdgcn_readlane(Dact, j);
+                an = 0.0;
+                dn = 0.0;
+                if (lane < Dn) {
+                    an = row1(B, en)[lane * B.ds];
+                    dn = W.dir[(long long)en * V.D + lane];
+                }
+            }
             if (ej == last_dead) {  // a later state of an edge this chunk already rejected
                 ++n_dead;
                 continue;
             }
-            const int ij = __builtin_amdgcn_readlane(i, j);
-            const int Nj = __builtin_amdgcn_readlane(N, j);
-            const int mj = __builtin_amdgcn_readlane(mode, j);
-            const int Dj = __builtin_amdgcn_readlane(Dact, j);
-            const double qv = arm_load_state(B, ej, Nj, ij, Dj);
+            const double qv = interp_dir_f64(aj, dj, ij);  // lanes >= D: unused
             ++n_eval;
             if (arm_state_collides<GRASP>(V, Wl, R, qv, mj, B.mask, rechecks, cnt)) {
                 if (lane == 0) B.out[ej] = 0;
@@ -732,7 +759,7 @@ static int grid_for(long long n, int block) {
 
 void launch_plan_build(const SceneRec& sc, const EdgeBatchView& B, const PlanWork& W, int lgG,
                        hipStream_t st) {
-    const int want_dir = sc.kind != KIND_ARMS;
+    const int want_dir = 1;
     k_plan_init<<<64, 256, 0, st>>>(W);
     // PREP_LPE lanes per edge and a few edges per group, so each block's LDS histogram covers
     // many edges
