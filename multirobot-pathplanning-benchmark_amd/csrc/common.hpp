@@ -418,7 +418,7 @@ struct PlanWork {
     int* hist;         // [MAX_PASSES+1]
     int* cursor;       // [MAX_PASSES+1]
     long long* pass_start;  // [MAX_PASSES+1] slot offset of each pass (G * sum E)
-    long long* meta;   // [8]: 0 total slots, 1 n_pass, 2 error count, 3 long edges, 4 tickets
+    long long* meta;   // [8]: 0 total slots, 1 n_pass, 2 error count, 3 long edges, 5 largest pass count
     int* long_list;    // [n] edges with more passes than MAX_PASSES
     unsigned long long* ticket;  // [8 * 32]: one chunk counter per XCD, 256 bytes apart
     unsigned long long* stats;  // [8]

@@ -78,9 +78,12 @@ __global__ void k_plan_init(PlanWork W) {
 __global__ void __launch_bounds__(256) k_plan_prep(EdgeBatchView B, PlanWork W, int D, int lgG,
                                                    int want_dir) {
     __shared__ int h[PLAN_LDS_BINS];
+    __shared__ int smax;
     for (int i = threadIdx.x; i < PLAN_LDS_BINS; i += blockDim.x) h[i] = 0;
+    if (threadIdx.x == 0) smax = 0;
     __syncthreads();
     const int sub = threadIdx.x & (PREP_LPE - 1);
+    int mymax = 0;  // largest pass count seen by this thread
     const long long per_grid = (long long)gridDim.x * (blockDim.x / PREP_LPE);
     // the lanes of a group share e, so they run the same iterations and their shuffles stay within
     // active lanes
@@ -137,73 +140,82 @@ __global__ void __launch_bounds__(256) k_plan_prep(EdgeBatchView B, PlanWork W, 
         B.out[e] = w.err ? 0 : 1;
         if (S < PLAN_LDS_BINS) atomicAdd(&h[S], 1);
         else atomicAdd(&W.hist[S], 1);
+        mymax = S > mymax ? (int)S : mymax;
     }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {  // wave maximum, then one LDS atomic per wave
+        const int u = __shfl_xor(mymax, o, 64);
+        mymax = u > mymax ? u : mymax;
+    }
+    if ((threadIdx.x & 63) == 0 && mymax > 0) atomicMax(&smax, mymax);
     __syncthreads();
     for (int i = threadIdx.x; i < PLAN_LDS_BINS; i += blockDim.x)
         if (h[i]) atomicAdd(&W.hist[i], h[i]);
+    if (threadIdx.x == 0 && smax > 0) atomicMax((unsigned long long*)&W.meta[5], (unsigned long long)smax);
 }
 
-// one block of 1024 threads: suffix sums of the pass histogram and prefix sums of E_p
+// Inclusive prefix sum over the threads of a 1024-thread block: shuffles within each wave, then
+// the 16 wave totals through LDS (two barriers).
+__device__ inline long long block_prefix_1024(long long v, long long* wsum) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const long long u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    if (lane == 63) wsum[wid] = v;
+    __syncthreads();
+    long long off = 0;
+    for (int w = 0; w < wid; ++w) off += wsum[w];
+    __syncthreads();  // wsum is reused by the next call
+    return v + off;
+}
+
+// One block of 1024 threads: E_p = number of edges with more than p passes (suffix sums of the
+// pass histogram) and the slot offset of each pass (prefix sums of E_p).  Only passes below the
+// largest pass count (written by the prep kernel) are scanned; slot lookups never read further.
 __global__ void __launch_bounds__(1024) k_plan_scan(PlanWork W, int lgG) {
-    constexpr int NB = MRCC_MAX_PASSES;         // passes 0..NB-1; hist has NB+1 buckets
-    constexpr int PER = NB / 1024;
-    __shared__ long long part[1024];
+    constexpr int PER_MAX = MRCC_MAX_PASSES / 1024;
+    __shared__ long long wsum[16];
     const int t = threadIdx.x;
-    // E_p = sum_{b > p} hist[b]; thread t owns p in [t*PER, (t+1)*PER)
-    long long loc[PER];
+    const int maxS = (int)W.meta[5];             // largest pass count, <= MRCC_MAX_PASSES
+    const int nb = maxS + 1;                     // passes 0..maxS (E_maxS = 0)
+    const int per = (nb + 1023) / 1024;          // <= PER_MAX + 1
+    // thread t owns passes [t*per, (t+1)*per) of [0, nb), in reverse order for the suffix sums
+    long long loc[PER_MAX + 1];
     long long s = 0;
-    for (int i = PER - 1; i >= 0; --i) {
-        const int p = t * PER + i;
-        s += W.hist[p + 1];
-        loc[i] = s;  // partial: sum_{b in (p, (t+1)*PER]}
+    for (int i = per - 1; i >= 0; --i) {
+        const int p = t * per + i;
+        s += (p < maxS) ? (long long)W.hist[p + 1] : 0;
+        loc[i] = s;  // sum_{b in (p, (t+1)*per]}
     }
-    part[t] = s;
+    // suffix sums over threads: the total minus the prefix sum
+    const long long rs = block_prefix_1024(s, wsum);  // sum of s over threads <= t
+    __shared__ long long total_s;
+    if (t == 1023) total_s = rs;
     __syncthreads();
-    // inclusive suffix scan of part
-    for (int off = 1; off < 1024; off <<= 1) {
-        long long v = (t + off < 1024) ? part[t + off] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    const long long after = (t + 1 < 1024) ? part[t + 1] : 0;
-    long long E[PER];
+    const long long after = total_s - rs;             // sum of s over threads > t
     long long esum = 0;
-    for (int i = 0; i < PER; ++i) {
-        E[i] = loc[i] + after;
-        esum += E[i];
-        W.cursor[t * PER + i] = (int)E[i];  // descending-order start of bucket p
+    for (int i = 0; i < per; ++i) {
+        const int p = t * per + i;
+        if (p >= nb) break;
+        const long long E = loc[i] + after;
+        loc[i] = E;
+        esum += E;
+        W.cursor[p] = (int)E;  // descending-order start of bucket p
     }
-    __syncthreads();
-    // exclusive prefix scan of E over p -> PS
-    part[t] = esum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        long long v = (t >= off) ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    // exclusive prefix of E over p
+    long long run = block_prefix_1024(esum, wsum) - esum;
+    for (int i = 0; i < per; ++i) {
+        const int p = t * per + i;
+        if (p >= nb) break;
+        W.pass_start[p] = run << lgG;
+        run += loc[i];
     }
-    long long run = part[t] - esum;
-    int npass_local = 0;
-    for (int i = 0; i < PER; ++i) {
-        W.pass_start[t * PER + i] = run << lgG;
-        run += E[i];
-        if (E[i] > 0) npass_local = t * PER + i + 1;
-    }
+    // pass_start[maxS] (written above, E_maxS = 0) is the slot count; lookups never read beyond
     if (t == 1023) {
-        W.pass_start[NB] = run << lgG;
-        W.cursor[NB] = 0;
-    }
-    // n_pass = max over threads
-    __shared__ int npass;
-    if (t == 0) npass = 0;
-    __syncthreads();
-    atomicMax(&npass, npass_local);
-    __syncthreads();
-    if (t == 0) {
-        W.meta[1] = npass;
-        W.meta[0] = W.pass_start[npass];
+        W.meta[1] = maxS;
+        W.meta[0] = run << lgG;
     }
 }
 

@@ -122,6 +122,28 @@ def test_c1_windows(c1, incl):
     assert np.array_equal(gpu_edges(sc, q1, q2, 0.01, N=N, N_start=S, N_max=M, incl=incl), ref)
 
 
+def test_c1_pass_count_limits(c1, B):
+    _say("C1 edges at the queue's pass-count limit (16384 passes of 8 states) and beyond it")
+    desc, sc, o, _ = c1
+    rng = np.random.default_rng(20)
+    Q = scenes.uniform_configs(rng, desc, 4000)
+    F = Q[o.check_configs(Q)]
+    # short free segments walked with huge explicit N: the states stay close to free endpoints
+    q1 = F[:12]
+    q2 = q1 + rng.uniform(-1e-3, 1e-3, q1.shape)
+    N = np.array([131074, 131082, 200000, 131073, 5, 2, 3, 131074, 70000, 150001, 131075, 64],
+                 dtype=np.int32)  # 131074: exactly 16384 passes of 8 states; 131082: 16385
+    ref = o.check_edges(q1, q2, 0.01, N=N)
+    for tune in (B.make_tune(slots_per_pass=8, use_graph=0), B.make_tune(slots_per_pass=8, use_graph=1)):
+        assert np.array_equal(gpu_edges(sc, q1, q2, 0.01, N=N, tune=tune), ref)
+    # and colliding walks through an obstacle, found from a long edge's slots
+    q3 = np.repeat(desc.rect_center[:1], 3, axis=0)
+    q3 = np.concatenate([q3, q3], axis=1)[:, :4]
+    q4 = q3 + 0.001
+    N3 = np.array([131074, 200000, 40], dtype=np.int32)
+    assert np.array_equal(gpu_edges(sc, q3, q4, 0.01, N=N3), o.check_edges(q3, q4, 0.01, N=N3))
+
+
 def test_c1_near_contact(c1):
     _say("C1 near-contact states (FP64 recheck path)")
     desc, sc, o, _ = c1
