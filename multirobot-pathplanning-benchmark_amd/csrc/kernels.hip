@@ -70,24 +70,24 @@ __global__ void k_plan_init(PlanWork W) {
 // addresses serialises in L2.
 #define PLAN_LDS_BINS 1024
 
-// PREP_LPE lanes share an edge: lane `sub` reads coordinates sub, sub + PREP_LPE, ... of both
-// rows (PREP_LPE consecutive doubles per row and step instead of one), the maximum is reduced
-// across the group, and lane sub == 0 writes the edge's queue entries.
-#define PREP_LPE 4
-
+// A group of `lpe` lanes (the power of two >= D, at most 64) shares an edge: lane `sub` reads
+// coordinates sub, sub + lpe, ... of both rows, so a group reads its rows as contiguous
+// segments; the maximum is reduced across the group, and lane sub == 0 writes the edge's queue
+// entries.
 __global__ void __launch_bounds__(256) k_plan_prep(EdgeBatchView B, PlanWork W, int D, int lgG,
-                                                   int want_dir) {
+                                                   int want_dir, int lg_lpe) {
     __shared__ int h[PLAN_LDS_BINS];
     __shared__ int smax;
     for (int i = threadIdx.x; i < PLAN_LDS_BINS; i += blockDim.x) h[i] = 0;
     if (threadIdx.x == 0) smax = 0;
     __syncthreads();
-    const int sub = threadIdx.x & (PREP_LPE - 1);
+    const int lpe = 1 << lg_lpe;
+    const int sub = threadIdx.x & (lpe - 1);
     int mymax = 0;  // largest pass count seen by this thread
-    const long long per_grid = (long long)gridDim.x * (blockDim.x / PREP_LPE);
+    const long long per_grid = (long long)gridDim.x * (blockDim.x >> lg_lpe);
     // the lanes of a group share e, so they run the same iterations and their shuffles stay within
     // active lanes
-    for (long long e = (long long)blockIdx.x * (blockDim.x / PREP_LPE) + threadIdx.x / PREP_LPE; e < B.n;
+    for (long long e = (long long)blockIdx.x * (blockDim.x >> lg_lpe) + (threadIdx.x >> lg_lpe); e < B.n;
          e += per_grid) {
         int N;
         if (B.N) {
@@ -99,23 +99,21 @@ __global__ void __launch_bounds__(256) k_plan_prep(EdgeBatchView B, PlanWork W, 
             const double* a = row1(B, e);
             const double* b = row2(B, e);
             double m = sub == 0 ? __builtin_fabs(a[0] - b[0]) : 0.0;
-#pragma unroll 4
-            for (int k = sub == 0 ? PREP_LPE : sub; k < D; k += PREP_LPE) {
+            for (int k = sub == 0 ? lpe : sub; k < D; k += lpe) {
                 const double v = __builtin_fabs(a[k * B.ds] - b[k * B.ds]);
                 m = v > m ? v : m;
             }
-#pragma unroll
-            for (int o = 1; o < PREP_LPE; o <<= 1) {
-                const double v = __shfl_xor(m, o, PREP_LPE);
+            for (int o = 1; o < lpe; o <<= 1) {
+                const double v = __shfl_xor(m, o, lpe);
                 m = v > m ? v : m;
             }
-            m = __shfl(m, 0, PREP_LPE);
+            m = __shfl(m, 0, lpe);
             N = edge_N_from_max(m, B.res);
         }
         if (want_dir && W.dir) {
             const double* a = row1(B, e);
             const double* b = row2(B, e);
-            for (int k = sub; k < D; k += PREP_LPE) {
+            for (int k = sub; k < D; k += lpe) {
 #pragma clang fp contract(off)
                 W.dir[e * D + k] = (b[k * B.ds] - a[k * B.ds]) / (double)(N - 1);
             }
@@ -773,13 +771,15 @@ void launch_plan_build(const SceneRec& sc, const EdgeBatchView& B, const PlanWor
                        hipStream_t st) {
     const int want_dir = 1;
     k_plan_init<<<64, 256, 0, st>>>(W);
-    // PREP_LPE lanes per edge and a few edges per group, so each block's LDS histogram covers
-    // many edges
-    int gp = grid_for(B.n, 4 * (256 / PREP_LPE));
+    // lanes per edge: the power of two >= D (at most 64); a few edges per group, so each block's
+    // LDS histogram covers many edges
+    int lg_lpe = 0;
+    while ((1 << lg_lpe) < sc.D && lg_lpe < 6) ++lg_lpe;
+    int gp = grid_for(B.n, 4 * (256 >> lg_lpe));
     if (gp > 1024) gp = 1024;
     int gs = grid_for(B.n, 256 * SCATTER_EPT);
     if (gs > 1024) gs = 1024;
-    k_plan_prep<<<gp, 256, 0, st>>>(B, W, sc.D, lgG, want_dir);
+    k_plan_prep<<<gp, 256, 0, st>>>(B, W, sc.D, lgG, want_dir, lg_lpe);
     k_plan_scan<<<1, 1024, 0, st>>>(W, lgG);
     k_plan_scatter<<<gs, 256, 0, st>>>(W, B.n);
 }
