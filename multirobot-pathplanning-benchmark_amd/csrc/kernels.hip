@@ -70,10 +70,9 @@ __global__ void k_plan_init(PlanWork W) {
 // addresses serialises in L2.
 #define PLAN_LDS_BINS 1024
 
-// A group of `lpe` lanes (the power of two >= D, at most 64) shares an edge: lane `sub` reads
-// coordinates sub, sub + lpe, ... of both rows, so a group reads its rows as contiguous
-// segments; the maximum is reduced across the group, and lane sub == 0 writes the edge's queue
-// entries.
+// A group of `lpe` lanes (a power of two) shares an edge: lane `sub` reads coordinates sub,
+// sub + lpe, ... of both rows, the maximum is reduced across the group, and lane sub == 0 writes
+// the edge's queue entries.
 __global__ void __launch_bounds__(256) k_plan_prep(EdgeBatchView B, PlanWork W, int D, int lgG,
                                                    int want_dir, int lg_lpe) {
     __shared__ int h[PLAN_LDS_BINS];
@@ -771,10 +770,10 @@ void launch_plan_build(const SceneRec& sc, const EdgeBatchView& B, const PlanWor
                        hipStream_t st) {
     const int want_dir = 1;
     k_plan_init<<<64, 256, 0, st>>>(W);
-    // lanes per edge: the power of two >= D (at most 64); a few edges per group, so each block's
-    // LDS histogram covers many edges
-    int lg_lpe = 0;
-    while ((1 << lg_lpe) < sc.D && lg_lpe < 6) ++lg_lpe;
+    // 4 lanes per edge (fewer for D < 4): measured faster than one group of >= D lanes, whose
+    // per-edge tail runs on one lane of a wide group; a few edges per group, so each block's LDS
+    // histogram covers many edges
+    const int lg_lpe = sc.D >= 4 ? 2 : (sc.D >= 2 ? 1 : 0);
     int gp = grid_for(B.n, 4 * (256 >> lg_lpe));
     if (gp > 1024) gp = 1024;
     int gs = grid_for(B.n, 256 * SCATTER_EPT);
