@@ -350,16 +350,16 @@ __host__ __device__ inline ArmStaticLayout arm_static_layout(int L, int na, int 
 
 #define ARM_PEND_MAX 64  // band pairs awaiting their binary64 recheck, per wave
 
-// Wave region: X [12 L] doubles, held boxes G64 [12 na] doubles (grasp scenes only), a union of
-// the joint cos / sin [2 D] doubles (kinematics phase) and the band-pair queue (narrow phase),
-// sphere table [S] and link spheres [L] float4, candidates [64] int, held boxes G32 [16 na] float
-// (grasp scenes only).  Every part is a multiple of 16 bytes.
-__host__ __device__ inline int arm_union_bytes(int D) {
-    return 16 * D > 8 * ARM_PEND_MAX ? 16 * D : 8 * ARM_PEND_MAX;
-}
+// Wave region: X [12 L] doubles, held boxes G64 [12 na] doubles (grasp scenes only), the
+// band-pair queue, one area that holds the joints' kinematics coefficients [8 D] doubles during
+// the kinematics and the sphere table [S] float4 afterwards, link spheres [L] float4, three
+// candidate lists [64] int, held boxes G32 [16 na] float (grasp scenes only).  Every part is a
+// multiple of 16 bytes.
+__host__ __device__ inline int arm_sph_bytes(int D, int S) { return 16 * S > 64 * D ? 16 * S : 64 * D; }
 
 __host__ __device__ inline int arm_wave_bytes(int D, int L, int na, int S, int grasp) {
-    return 96 * L + (grasp ? 96 * na : 0) + arm_union_bytes(D) + 16 * (S + L) + 3 * 4 * 64 + (grasp ? 64 * na : 0);
+    return 96 * L + (grasp ? 96 * na : 0) + 8 * ARM_PEND_MAX + arm_sph_bytes(D, S) + 16 * L + 3 * 4 * 64 +
+           (grasp ? 64 * na : 0);
 }
 
 // dynamic LDS of an arm kernel launched with `nw` waves per workgroup
@@ -370,8 +370,8 @@ __host__ __device__ inline size_t arm_lds_bytes(int D, int L, int na, int nb, in
 
 struct ArmLds {
     // per wave
-    double* cs;     // [D] joint cosines (kinematics phase; shares memory with pend)
-    double* sn;     // [D]
+    double* fkc;    // [D*8] per joint: cos, sin, and the products of the DH step (kinematics
+                    // phase; shares memory with sph)
     double* X;
     double* G64;
     float4* sph;
@@ -424,12 +424,12 @@ __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
     w += 96 * V.L;
     W.G64 = (double*)w;
     if (V.grasp) w += 96 * V.na;
-    W.cs = (double*)w;
-    W.sn = W.cs + V.D;
     W.pend = (int2*)w;
-    w += arm_union_bytes(V.D);
+    w += 8 * ARM_PEND_MAX;
     W.sph = (float4*)w;
-    W.linkC = W.sph + V.S;
+    W.fkc = (double*)w;
+    w += arm_sph_bytes(V.D, V.S);
+    W.linkC = (float4*)w;
     W.cand = (int*)(W.linkC + V.L);
     W.lst = W.cand + 64;
     W.lbc = W.lst + 64;
@@ -491,6 +491,32 @@ __device__ inline void wave_sync() {
 }
 
 __device__ inline int wave_lane() { return threadIdx.x & 63; }
+
+// value of lane (lane & ~3) + j of the lane's quad (DPP quad_perm broadcast; all 4 lanes active)
+__device__ inline double quad_bcast_f64(double v, int j) {
+    const unsigned long long u = __double_as_longlong(v);
+    const int lo = (int)(unsigned)u, hi = (int)(unsigned)(u >> 32);
+    int rlo, rhi;
+    switch (j) {
+        case 0:
+            rlo = __builtin_amdgcn_mov_dpp(lo, 0x00, 0xf, 0xf, false);
+            rhi = __builtin_amdgcn_mov_dpp(hi, 0x00, 0xf, 0xf, false);
+            break;
+        case 1:
+            rlo = __builtin_amdgcn_mov_dpp(lo, 0x55, 0xf, 0xf, false);
+            rhi = __builtin_amdgcn_mov_dpp(hi, 0x55, 0xf, 0xf, false);
+            break;
+        case 2:
+            rlo = __builtin_amdgcn_mov_dpp(lo, 0xaa, 0xf, 0xf, false);
+            rhi = __builtin_amdgcn_mov_dpp(hi, 0xaa, 0xf, 0xf, false);
+            break;
+        default:
+            rlo = __builtin_amdgcn_mov_dpp(lo, 0xff, 0xf, 0xf, false);
+            rhi = __builtin_amdgcn_mov_dpp(hi, 0xff, 0xf, 0xf, false);
+            break;
+    }
+    return __longlong_as_double(((long long)(unsigned)rhi << 32) | (unsigned)rlo);
+}
 
 __device__ inline unsigned lanes_below(unsigned long long m) {
     return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -614,53 +640,73 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
     const int na = arm_mact(V, W, mode);
     const int dof = V.dof;
     const int Dact = na * dof;
-    // 1. joint sines / cosines (binary64), finiteness
+    // 1. joint sines / cosines (binary64), finiteness; lane l also forms joint l's coefficients
+    //    of the DH step (dh_row_f64's products): [cs, sn, A01, A11, A02, A12, A03, A13]
     bool bad = false;
     if (lane < Dact) {
         bad = !__builtin_isfinite(qv);
-        SinCos t = sincos_f64(qv);
-        W.cs[lane] = t.c;
-        W.sn[lane] = t.s;
+        const SinCos t = sincos_f64(qv);
+        const double* p = W.dh + 4 * lane;  // d a cos(alpha) sin(alpha)
+        const double cs = t.c, sn = t.s, a = p[1], ca = p[2], sa = p[3];
+        double2* f = (double2*)(W.fkc + 8 * lane);
+        f[0] = make_double2(cs, sn);
+        f[1] = make_double2(-(sn * ca), cs * ca);
+        f[2] = make_double2(sn * sa, -(cs * sa));
+        f[3] = make_double2(a * cs, a * sn);
     }
     if (__any(bad)) return true;  // non-finite coordinates are reported as collisions
     wave_sync();
-    // 2. forward kinematics (binary64): lane (arm a, row r) carries row r of the arm's transform
-    //    through the chain (each entry with the operations of dh_step_f64); grasped boxes
-    if (lane < 3 * na) {
-        const int a = lane / 3, r = lane - 3 * a;
-        double R0 = W.base[12 * a + 3 * r], R1 = W.base[12 * a + 3 * r + 1], R2 = W.base[12 * a + 3 * r + 2];
-        double t = W.base[12 * a + 9 + r];
-        // joint k's parameters are loaded one step ahead, off the chain's critical path
-        const int l0 = a * dof;
-        double p0 = W.dh[4 * l0], p1 = W.dh[4 * l0 + 1], p2 = W.dh[4 * l0 + 2], p3 = W.dh[4 * l0 + 3];
-        double cn = W.cs[l0], sn = W.sn[l0];
-        for (int k = 0; k < dof; ++k) {
-            const int l = l0 + k;
-            const double pc[4] = {p0, p1, p2, p3};
-            const double c0 = cn, s0 = sn;
-            if (k + 1 < dof) {
-                p0 = W.dh[4 * l + 4]; p1 = W.dh[4 * l + 5]; p2 = W.dh[4 * l + 6]; p3 = W.dh[4 * l + 7];
-                cn = W.cs[l + 1];
-                sn = W.sn[l + 1];
+    // 2. forward kinematics (binary64).  Lane (arm a, row r, column c) computes entry c of row r
+    //    of the arm's transform at every joint, with the operations of dh_row_f64:
+    //      c < 3: (R0 * k0 + R1 * k1) + R2 * k2,   c = 3: ((R0 * k0 + R1 * k1) + R2 * k2) + t
+    //    with (k0, k1) the joint's coefficient pair c and k2 = 0, sin(alpha), cos(alpha), d; the
+    //    quad then shares the row's four new entries (DPP broadcasts within the quad).
+    //    Arms are processed 5 at a time (60 lanes).  Grasped boxes are composed from the last row.
+    {
+        const int c = lane & 3, rr = (lane >> 2) % 3, ag = lane / 12;
+        const int k2i = c == 1 ? 3 : (c == 2 ? 2 : 0);  // sa, ca, d in the DH table
+        for (int a0 = 0; a0 < na; a0 += 5) {
+            const int a = a0 + ag;
+            if (ag < 5 && a < na) {
+                double R0 = W.base[12 * a + 3 * rr], R1 = W.base[12 * a + 3 * rr + 1],
+                       R2 = W.base[12 * a + 3 * rr + 2], t = W.base[12 * a + 9 + rr];
+                const int l0 = a * dof;
+                // joint k's coefficients are loaded one step ahead, off the chain's critical path
+                double2 kk = ((const double2*)(W.fkc + 8 * l0))[c];
+                double k2 = c == 0 ? 0.0 : W.dh[4 * l0 + k2i];
+                for (int k = 0; k < dof; ++k) {
+                    const int l = l0 + k;
+                    const double k0 = kk.x, k1 = kk.y, kz = k2;
+                    if (k + 1 < dof) {
+                        kk = ((const double2*)(W.fkc + 8 * (l + 1)))[c];
+                        k2 = c == 0 ? 0.0 : W.dh[4 * (l + 1) + k2i];
+                    }
+                    double y;
+                    {
+#pragma clang fp contract(off)
+                        y = (R0 * k0 + R1 * k1) + R2 * kz;
+                        const double yt = y + t;
+                        y = c == 3 ? yt : y;
+                    }
+                    W.X[12 * l + (c < 3 ? 3 * rr + c : 9 + rr)] = y;
+                    R0 = quad_bcast_f64(y, 0);
+                    R1 = quad_bcast_f64(y, 1);
+                    R2 = quad_bcast_f64(y, 2);
+                    t = quad_bcast_f64(y, 3);
+                }
+                if (GRASP && c == 0 && V.m_grasp[mode * V.na + a]) {
+                    const double* Rel = V.m_rel + 12 * (mode * V.na + a);
+                    double g[4];
+                    compose_row_f64(R0, R1, R2, t, Rel, g);
+                    for (int cc = 0; cc < 3; ++cc) {
+                        W.G64[12 * a + 3 * rr + cc] = g[cc];
+                        W.G32[16 * a + 3 * rr + cc] = (float)g[cc];
+                    }
+                    W.G64[12 * a + 9 + rr] = g[3];
+                    W.G32[16 * a + 9 + rr] = (float)g[3];
+                    W.G32[16 * a + 12 + rr] = (float)V.m_half[3 * (mode * V.na + a) + rr];
+                }
             }
-            dh_row_f64(R0, R1, R2, t, pc, c0, s0);
-            double* X = W.X + 12 * l;
-            X[3 * r] = R0;
-            X[3 * r + 1] = R1;
-            X[3 * r + 2] = R2;
-            X[9 + r] = t;
-        }
-        if (GRASP && V.m_grasp[mode * V.na + a]) {
-            const double* Rel = V.m_rel + 12 * (mode * V.na + a);
-            double g[4];
-            compose_row_f64(R0, R1, R2, t, Rel, g);
-            for (int c = 0; c < 3; ++c) {
-                W.G64[12 * a + 3 * r + c] = g[c];
-                W.G32[16 * a + 3 * r + c] = (float)g[c];
-            }
-            W.G64[12 * a + 9 + r] = g[3];
-            W.G32[16 * a + 9 + r] = (float)g[3];
-            W.G32[16 * a + 12 + r] = (float)V.m_half[3 * (mode * V.na + a) + r];
         }
     }
     wave_sync();
