@@ -255,26 +255,32 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
         if ((double)link_R[l] < R) link_R[l] = nextafterf(link_R[l], INFINITY);
         Rl_max = std::max(Rl_max, (double)link_R[l]);
     }
-    std::vector<int> lpairs, lpairs2, lpair_count(na + 1, 0);
-    for (int bm = 0; bm < na; ++bm)       // pairs ordered by the arm of their second link
-        for (int l = 0; l < L; ++l)
-            for (int m = bm * dof; m < (bm + 1) * dof; ++m) {
-                if (m <= l) continue;
-                const int al = l / dof;
-                if (al == bm && m - l < 2) continue;
-                if (link_first[l] == link_first[l + 1] || link_first[m] == link_first[m + 1]) continue;
-                lpairs.push_back((l << 8) | m);
-                lpairs2.push_back(m | (l << 8) | (bm << 16) | (al << 20));
-                lpairs2.push_back(link_first[l] | (H.counts[l] << 9) | (link_first[m] << 13) | (H.counts[m] << 22));
-            }
-    for (int na_ = 1; na_ <= na; ++na_) {
-        int c = 0;
-        for (int v : lpairs)
-            if ((v & 255) / dof < na_) ++c;
-        lpair_count[na_] = c;
+    // pair list: first the self pairs of every arm (arm by arm), then one segment per arm pair
+    // (a, b), a < b, in the order b = 1..na-1, a = 0..b-1, so that the pairs of the first n arms
+    // form a prefix of each part.  lpair_sets[n] = self pairs of arms < n (n = 0..na); entry
+    // na + 1 + k describes segment k = b (b - 1) / 2 + a: first pair | pairs << 12 | a << 20 | b << 24.
+    std::vector<int> lpairs2, lpair_sets(na + 1 + na * (na - 1) / 2, 0);
+    auto add_pair = [&](int l, int m) {
+        if (link_first[l] == link_first[l + 1] || link_first[m] == link_first[m + 1]) return false;
+        lpairs2.push_back(m | (l << 8) | ((m / dof) << 16) | ((l / dof) << 20));
+        lpairs2.push_back(link_first[l] | (H.counts[l] << 9) | (link_first[m] << 13) | (H.counts[m] << 22));
+        return true;
+    };
+    int n_pairs = 0;
+    for (int a = 0; a < na; ++a) {
+        for (int l = a * dof; l < (a + 1) * dof; ++l)
+            for (int m = l + 2; m < (a + 1) * dof; ++m) n_pairs += add_pair(l, m);
+        lpair_sets[a + 1] = n_pairs;
     }
+    for (int b = 1; b < na; ++b)
+        for (int a = 0; a < b; ++a) {
+            const int first = n_pairs;
+            for (int l = a * dof; l < (a + 1) * dof; ++l)
+                for (int m = b * dof; m < (b + 1) * dof; ++m) n_pairs += add_pair(l, m);
+            lpair_sets[na + 1 + b * (b - 1) / 2 + a] = first | ((n_pairs - first) << 12) | (a << 20) | (b << 24);
+        }
     Packer P;
-    size_t o_lbc = P.add(link_bc), o_lR = P.add(link_R), o_lp = P.add(lpairs2), o_lpc = P.add(lpair_count);
+    size_t o_lbc = P.add(link_bc), o_lR = P.add(link_R), o_lp = P.add(lpairs2), o_lpc = P.add(lpair_sets);
     size_t o_base = P.add(H.base), o_dh = P.add(dh4), o_lf = P.add(link_first), o_sl = P.add(sph_link),
            o_slo = P.add(sph_lo), o_s64 = P.add(H.sph), o_sr = P.add(sph_r32), o_b32 = P.add(box32),
            o_b64 = P.add(box64), o_bch64 = P.add(bx), o_bch32 = P.add(boxch32), o_clu = P.add(clu32),
@@ -326,9 +332,16 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     V.clu32 = at<float>(d, o_clu);
     V.link_bc64 = at<double>(d, o_lbc);
     V.link_R32 = at<float>(d, o_lR);
-    V.NP = (int)lpairs.size();
+    V.NP = n_pairs;
     V.lpairs2 = at<int2>(d, o_lp);
-    V.lpair_count = at<int>(d, o_lpc);
+    V.lpair_sets = at<int>(d, o_lpc);
+    // arm boxes separated by more than Marm on some axis cannot hold a link pair that passes the
+    // link test: 2 sqrt(Mcull) covers the test's margin, 2^-20 (1 + B + R) the rounding of the boxes
+    {
+        const double m = 2.0 * sqrt((double)V.Mcull) + ldexp(1.0, -20) * (1.0 + B + Rl_max);
+        V.Marm = (float)m;
+        if ((double)V.Marm < m) V.Marm = nextafterf(V.Marm, INFINITY);
+    }
     sc->D = V.D;
     sc->n_robots = na;
     V.grasp = 0;

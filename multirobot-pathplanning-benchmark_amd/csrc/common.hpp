@@ -382,8 +382,11 @@ struct ArmView {
     int NP;                  // link pairs in the pair list
     const int2* lpairs2;     // [NP] (m | l << 8 | arm(m) << 16 | arm(l) << 20,
                              //  fl | nl << 9 | fm << 13 | nm << 22): link pair l < m and the sphere
-                             // ranges [fl, fl+nl), [fm, fm+nm); sorted by arm of m
-    const int* lpair_count;  // [na+1] pairs whose arms are all < n_active
+                             // ranges [fl, fl+nl), [fm, fm+nm); the self pairs of every
+                             // arm, then one segment per arm pair (a, b), a < b
+    const int* lpair_sets;   // [na+1 + na(na-1)/2]: [n] self pairs of arms < n; [na+1+k]
+                             // segment k = b(b-1)/2 + a: first | count << 12 | a << 20 | b << 24
+    float Marm;              // binary32 margin of the arm-box separation test
     float bb_lo[3], bb_hi[3];   // box containing every obstacle box (binary32, rounded outward)
     int nclu;                // obstacle box clusters (0: the link-box test has one level)
     const float* clu32;      // [nclu*8] cluster bounds: lo xyz, first box (int bits), hi xyz, count

@@ -319,7 +319,7 @@ __device__ inline bool planar_collides(const PlanarView& S, const double* q, uns
 #define ARM_PAIRS_LDS_MAX 1024  // link-pair lists up to this length are staged in LDS
 
 struct ArmStaticLayout {
-    int o_dh, o_base, o_sphL, o_linkBC, o_box, o_clu, o_sphR, o_sphLink, o_linkR, o_linkFC, o_mact, o_pairs;
+    int o_dh, o_base, o_sphL, o_linkBC, o_box, o_clu, o_sphR, o_sphLink, o_linkR, o_linkFC, o_mact, o_sets, o_pairs;
     int bytes;
 };
 
@@ -342,6 +342,7 @@ __host__ __device__ inline ArmStaticLayout arm_static_layout(int L, int na, int 
     o.o_linkR = b;   b += 4 * L;
     o.o_linkFC = b;  b += 4 * L;
     o.o_mact = b;    b += (nm <= MRCC_MODES_LDS_MAX ? 4 * nm : 0);
+    o.o_sets = b;    b += 4 * (na + 1 + na * (na - 1) / 2);
     b = align16(b);
     o.o_pairs = b;   b += (NP <= ARM_PAIRS_LDS_MAX ? 8 * NP : 0);
     o.bytes = align16(b);
@@ -380,6 +381,7 @@ struct ArmLds {
     int* cand;      // [64] compacted candidates of one broad-phase round
     int* lst;       // [64] links that pass the cull against the box around all obstacles
     int* lbc;       // [64] (link, box) candidates of the two-level link-box test
+    float4* abox;   // [2 na] box (lo, hi) around each arm's link spheres (shares memory with lst)
     float* G32;
     // per workgroup (the scene)
     const double* dh;      // [L*4] d a cos(alpha) sin(alpha)
@@ -395,6 +397,7 @@ struct ArmLds {
     const int* linkFC;     // [L] first sphere | count << 16
     const int2* pairs;     // [NP] link pairs with their sphere ranges (NP <= ARM_PAIRS_LDS_MAX)
     const int* mact;       // [nm] active arms of each mode (nm <= MRCC_MODES_LDS_MAX)
+    const int* sets;       // self-pair counts and arm-pair segments of the pair list
 };
 
 // tests executed by one lane (broad phase link-box / sphere-box / link-pair / sphere-pair)
@@ -419,6 +422,7 @@ __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
     W.linkFC = (const int*)(b + o.o_linkFC);
     W.pairs = (const int2*)(b + o.o_pairs);
     W.mact = (const int*)(b + o.o_mact);
+    W.sets = (const int*)(b + o.o_sets);
     unsigned char* w = b + o.bytes + (threadIdx.x >> 6) * arm_wave_bytes(V.D, V.L, V.na, V.S, V.grasp);
     W.X = (double*)w;
     w += 96 * V.L;
@@ -433,6 +437,7 @@ __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
     W.cand = (int*)(W.linkC + V.L);
     W.lst = W.cand + 64;
     W.lbc = W.lst + 64;
+    W.abox = (float4*)W.lst;  // 2 MRCC_MAX_ARMS float4 = 64 ints
     W.G32 = (float*)(W.lbc + 64);
     return W;
 }
@@ -473,6 +478,10 @@ __device__ inline void arm_stage_scene(void* base, const ArmView& V) {
     if (V.nm <= MRCC_MODES_LDS_MAX) {
         int* ma = (int*)(b + o.o_mact);
         for (int i = t; i < V.nm; i += nt) ma[i] = V.m_active[i];
+    }
+    {
+        int* st = (int*)(b + o.o_sets);
+        for (int i = t; i < V.na + 1 + V.na * (V.na - 1) / 2; i += nt) st[i] = V.lpair_sets[i];
     }
     if (V.NP <= ARM_PAIRS_LDS_MAX) {
         int2* pr = (int2*)(b + o.o_pairs);
@@ -736,13 +745,61 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
     if (V.broadphase) {
         const float Mc = V.Mcull;
         // 4'. link pairs of the pair set, then the sphere pairs of candidate link pairs (8 x 8);
-        //     first, since self and inter-arm contacts dominate collisions of random states
-        const int NP = V.lpair_count[na];
-        for (int p0 = 0; p0 < NP; p0 += 64) {
+        //     first, since self and inter-arm contacts dominate collisions of random states.
+        //     The self pairs of the active arms are walked first (most collisions of random states
+        //     are self collisions); then the segment of an arm pair, only when the boxes around
+        //     the two arms' link spheres are not separated by Marm.
+        int p0 = 0, p_end = W.sets[na];
+        int sg = 0;
+        bool segs_known = false;
+        unsigned long long segs = 0ull;
+        for (;;) {
+            if (p0 >= p_end) {
+                if (!segs_known) {
+                    segs_known = true;
+                    if (lane < na) {
+                        float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+                        float4 hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+#pragma unroll
+                        for (int k = 0; k < MRCC_MAX_DOF; ++k) {
+                            const int l = lane * dof + k;
+                            if (k < dof && (W.linkFC[l] >> 16) > 0) {
+                                const float4 c = W.linkC[l];
+                                lo.x = fminf(lo.x, c.x - c.w); lo.y = fminf(lo.y, c.y - c.w); lo.z = fminf(lo.z, c.z - c.w);
+                                hi.x = fmaxf(hi.x, c.x + c.w); hi.y = fmaxf(hi.y, c.y + c.w); hi.z = fmaxf(hi.z, c.z + c.w);
+                            }
+                        }
+                        W.abox[2 * lane] = lo;
+                        W.abox[2 * lane + 1] = hi;
+                    }
+                    wave_sync();
+                    bool keep = false;
+                    if (lane < (na * (na - 1)) >> 1) {
+                        sg = W.sets[V.na + 1 + lane];
+                        const int a = (sg >> 20) & 15, b = (sg >> 24) & 15;
+                        if (((sg >> 12) & 255) > 0 && (all_arms || (((mask >> a) & 1u) && ((mask >> b) & 1u)))) {
+                            const float4 la = W.abox[2 * a], ha = W.abox[2 * a + 1];
+                            const float4 lb = W.abox[2 * b], hb = W.abox[2 * b + 1];
+                            const float d = V.Marm;
+                            keep = !(lb.x - ha.x > d || la.x - hb.x > d || lb.y - ha.y > d || la.y - hb.y > d ||
+                                     lb.z - ha.z > d || la.z - hb.z > d);
+                            ++cnt.lp;
+                        }
+                    }
+                    segs = __ballot(keep);
+                }
+                if (segs == 0ull) break;
+                const int k = __builtin_ctzll(segs);
+                segs &= segs - 1ull;
+                const int sk = __builtin_amdgcn_readlane(sg, k);
+                p0 = sk & 4095;
+                p_end = p0 + ((sk >> 12) & 255);
+            }
             const int pi = p0 + lane;
+            p0 += 64;
             bool cand = false;
             int y = 0;
-            if (pi < NP) {
+            if (pi < p_end) {
                 const int2 pr = arm_pair(V, W, pi);
                 const int l = (pr.x >> 8) & 255, mm = pr.x & 255;
                 y = pr.y;
