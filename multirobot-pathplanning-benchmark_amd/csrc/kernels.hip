@@ -70,9 +70,24 @@ __global__ void k_plan_init(PlanWork W) {
 // addresses serialises in L2.
 #define PLAN_LDS_BINS 1024
 
-// A group of `lpe` lanes (a power of two) shares an edge: lane `sub` reads coordinates sub,
-// sub + lpe, ... of both rows, the maximum is reduced across the group, and lane sub == 0 writes
-// the edge's queue entries.
+// A group of `lpe` lanes (a power of two) shares an edge: lane `sub` holds coordinates sub,
+// sub + lpe, ... of both rows in registers, the maximum is reduced across the group, the direction
+// is written from the same registers, and lane sub == 0 writes the edge's queue entries.  The rows
+// of the group's next edge are loaded before the current edge is processed.  D <= PREP_MAXC << lg_lpe.
+#define PREP_MAXC 8
+
+__device__ inline void prep_load_rows(const EdgeBatchView& B, long long e, int D, int sub, int lg_lpe,
+                                      double* av, double* bv) {
+    const double* a = row1(B, e);
+    const double* b = row2(B, e);
+#pragma unroll
+    for (int j = 0; j < PREP_MAXC; ++j) {
+        const int k = sub + (j << lg_lpe);
+        av[j] = k < D ? a[k * B.ds] : 0.0;
+        bv[j] = k < D ? b[k * B.ds] : 0.0;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_plan_prep(EdgeBatchView B, PlanWork W, int D, int lgG,
                                                    int want_dir, int lg_lpe) {
     __shared__ int h[PLAN_LDS_BINS];
@@ -86,8 +101,18 @@ __global__ void __launch_bounds__(256) k_plan_prep(EdgeBatchView B, PlanWork W, 
     const long long per_grid = (long long)gridDim.x * (blockDim.x >> lg_lpe);
     // the lanes of a group share e, so they run the same iterations and their shuffles stay within
     // active lanes
-    for (long long e = (long long)blockIdx.x * (blockDim.x >> lg_lpe) + (threadIdx.x >> lg_lpe); e < B.n;
-         e += per_grid) {
+    const bool need_rows = !B.N || (want_dir && W.dir);
+    const long long e_first = (long long)blockIdx.x * (blockDim.x >> lg_lpe) + (threadIdx.x >> lg_lpe);
+    double an[PREP_MAXC], bn[PREP_MAXC];  // rows of the next edge
+    if (need_rows && e_first < B.n) prep_load_rows(B, e_first, D, sub, lg_lpe, an, bn);
+    for (long long e = e_first; e < B.n; e += per_grid) {
+        double av[PREP_MAXC], bv[PREP_MAXC];
+#pragma unroll
+        for (int j = 0; j < PREP_MAXC; ++j) {
+            av[j] = an[j];
+            bv[j] = bn[j];
+        }
+        if (need_rows && e + per_grid < B.n) prep_load_rows(B, e + per_grid, D, sub, lg_lpe, an, bn);
         int N;
         if (B.N) {
             N = B.N[e];
@@ -95,12 +120,14 @@ __global__ void __launch_bounds__(256) k_plan_prep(EdgeBatchView B, PlanWork W, 
             // the sequential rule (edge_N_dev): m = |a0 - b0|, then m = v > m ? v : m, so a NaN
             // in coordinate 0 sticks and a NaN elsewhere is skipped.  Lane 0 starts from
             // coordinate 0, the other lanes from 0.0; lane 0's reduction keeps the rule.
-            const double* a = row1(B, e);
-            const double* b = row2(B, e);
-            double m = sub == 0 ? __builtin_fabs(a[0] - b[0]) : 0.0;
-            for (int k = sub == 0 ? lpe : sub; k < D; k += lpe) {
-                const double v = __builtin_fabs(a[k * B.ds] - b[k * B.ds]);
-                m = v > m ? v : m;
+            double m = sub == 0 ? __builtin_fabs(av[0] - bv[0]) : 0.0;
+#pragma unroll
+            for (int j = 0; j < PREP_MAXC; ++j) {
+                const int k = sub + (j << lg_lpe);
+                if (k > 0 && k < D) {
+                    const double v = __builtin_fabs(av[j] - bv[j]);
+                    m = v > m ? v : m;
+                }
             }
             for (int o = 1; o < lpe; o <<= 1) {
                 const double v = __shfl_xor(m, o, lpe);
@@ -110,11 +137,14 @@ __global__ void __launch_bounds__(256) k_plan_prep(EdgeBatchView B, PlanWork W, 
             N = edge_N_from_max(m, B.res);
         }
         if (want_dir && W.dir) {
-            const double* a = row1(B, e);
-            const double* b = row2(B, e);
-            for (int k = sub; k < D; k += lpe) {
+            const double den = (double)(N - 1);
+#pragma unroll
+            for (int j = 0; j < PREP_MAXC; ++j) {
+                const int k = sub + (j << lg_lpe);
+                if (k < D) {
 #pragma clang fp contract(off)
-                W.dir[e * D + k] = (b[k * B.ds] - a[k * B.ds]) / (double)(N - 1);
+                    W.dir[e * D + k] = (bv[j] - av[j]) / den;
+                }
             }
         }
         if (sub != 0) continue;
@@ -773,7 +803,8 @@ void launch_plan_build(const SceneRec& sc, const EdgeBatchView& B, const PlanWor
     // 4 lanes per edge (fewer for D < 4): measured faster than one group of >= D lanes, whose
     // per-edge tail runs on one lane of a wide group; a few edges per group, so each block's LDS
     // histogram covers many edges
-    const int lg_lpe = sc.D >= 4 ? 2 : (sc.D >= 2 ? 1 : 0);
+    const int lg_lpe = sc.D > 32 ? 3 : (sc.D >= 4 ? 2 : (sc.D >= 2 ? 1 : 0));
+    // D <= MRCC_MAX_D = 64 = PREP_MAXC << 3, so every coordinate has a register in k_plan_prep
     int gp = grid_for(B.n, 4 * (256 >> lg_lpe));
     if (gp > 1024) gp = 1024;
     int gs = grid_for(B.n, 256 * SCATTER_EPT);
