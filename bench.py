@@ -476,7 +476,51 @@ def secondary_workloads(args, backend, scenes, dev):
     out["c5_mixed_arms"] = ent
     log(f"c5_mixed_arms: {ent}")
     out["c4_clearance"] = clearance_workload(args, backend, scenes, dev, threads)
+    out["config_cost"] = cost_workload(args, dev, threads)
     return out
+
+
+def cost_workload(args, dev, threads):
+    """The planners' configuration cost beside the collision path (abstract_env.py:238-253 with
+    the default cost_metric "euclidean" and cost_reduction "max", w = 0.01): one configuration
+    against 2^20 stored configurations of the 4 x 7-DoF layout, as a nearest-neighbour query does.
+    HBM-bound: each row is read once (algorithmic bytes = n * (8 D + 8) + 8 D per launch)."""
+    import torch
+    from oracle import oracle as O
+
+    cfg = importlib.import_module(f"{PKG}.configuration")
+    n, D = 1 << 20, 28
+    sl = np.array([[0, 7], [7, 14], [14, 21], [21, 28]])
+    rng = np.random.default_rng(9)
+    Q = rng.uniform(-2.9, 2.9, (n, D))
+    pt = rng.uniform(-2.9, 2.9, D)
+    Qd, Pd = torch.from_numpy(Q).to(dev), torch.from_numpy(pt).to(dev)
+    for _ in range(3):
+        got = cfg.batch_config_cost(Pd, Qd, "euclidean", "max", tmp_agent_slice=sl)
+    torch.cuda.synchronize()
+    reps = 50
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        got = cfg.batch_config_cost(Pd, Qd, "euclidean", "max", tmp_agent_slice=sl)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = n * (8 * D + 8) + 8 * D
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    ent = {"rows": n, "dim": D, "robots": 4, "metric": "euclidean", "reduction": "max", "ms": ms,
+           "rows_per_s": n / (ms * 1e-3),
+           "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": gbs / PEAK_HBM_GBS, "bytes_per_launch": nbytes}}
+    if not args.skip_cpu:
+        t0 = time.perf_counter()
+        ref = O.config_cost(pt, Q, sl, "euclidean", "max", threads=threads)
+        dt = time.perf_counter() - t0
+        ent["cpu_oracle"] = {"value": n / dt, "unit": "rows/s", "cores": threads}
+        ent["mismatches"] = int(np.sum(ref != got.cpu().numpy()))
+        ent["checks_compared"] = n
+    log(f"config_cost: {ent}")
+    return ent
 
 
 def clearance_workload(args, backend, scenes, dev, threads):

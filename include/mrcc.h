@@ -212,6 +212,31 @@ int mrcc_clearance_forward(const mrcc_scene* scene, const float* q, int64_t n, f
 int mrcc_clearance_backward(const mrcc_scene* scene, const float* q, int64_t n, float margin,
                             const float* grad_cost, float* grad_q, void* stream);
 
+/* ---- configuration-pair costs and distances (device) ------------------------------------- */
+/* The planners' metric functions (core/configuration.py):
+ *   batch_config_cost / _batch_config_cost_impl  :437-510 (config_cost :352-363)
+ *   NpConfiguration._batch_dist                   :303-329 (batch_config_dist :342-349)
+ * For row r < n the difference d_k = p[r*p_row_stride + k] - q[r*q_row_stride + k] (k < dim;
+ * p_row_stride = 0 broadcasts one configuration) is reduced over each slice i, coordinates
+ * [slices[2i], slices[2i+1]), to a value v_i by `metric`:
+ *   MRCC_COST_MAX        np.max(np.abs(d[s:e])) (:503; a NaN propagates)
+ *   MRCC_COST_EUCLIDEAN  sqrt of sum_squared += d_k * d_k in coordinate order, contracted
+ *                        (compute_sliced_euclidean_dists :101-126)
+ *   MRCC_COST_ABSMAX     m = |d_s|, then m = |d_k| > m ? |d_k| : m (compute_abs_max_reduction
+ *                        :174-197)
+ * and the v_i are reduced by `reduction`:
+ *   MRCC_REDUCE_MAX_SUM  max_i v_i + w * (v_0 + v_1 + ...), contracted (compute_max_sum_reduction
+ *                        :150-171; max by v_i > m)
+ *   MRCC_REDUCE_SUM      0.0 + v_0 + v_1 + ... (compute_sum_reduction :129-147)
+ *   MRCC_REDUCE_MAX      m = v_0, then v_i > m ? v_i : m (compute_max_reduction :200-219)
+ * p, q, out: device double; slices: host int32[2*n_slices], 1 <= n_slices <= 64, 0 <= s <= e <= dim
+ * (s < e unless metric is MRCC_COST_EUCLIDEAN); 1 <= dim <= 1024.  Stream-ordered. */
+enum { MRCC_COST_MAX = 0, MRCC_COST_EUCLIDEAN = 1, MRCC_COST_ABSMAX = 2 };
+enum { MRCC_REDUCE_MAX_SUM = 0, MRCC_REDUCE_SUM = 1, MRCC_REDUCE_MAX = 2 };
+int mrcc_config_cost(const double* p, int64_t p_row_stride, const double* q, int64_t q_row_stride,
+                     int64_t n, int32_t dim, const int32_t* slices, int32_t n_slices, int32_t metric,
+                     int32_t reduction, double w, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -3,21 +3,23 @@
 A new backend behind the collision API of vhartman/multirobot-pathplanning-benchmark
 (problems/planning_env.py:1724-1881): configuration validity, interpolated-edge validity and the
 forward kinematics feeding them run in hand-written HIP kernels (libmrcc.so, C ABI in
-include/mrcc.h); this package is the Python host side (ctypes), mirroring the reference's
-``BaseProblem`` interface.
+include/mrcc.h), and so do the planners' configuration costs and distances
+(core/configuration.py); this package is the Python host side (ctypes), mirroring the reference's
+``BaseProblem`` interface and configuration functions.
 
 The directory name is not a Python identifier; import it with
 ``importlib.import_module("multirobot-pathplanning-benchmark_amd")``.
 """
 from . import scenes
 from ._lib import LIB_PATH, MrccError, lib
-from .configuration import Configuration, NpConfiguration, config_dist, edge_num_states
+from .configuration import (Configuration, NpConfiguration, batch_config_cost, batch_config_dist, config_cost,
+                            config_dist, edge_num_states)
 from .planning_env import (BaseProblem, Mode, State, generate_binary_search_indices, get_env_by_name,
                            list_envs, register)
 
 __all__ = [
     "LIB_PATH", "MrccError", "lib", "scenes", "Configuration", "NpConfiguration", "config_dist",
-    "edge_num_states", "BaseProblem", "Mode", "State", "generate_binary_search_indices",
+    "batch_config_dist", "config_cost", "batch_config_cost", "edge_num_states", "BaseProblem", "Mode", "State", "generate_binary_search_indices",
     "get_env_by_name", "list_envs", "register", "backend", "envs",
 ]
 

@@ -85,7 +85,13 @@ SIGNATURES = {
     "mrcc_sincos": (_c.c_int, [_vp, _i64, _vp, _vp, _vp]),
     "mrcc_clearance_forward": (_c.c_int, [_vp, _vp, _i64, _c.c_float, _vp, _vp]),
     "mrcc_clearance_backward": (_c.c_int, [_vp, _vp, _i64, _c.c_float, _vp, _vp, _vp]),
+    "mrcc_config_cost": (_c.c_int, [_vp, _i64, _vp, _i64, _i64, _i32, _ip, _i32, _i32, _i32, _c.c_double,
+                                    _vp, _vp]),
 }
+
+# metric / reduction codes of mrcc_config_cost
+COST_MAX, COST_EUCLIDEAN, COST_ABSMAX = 0, 1, 2
+REDUCE_MAX_SUM, REDUCE_SUM, REDUCE_MAX = 0, 1, 2
 
 _lib = None
 

@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import EdgeBatch, Tune, check, dptr, iptr, lib
-from .configuration import Configuration, NpConfiguration, config_dist
+from .configuration import Configuration, NpConfiguration, batch_config_cost, config_cost
 from .planning_env import BaseProblem, Mode, State, generate_binary_search_indices
 from .scenes import SceneDesc
 
@@ -359,4 +359,10 @@ class MrccEnvironment(BaseProblem):
 
     # ---- misc ------------------------------------------------------------------------------
     def config_cost(self, start: Configuration, end: Configuration) -> float:
-        return config_dist(start, end, "euclidean")
+        """abstract_env.py:238-239: config_cost(start, end, cost_metric, cost_reduction)."""
+        return config_cost(start, end, self.cost_metric, self.cost_reduction)
+
+    def batch_config_cost(self, starts, ends, tmp_agent_slice=None):
+        """abstract_env.py:241-253 (one device launch, mrcc_config_cost)."""
+        return batch_config_cost(starts, ends, self.cost_metric, self.cost_reduction,
+                                 tmp_agent_slice=tmp_agent_slice)

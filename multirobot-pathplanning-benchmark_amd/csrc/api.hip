@@ -1108,4 +1108,43 @@ int mrcc_clearance_backward(const mrcc_scene* scene, const float* q, int64_t n, 
     return MRCC_OK;
 }
 
+int mrcc_config_cost(const double* p, int64_t p_row_stride, const double* q, int64_t q_row_stride,
+                     int64_t n, int32_t dim, const int32_t* slices, int32_t n_slices, int32_t metric,
+                     int32_t reduction, double w, double* out, void* stream) {
+    CHECK_INJECT();
+    if (n < 0) return fail(MRCC_ERR_INVALID, "n < 0");
+    if (dim < 1 || dim > MRCC_COST_MAX_D) return fail(MRCC_ERR_INVALID, "dim must be in 1..%d", MRCC_COST_MAX_D);
+    if (metric < MRCC_COST_MAX || metric > MRCC_COST_ABSMAX) return fail(MRCC_ERR_INVALID, "unknown metric %d", metric);
+    if (reduction < MRCC_REDUCE_MAX_SUM || reduction > MRCC_REDUCE_MAX)
+        return fail(MRCC_ERR_INVALID, "unknown reduction %d", reduction);
+    if (!slices || n_slices < 1 || n_slices > MRCC_COST_MAX_SLICES)
+        return fail(MRCC_ERR_INVALID, "1..%d slices required", MRCC_COST_MAX_SLICES);
+    if (p_row_stride < 0 || q_row_stride < 0) return fail(MRCC_ERR_INVALID, "negative row stride");
+    CostArgs A;
+    memset(&A, 0, sizeof(A));
+    for (int i = 0; i < n_slices; ++i) {
+        const int s = slices[2 * i], e = slices[2 * i + 1];
+        if (s < 0 || e > dim || s > e || (s == e && metric != MRCC_COST_EUCLIDEAN))
+            return fail(MRCC_ERR_INVALID, "slice %d [%d, %d) is empty or outside [0, %d)", i, s, e, dim);
+        A.sl[2 * i] = s;
+        A.sl[2 * i + 1] = e;
+    }
+    if (n == 0) return MRCC_OK;
+    if (!p || !q || !out) return fail(MRCC_ERR_INVALID, "null argument");
+    A.P = p;
+    A.ps = p_row_stride;
+    A.Q = q;
+    A.qs = q_row_stride;
+    A.n = n;
+    A.D = dim;
+    A.ns = n_slices;
+    A.metric = metric;
+    A.red = reduction;
+    A.w = w;
+    A.out = out;
+    launch_config_cost(A, 32 * device_cus(), (hipStream_t)stream);
+    HIP_OK(hipGetLastError());
+    return MRCC_OK;
+}
+
 }  // extern "C"

@@ -43,3 +43,21 @@ void launch_arm_fk(const SceneRec& sc, const double* q, long long n, long long r
 void launch_sincos(const double* x, long long n, double* s, double* c, hipStream_t st);
 void launch_clearance(const SceneRec& sc, const float* q, long long n, float margin,
                       const float* grad_cost, float* cost, float* grad_q, hipStream_t st);
+
+// configuration-pair costs (cost.hip): row r of the result reduces d = P[r*ps + k] - Q[r*qs + k]
+// (k < D) over the slices [sl[2i], sl[2i+1])
+#define MRCC_COST_MAX_SLICES 64
+#define MRCC_COST_MAX_D 1024
+#define MRCC_COST_TILE_BYTES 65536  // LDS of one workgroup's tile of rows
+struct CostArgs {
+    const double* P;
+    long long ps;
+    const double* Q;
+    long long qs;
+    long long n;
+    int D, ns, metric, red;
+    double w;
+    double* out;
+    int sl[2 * MRCC_COST_MAX_SLICES];
+};
+void launch_config_cost(const CostArgs& A, int max_grid, hipStream_t st);

@@ -18,7 +18,7 @@ from typing import Any, Dict, List, Optional
 import numpy as np
 from numpy.typing import NDArray
 
-from .configuration import Configuration, config_dist
+from .configuration import Configuration, edge_num_states
 
 
 @cache
@@ -167,7 +167,7 @@ class BaseProblem(ABC):
                     q2 = path[i + 1].q
                     mode = path[i].mode
                     if i not in Ns:
-                        Ns[i] = max(2, int(config_dist(q1, q2, "max") / resolution) + 1)
+                        Ns[i] = edge_num_states(q1.state(), q2.state(), resolution)
                     if N_start == 0:
                         if (i != 0 or (check_start_and_end and i == 0)) and not self.is_collision_free(q1, mode):
                             return False

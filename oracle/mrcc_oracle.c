@@ -746,6 +746,84 @@ int or_clearance_cost_batch(const or_scene* s, const double* q, int64_t n, doubl
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/* configuration-pair costs (core/configuration.py) */
+
+/* compute_sliced_euclidean_dists (:101-126): sum_squared = 0.0; sum_squared += d*d in coordinate
+ * order (numba fastmath contracts the step); sqrt.  This is also np.linalg.norm of a short vector
+ * (the fused chain), which test_config.py:110-126 compares for equality. */
+static double cost_euclid(const double* d, int s, int e) {
+    double acc = 0.0;
+    for (int k = s; k < e; ++k) acc = fma(d[k], d[k], acc);
+    return sqrt(acc);
+}
+
+/* np.max(np.abs(diff[:, s:e]), axis=1) (:503): the maximum; a NaN propagates */
+static double cost_np_absmax(const double* d, int s, int e) {
+    double m = fabs(d[s]);
+    for (int k = s + 1; k < e; ++k) {
+        const double u = fabs(d[k]);
+        if (isnan(m)) break;
+        if (isnan(u) || u > m) m = u;
+    }
+    return m;
+}
+
+/* compute_abs_max_reduction (:174-197): max_val = abs(d[0]); abs_val > max_val replaces it */
+static double cost_seq_absmax(const double* d, int s, int e) {
+    double m = fabs(d[s]);
+    for (int k = s + 1; k < e; ++k) {
+        const double u = fabs(d[k]);
+        if (u > m) m = u;
+    }
+    return m;
+}
+
+int or_config_cost(const double* p, int64_t ps, const double* q, int64_t qs, int64_t n, int dim,
+                   const int32_t* sl, int n_slices, int metric, int reduction, double w, double* out,
+                   int threads) {
+    if (dim < 1 || dim > 1024 || n_slices < 1 || n_slices > 64 || metric < 0 || metric > 2 ||
+        reduction < 0 || reduction > 2)
+        return -1;
+    for (int i = 0; i < n_slices; ++i)
+        if (sl[2 * i] < 0 || sl[2 * i + 1] > dim || sl[2 * i] > sl[2 * i + 1] ||
+            (sl[2 * i] == sl[2 * i + 1] && metric != 1))
+            return -1;
+#ifdef _OPENMP
+    if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(threads)
+#endif
+    for (int64_t r = 0; r < n; ++r) {
+        double d[1024];
+        double v[64] = {0.0};
+        for (int k = 0; k < dim; ++k) d[k] = p[r * ps + k] - q[r * qs + k];
+        for (int i = 0; i < n_slices; ++i) {
+            const int s = sl[2 * i], e = sl[2 * i + 1];
+            v[i] = metric == 1 ? cost_euclid(d, s, e) : (metric == 0 ? cost_np_absmax(d, s, e)
+                                                                      : cost_seq_absmax(d, s, e));
+        }
+        double res;
+        if (reduction == 1) {  /* compute_sum_reduction (:129-147) */
+            double sum_val = 0.0;
+            for (int i = 0; i < n_slices; ++i) sum_val += v[i];
+            res = sum_val;
+        } else {
+            /* compute_max_reduction (:200-219) / compute_max_sum_reduction (:150-171):
+             * max_val = sum_val = dists[0]; dists[i] > max_val replaces it; sum_val += dists[i];
+             * max_val + w * sum_val, contracted */
+            double max_val = v[0], sum_val = v[0];
+            for (int i = 1; i < n_slices; ++i) {
+                if (v[i] > max_val) max_val = v[i];
+                sum_val += v[i];
+            }
+            res = reduction == 2 ? max_val : fma(w, sum_val, max_val);
+        }
+        out[r] = res;
+    }
+    (void)threads;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------------------------ */
 int or_test_obb2(const double* A8, const double* B8) { return obb2_overlap(A8, B8); }
 int or_test_obb3(const double* A12, const double* a3, const double* B12, const double* b3) {
     return obb3_overlap(A12, a3, B12, b3);

@@ -17,6 +17,8 @@
  *   or_planar_config_free     2D oriented boxes, closed separating-axis test
  *   or_arm_fk / or_arm_config_free   DH arms, link spheres, AABB obstacles, grasped boxes
  *   or_clearance_cost / _grad        summed squared penetration and its joint gradient
+ * and the planners' cost / distance functions beside the path (core/configuration.py):
+ *   or_config_cost            :101-219 numba reductions, :303-329 _batch_dist, :437-510 batch_config_cost
  * Arithmetic is IEEE binary64 with contraction disabled (-ffp-contract=off); where the reference
  * goes through numpy the numpy rounding sequence is reproduced (fused np.linalg.norm on small
  * vectors, pairwise np.sum, libm pow for r**2).
@@ -76,6 +78,16 @@ double or_clearance_cost(const or_scene* s, const double* q, double margin);
 int or_clearance_grad(const or_scene* s, const double* q, double margin, double* grad);
 int or_clearance_cost_batch(const or_scene* s, const double* q, int64_t n, double margin,
                             double* cost, int threads);
+
+/* Costs / distances of configuration pairs (core/configuration.py): per row r, d_k =
+ * p[r*ps + k] - q[r*qs + k]; per slice [sl[2i], sl[2i+1]) a value by `metric`
+ * (0 np.max(np.abs) :503, 1 compute_sliced_euclidean_dists :101-126, 2
+ * compute_abs_max_reduction :174-197), then a reduction over the slices (0
+ * compute_max_sum_reduction :150-171, 1 compute_sum_reduction :129-147, 2 compute_max_reduction
+ * :200-219).  Contracted steps are explicit fma().  Returns 0, or -1 for a bad argument. */
+int or_config_cost(const double* p, int64_t ps, const double* q, int64_t qs, int64_t n, int dim,
+                   const int32_t* sl, int n_slices, int metric, int reduction, double w, double* out,
+                   int threads);
 
 /* single predicates (for pinning tests): 1 overlap / 0 separated */
 int or_test_obb2(const double* A8, const double* B8);

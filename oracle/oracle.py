@@ -65,6 +65,9 @@ def lib():
         L.or_clearance_grad.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp]
         L.or_clearance_cost_batch.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int64,
                                               ctypes.c_double, _dp, ctypes.c_int]
+        L.or_config_cost.argtypes = [_dp, ctypes.c_int64, _dp, ctypes.c_int64, ctypes.c_int64,
+                                     ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_double, _dp, ctypes.c_int]
         L.or_test_obb2.argtypes = [_dp, _dp]
         L.or_test_obb3.argtypes = [_dp, _dp, _dp, _dp]
         L.or_test_sph_obb.argtypes = [_dp, ctypes.c_double, _dp, _dp]
@@ -252,3 +255,28 @@ class Scene:
         assert lib().or_clearance_cost_batch(self.h, p, q.shape[0], float(margin),
                                              out.ctypes.data_as(_dp), threads) == 0
         return out
+
+
+COST_METRICS = {"max": 0, "euclidean": 1, "absmax": 2}
+COST_REDUCTIONS = {"max": 0, "sum": 1, "max_only": 2}
+
+
+def config_cost(p, q, slices, metric="max", reduction="max", w=0.01, threads=1):
+    """or_config_cost over rows: p (n, D) or (D,) broadcast, q (n, D) float64; slices [(s, e)].
+    metric: "max" (np.max(np.abs)), "euclidean", "absmax" (compute_abs_max_reduction);
+    reduction: "max" (max + w * sum), "sum", "max_only" (compute_max_reduction)."""
+    q = np.ascontiguousarray(np.atleast_2d(np.asarray(q, dtype=np.float64)))
+    p = np.ascontiguousarray(np.asarray(p, dtype=np.float64))
+    n, D = q.shape
+    ps = 0 if p.ndim == 1 else D
+    if p.ndim != 1:
+        assert p.shape == q.shape
+    sl = np.ascontiguousarray(np.asarray(slices, dtype=np.int32).reshape(-1))
+    out = np.empty(n, dtype=np.float64)
+    (_, pp), (_, pq), (_, psl) = _d(p), _d(q), _i(sl)
+    rc = lib().or_config_cost(pp, ps, pq, D, n, D, psl, len(sl) // 2, COST_METRICS[metric],
+                              COST_REDUCTIONS[reduction], float(w),
+                              out.ctypes.data_as(_dp), int(threads))
+    if rc != 0:
+        raise ValueError("or_config_cost: bad argument")
+    return out

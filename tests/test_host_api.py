@@ -1,8 +1,9 @@
 """Host-side mirror of the reference interface (no device work).
 
-config_dist invariants re-expressed from tests/test_config.py:134-185 (zero distance, equal points,
-translation), the "max" metric that sets the number of edge states, the NpConfiguration layout,
-Mode / State, the registry (with the registry keys of abstract_env.py:381, :443-446)."""
+The "max" metric that sets the number of edge states, the NpConfiguration layout, Mode / State,
+the registry (with the registry keys of abstract_env.py:381, :443-446), and that the device-backed
+cost functions refuse to run without a device.  The distance / cost invariants of
+tests/test_config.py run on the device in test_gpu_cost.py."""
 import numpy as np
 import pytest
 
@@ -17,36 +18,26 @@ def _cfg(dims, pt):
     return pkg.NpConfiguration(pt, slices)
 
 
-@pytest.mark.parametrize("metric", METRICS)
-@pytest.mark.parametrize("dims", DIMS)
-def test_config_dist_zero_and_equal(metric, dims):
-    rng = np.random.default_rng(0)
-    D = sum(dims)
-    assert pkg.NpConfiguration._batch_dist(_cfg(dims, np.zeros(D)), np.zeros((50, D)), metric).max() == 0
-    pt = rng.random(D)
-    assert pkg.NpConfiguration._batch_dist(_cfg(dims, pt), np.tile(pt, (50, 1)), metric).max() == 0
+def test_edge_num_states_rule():
+    """N of an edge from the max metric (abstract_env.py:321-323): a NaN counts only in coordinate
+    0 (compute_abs_max_reduction, configuration.py:174-197); NaN there makes int() raise."""
+    assert pkg.edge_num_states(np.zeros(3), np.array([0.5, -0.25, 0.1]), 0.1) == 6
+    assert pkg.edge_num_states(np.zeros(3), np.zeros(3), 0.1) == 2
+    assert pkg.edge_num_states(np.zeros(3), np.array([0.05, np.nan, 0.31]), 0.1) == 4
+    with pytest.raises(ValueError):
+        pkg.edge_num_states(np.zeros(3), np.array([np.nan, 0.0, 0.31]), 0.1)
 
 
-@pytest.mark.parametrize("metric", METRICS)
-@pytest.mark.parametrize("dims", DIMS)
-def test_config_dist_translation(metric, dims):
-    rng = np.random.default_rng(1)
-    D = sum(dims)
-    c = _cfg(dims, rng.random(D))
-    pts = rng.random((100, D))
-    d0 = pkg.NpConfiguration._batch_dist(c, pts, metric)
-    d1 = pkg.NpConfiguration._batch_dist(c.from_flat(c.state() + 1), pts + 1, metric)
-    assert np.allclose(d0, d1)
-
-
-@pytest.mark.parametrize("dims", DIMS)
-def test_max_metric_manual(dims):
-    rng = np.random.default_rng(2)
-    D = sum(dims)
-    a, b = _cfg(dims, rng.random(D)), _cfg(dims, rng.random(D))
-    assert pkg.config_dist(a, b, "max") == np.max(np.abs(a.state() - b.state()))
-    e = pkg.config_dist(a, b, "max_euclidean")
-    assert np.isclose(e, max(np.linalg.norm(a[i] - b[i]) for i in range(len(dims))))
+def test_cost_functions_need_a_device():
+    """batch_config_cost / batch_config_dist run on the device and have no CPU fallback."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    c = _cfg([2, 2], np.zeros(4))
+    with pytest.raises(RuntimeError):
+        pkg.batch_config_cost(c, np.zeros((3, 4)))
+    with pytest.raises(RuntimeError):
+        pkg.batch_config_dist(c, np.zeros((3, 4)), "euclidean")
 
 
 def test_np_configuration_layout():

@@ -290,7 +290,7 @@ class OracleEnv(pkg.BaseProblem):
         if resolution is None:
             resolution = self.collision_resolution
         if N is None:
-            N = max(2, int(pkg.config_dist(q1, q2, "max") / resolution) + 1)
+            N = pkg.edge_num_states(q1.state(), q2.state(), resolution)
         if N_start > N:
             assert False
         N_max = N if N_max is None else min(N, N_max)
