@@ -48,7 +48,7 @@ void launch_clearance(const SceneRec& sc, const float* q, long long n, float mar
 // (k < D) over the slices [sl[2i], sl[2i+1])
 #define MRCC_COST_MAX_SLICES 64
 #define MRCC_COST_MAX_D 1024
-#define MRCC_COST_TILE_BYTES 65536  // LDS of one workgroup's tile of rows
+#define MRCC_COST_TILE_BYTES 8192  // LDS of one workgroup's tile of rows (narrow rows)
 struct CostArgs {
     const double* P;
     long long ps;
