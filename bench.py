@@ -492,7 +492,66 @@ def secondary_workloads(args, backend, scenes, dev):
     log(f"c5_mixed_arms: {ent}")
     out["c4_clearance"] = clearance_workload(args, backend, scenes, dev, threads)
     out["config_cost"] = cost_workload(args, dev, threads)
+    out["rrt_call_pattern"] = call_pattern_workload(dev)
     return out
+
+
+def call_pattern_workload(dev, iters=200, k=30):
+    """The RRT* call pattern (rrtstar_base.py:1339-1446) on the 4 x 7-DoF arm environment: per
+    new node, the parent choice over k near candidates, then the rewire checks of the eligible
+    ones; per-call loop (one environment call per edge, as the planner does) against the batched
+    helpers (one launch per decision).  Both decide the same; the time per iteration is reported."""
+    import torch
+
+    pkg = importlib.import_module(PKG)
+    importlib.import_module(f"{PKG}.envs")
+    callers = importlib.import_module(f"{PKG}.callers")
+    env = pkg.get_env_by_name("mrcc.quad_arms")
+    rng = np.random.default_rng(12)
+    D = env.desc.dim
+    cases = []
+    for _ in range(iters):
+        centre = np.clip(rng.normal(0.0, 0.4, D), -2.9, 2.9)
+        rows = np.clip(centre + rng.normal(0.0, 0.35, (k, D)), -2.9, 2.9)
+        q_new = env.start_pos.from_flat(centre)
+        near = [env.start_pos.from_flat(r) for r in rows]
+        ids = rng.permutation(10 * k)[:k]
+        cases.append((q_new, near, ids, ids[rng.integers(k)], env.batch_config_cost(q_new, rows),
+                      rng.uniform(0, 3, k), rng.random(k) < 0.7))
+
+    def loop(c):
+        q_new, near, ids, nearest, bc, nc, elig = c
+        idx = int(np.where(ids == nearest)[0][0])
+        cn = nc + bc
+        chosen = idx
+        mask = cn < cn[idx]
+        if np.any(mask):
+            for i in np.where(mask)[0][np.argsort(cn[mask])]:
+                if env.is_edge_collision_free(near[i], q_new, env.start_mode):
+                    chosen = int(i)
+                    break
+        rew = [bool(e) and env.is_edge_collision_free(q_new, near[i], env.start_mode) for i, e in enumerate(elig)]
+        return chosen, rew
+
+    def batched(c):
+        q_new, near, ids, nearest, bc, nc, elig = c
+        chosen = callers.find_parent_batched(env, env.start_mode, near, q_new, ids, nearest, bc, nc)[0]
+        return chosen, list(callers.rewire_batched(env, env.start_mode, q_new, near, elig))
+
+    loop(cases[0]), batched(cases[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    a = [loop(c) for c in cases]
+    t_loop = (time.perf_counter() - t0) / iters
+    t0 = time.perf_counter()
+    b = [batched(c) for c in cases]
+    t_b = (time.perf_counter() - t0) / iters
+    edges = sum(int(np.sum(c[6])) for c in cases)
+    ent = {"iterations": iters, "near_candidates": k, "rewire_edges": edges,
+           "per_call_ms_per_iteration": t_loop * 1e3, "batched_ms_per_iteration": t_b * 1e3,
+           "speedup": t_loop / t_b, "same_decisions": a == b}
+    log(f"rrt_call_pattern: {ent}")
+    return ent
 
 
 def cost_workload(args, dev, threads):

@@ -28,6 +28,6 @@ def __getattr__(name):
     # the backend (and the registered environments) load libmrcc lazily
     import importlib
 
-    if name in ("backend", "envs", "clearance", "autotune"):
+    if name in ("backend", "envs", "clearance", "autotune", "callers"):
         return importlib.import_module(f"{__name__}.{name}")
     raise AttributeError(name)

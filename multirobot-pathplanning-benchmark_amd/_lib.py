@@ -103,6 +103,15 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() "
                               f"(hipcc for gfx950); there is no CPU fallback")
+        # PyTorch ships its own HIP runtime.  Loading it first lets the library bind to that same
+        # runtime, so device buffers and streams from torch are valid here; loaded the other way
+        # round, torch could no longer initialise its device.
+        try:
+            import torch
+
+            torch.cuda.is_available()
+        except ImportError:
+            pass
         L = _c.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
