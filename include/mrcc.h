@@ -183,6 +183,17 @@ int mrcc_check_edge_host(const mrcc_scene* scene, const double* q1, const double
                          int32_t* result);
 int mrcc_check_config_host(const mrcc_scene* scene, const double* q, int32_t mode_id,
                            uint32_t robot_mask, int32_t* result);
+/* A short list of edges on host memory (a planner's candidate list: find_parent / rewire
+ * rrtstar_base.py:1339-1446, PRM edges prm_graph.py:663-684, EIT windows planner_eitstar.py:630-665):
+ * rows q1[i*D ..], q2[i*D ..] (host double), optional per-edge host int32 arrays N (<= 0: from the
+ * resolution), N_start, N_max (< 0: None), mode_ids (NULL: defaults).  One pinned staging copy
+ * each way and one launch (several wavefronts per edge for short lists), synchronous.
+ * result[i] = 1 when edge i is collision free (is_edge_collision_free, planning_env.py:1746-1763),
+ * 0 otherwise.  N_start > N (the reference asserts) or N_start < 0 -> MRCC_ERR_INVALID. */
+int mrcc_check_edges_host(const mrcc_scene* scene, const double* q1, const double* q2, int64_t n,
+                          double resolution, const int32_t* N, const int32_t* N_start,
+                          const int32_t* N_max, int32_t include_endpoints, const int32_t* mode_ids,
+                          uint32_t robot_mask, uint8_t* result);
 /* The plain per-call form applied to every edge of a batch: n_edges launches of the
  * mrcc_check_edge kernel (host loop, no Python in between). */
 int mrcc_check_edges_per_call(const mrcc_scene* scene, const mrcc_edge_batch* batch, void* stream);

@@ -19,6 +19,7 @@ from .planning_env import BaseProblem, Mode, State, generate_binary_search_indic
 from .scenes import SceneDesc
 
 ALL_ROBOTS = 0xFFFFFFFF
+HOST_BATCH_MAX = 4096  # edge lists up to this length go through mrcc_check_edges_host
 
 
 def _torch():
@@ -325,11 +326,23 @@ class MrccEnvironment(BaseProblem):
                 assert False, "N_start > N"
             if np.any(np.asarray(N_start) < 0):
                 raise ValueError("N_start < 0 is not supported")
+        mask = ALL_ROBOTS if robots is None else self.robot_mask(robots)
+        if n <= HOST_BATCH_MAX and tune is None:
+            # a planner's candidate list: one staging copy each way and one launch, no plan
+            hv = lambda v: None if v is None else np.ascontiguousarray(np.broadcast_to(v, (n,)), dtype=np.int32)
+            Nh, Sh, Mh = hv(N), hv(N_start), hv(N_max)
+            Mh = None if Mh is None else np.where(Mh < 0, -1, Mh).astype(np.int32)
+            mh = None if modes is None else np.asarray([self.mode_id(m) for m in modes], dtype=np.int32)
+            res = np.empty(n, dtype=np.uint8)
+            p = lambda v: None if v is None else iptr(v)
+            check(lib().mrcc_check_edges_host(self.scene.handle, dptr(A), dptr(B), n, float(resolution), p(Nh),
+                                              p(Sh), p(Mh), int(bool(include_endpoints)), p(mh), mask,
+                                              res.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+            return res.astype(bool)
         a, b = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
         out = torch.empty(n, dtype=torch.uint8, device="cuda")
         Nt, St, Mt = iv(N), iv(N_start), iv(N_max)  # N_max entries < 0 mean None
         md = iv(None if modes is None else [self.mode_id(m) for m in modes])
-        mask = ALL_ROBOTS if robots is None else self.robot_mask(robots)
         batch = self.scene.edge_batch(a, b, out, resolution, Nt, St, Mt, include_endpoints, md, mask)
         self.scene.check_edges_device(batch, tune)
         return out.cpu().numpy().astype(bool)

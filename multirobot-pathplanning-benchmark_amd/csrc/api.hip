@@ -691,6 +691,8 @@ int mrcc_scene_destroy(mrcc_scene* scene) {
     if (sc->dev_mem) (void)hipFree(sc->dev_mem);
     if (sc->stage_dev) (void)hipFree(sc->stage_dev);
     if (sc->stage_host) (void)hipHostFree(sc->stage_host);
+    if (sc->batch_dev) (void)hipFree(sc->batch_dev);
+    if (sc->batch_host) (void)hipHostFree(sc->batch_host);
     delete box;
     return MRCC_OK;
 }
@@ -988,6 +990,100 @@ int mrcc_check_edge_host(const mrcc_scene* scene, const double* q1, const double
     HIP_OK(hipMemcpyAsync(sc->stage_out_host, sc->stage_out_dev, 1, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     *result = sc->stage_out_host[0] ? 1 : 0;
+    return MRCC_OK;
+}
+
+// Pinned and device staging of at least `bytes` for mrcc_check_edges_host (calls are synchronous,
+// so a new call never overwrites a block a previous one still reads).
+static int ensure_batch_stage(SceneRec* sc, size_t bytes) {
+    if (bytes <= sc->batch_cap) return MRCC_OK;
+    size_t cap = std::max(bytes, 2 * sc->batch_cap);
+    cap = (cap + 4095) & ~(size_t)4095;
+    if (sc->batch_dev) (void)hipFree(sc->batch_dev);
+    if (sc->batch_host) (void)hipHostFree(sc->batch_host);
+    sc->batch_dev = nullptr;
+    sc->batch_host = nullptr;
+    sc->batch_cap = 0;
+    HIP_OK(hipMalloc(&sc->batch_dev, cap));
+    HIP_OK(hipHostMalloc(&sc->batch_host, cap));
+    sc->batch_cap = cap;
+    return MRCC_OK;
+}
+
+int mrcc_check_edges_host(const mrcc_scene* scene, const double* q1, const double* q2, int64_t n,
+                          double resolution, const int32_t* N, const int32_t* N_start,
+                          const int32_t* N_max, int32_t include_endpoints, const int32_t* mode_ids,
+                          uint32_t robot_mask, uint8_t* result) {
+    CHECK_INJECT();
+    if (!scene) return fail(MRCC_ERR_INVALID, "scene is null");
+    if (n < 0) return fail(MRCC_ERR_INVALID, "n < 0");
+    if (n == 0) return MRCC_OK;
+    if (!q1 || !q2 || !result) return fail(MRCC_ERR_INVALID, "null argument");
+    if (!N && !(resolution > 0.0)) return fail(MRCC_ERR_INVALID, "resolution must be > 0");
+    SceneRec* sc = (SceneRec*)rec_of(scene);
+    const int D = sc->D;
+    // the reference's argument checks, per edge (abstract_env.py:321-331); entries N <= 0 of a
+    // given N array are replaced by the resolution's N
+    std::vector<int32_t> Nfill;
+    if (N) Nfill.assign(N, N + n);
+    for (int64_t i = 0; i < n; ++i) {
+        int Nh = N ? N[i] : 0;
+        if (Nh <= 0) {
+            const double* a = q1 + i * D;
+            const double* b = q2 + i * D;
+            double m = fabs(a[0] - b[0]);
+            for (int k = 1; k < D; ++k) m = std::max(m, fabs(a[k] - b[k]));
+            Nh = mrcc::edge_N_from_max(m, resolution);
+        }
+        const int ns = N_start ? N_start[i] : 0;
+        if (ns < 0) return fail(MRCC_ERR_INVALID, "edge %lld: N_start < 0", (long long)i);
+        if (ns > Nh) return fail(MRCC_ERR_INVALID, "edge %lld: N_start %d > N %d (the reference asserts)", (long long)i, ns, Nh);
+        if (Nh > MRCC_MAX_EDGE_STATES)
+            return fail(MRCC_ERR_INVALID, "edge %lld of %d states (at most %d)", (long long)i, Nh, MRCC_MAX_EDGE_STATES);
+        if (N) Nfill[i] = Nh;
+    }
+    // block: q1 rows, q2 rows, then the int32 arrays given, then one result byte per edge
+    const size_t rows = 8 * (size_t)n * D;
+    const int32_t* ints[4] = {N ? Nfill.data() : nullptr, N_start, N_max, mode_ids};
+    size_t off[4], o = 2 * rows;
+    for (int j = 0; j < 4; ++j) {
+        off[j] = o;
+        if (ints[j]) o += 4 * (size_t)n;
+    }
+    const size_t o_out = o, bytes = o + (size_t)n;
+    int rc = ensure_batch_stage(sc, bytes);
+    if (rc != MRCC_OK) return rc;
+    unsigned char* h = sc->batch_host;
+    memcpy(h, q1, rows);
+    memcpy(h + rows, q2, rows);
+    for (int j = 0; j < 4; ++j)
+        if (ints[j]) memcpy(h + off[j], ints[j], 4 * (size_t)n);
+    memset(h + o_out, 1, (size_t)n);  // verdicts start free; waves write 0 on a collision
+    hipStream_t st = nullptr;
+    HIP_OK(hipMemcpyAsync(sc->batch_dev, h, bytes, hipMemcpyHostToDevice, st));
+    unsigned char* d = sc->batch_dev;
+    EdgeBatchView v;
+    memset(&v, 0, sizeof(v));
+    v.q1 = (const double*)d;
+    v.q2 = (const double*)(d + rows);
+    v.rs = D;
+    v.ds = 1;
+    v.n = n;
+    v.res = resolution;
+    v.N = N ? (const int*)(d + off[0]) : nullptr;
+    v.N_start = N_start ? (const int*)(d + off[1]) : nullptr;
+    v.N_max = N_max ? (const int*)(d + off[2]) : nullptr;
+    v.modes = mode_ids ? (const int*)(d + off[3]) : nullptr;
+    v.incl = include_endpoints;
+    v.mask = robot_mask;
+    v.out = d + o_out;
+    // several waves per edge for short lists (states u, u + wpe, ... of the edge), one for long ones
+    const int wpe = n >= 2048 ? 1 : (int)std::min<int64_t>(64, std::max<int64_t>(1, 2048 / n));
+    launch_direct(*sc, v, nullptr, n, nullptr, wpe, 0, 32 * device_cus(), nullptr, st);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(h + o_out, d + o_out, (size_t)n, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    for (int64_t i = 0; i < n; ++i) result[i] = h[o_out + i] ? 1 : 0;
     return MRCC_OK;
 }
 

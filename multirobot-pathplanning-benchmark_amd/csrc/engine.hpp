@@ -25,6 +25,10 @@ struct SceneRec {
     double* stage_host;
     unsigned char* stage_out_dev;
     unsigned char* stage_out_host;
+    // staging of the host-memory batch entry point (grown on demand)
+    unsigned char* batch_dev;
+    unsigned char* batch_host;
+    size_t batch_cap;
 };
 
 void launch_plan_build(const SceneRec& sc, const EdgeBatchView& B, const PlanWork& W, int lgG,

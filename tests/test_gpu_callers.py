@@ -116,3 +116,84 @@ def test_sparse_checks_match_loop():
                 continue
             exp = env.is_edge_collision_free(starts[i], ends[i], env.start_mode, N_start=N_start, N_max=N_max, N=Ns[i])
             assert got[i] == exp, (N_start, i)
+
+
+# ---------------------------------------------------------------------------------------------
+# mrcc_check_edges_host (short lists on host memory) against the oracle
+def _host_edges(sc, q1, q2, res, N=None, N_start=None, N_max=None, incl=False, modes=None, mask=0xFFFFFFFF):
+    import ctypes
+    L = pkg.lib()
+    n = len(q1)
+    q1 = np.ascontiguousarray(q1, dtype=np.float64)
+    q2 = np.ascontiguousarray(q2, dtype=np.float64)
+    iv = lambda v: None if v is None else np.ascontiguousarray(v, dtype=np.int32)
+    arrs = [iv(N), iv(N_start), iv(N_max), iv(modes)]
+    ptr = lambda v: None if v is None else v.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    out = np.full(n, 7, dtype=np.uint8)
+    rc = L.mrcc_check_edges_host(sc.handle, q1.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 q2.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n, float(res),
+                                 ptr(arrs[0]), ptr(arrs[1]), ptr(arrs[2]), int(incl), ptr(arrs[3]), mask,
+                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    return rc, out
+
+
+@pytest.mark.parametrize("n", [1, 7, 30, 500, 3000])
+def test_host_list_c3_matches_oracle(n):
+    from helpers import O, backend, scenes
+    rng = np.random.default_rng(n)
+    desc = scenes.quad_arms(rng=rng)
+    sc = backend().Scene(desc)
+    o = O.Scene(desc)
+    base = near_home(rng, desc, n)
+    q1 = base
+    q2 = np.clip(base + rng.normal(0, 0.4, base.shape), desc.limits[0], desc.limits[1])
+    print(f"[case] c3 host list n={n}", flush=True)
+    rc, got = _host_edges(sc, q1, q2, 0.05)
+    assert rc == 0 and set(np.unique(got)) <= {0, 1}
+    exp = o.check_edges(q1, q2, 0.05)
+    assert np.array_equal(got.astype(bool), exp)
+    # windows of the binary order with explicit N, endpoints, and a robot mask
+    N = np.maximum(2, (np.max(np.abs(q1 - q2), axis=1) / 0.05).astype(np.int64) + 1)
+    S = np.minimum(N, rng.integers(0, 12, n))
+    M = np.where(rng.random(n) < 0.3, -1, S + rng.integers(0, 15, n))
+    for incl in (False, True):
+        rc, got = _host_edges(sc, q1, q2, 0.05, N=N, N_start=S, N_max=M, incl=incl, mask=0b0101)
+        exp = o.check_edges(q1, q2, 0.05, N=N, N_start=S, N_max=M, include_endpoints=incl, mask=0b0101)
+        assert rc == 0 and np.array_equal(got.astype(bool), exp), incl
+
+
+def test_host_list_c1_and_c5_match_oracle():
+    from helpers import O, backend, scenes
+    rng = np.random.default_rng(9)
+    desc = scenes.planar_disks(rng=rng)
+    sc, o = backend().Scene(desc), O.Scene(desc)
+    q1, q2 = scenes.random_pair_edges(rng, desc, 400)
+    rc, got = _host_edges(sc, q1, q2, 0.01)
+    assert rc == 0 and np.array_equal(got.astype(bool), o.check_edges(q1, q2, 0.01))
+    desc = scenes.mixed_arms(rng=rng, n_modes=64)
+    sc, o = backend().Scene(desc), O.Scene(desc)
+    V, i1, i2, modes, inst = scenes.mixed_instances(rng, desc, 64, 0.05)
+    k = min(len(i1), 1500)
+    rc, got = _host_edges(sc, V[i1[:k]], V[i2[:k]], 0.05, modes=modes[:k])
+    exp = o.check_edges(V[i1[:k]], V[i2[:k]], 0.05, modes=modes[:k])
+    assert rc == 0 and np.array_equal(got.astype(bool), exp)
+
+
+def test_host_list_errors():
+    from helpers import backend, scenes
+    rng = np.random.default_rng(10)
+    desc = scenes.quad_arms(rng=rng)
+    sc = backend().Scene(desc)
+    q = near_home(rng, desc, 4)
+    rc, _ = _host_edges(sc, q, q + 0.1, 0.05, N_start=[0, 0, 50, 0])  # N_start > N
+    assert rc == -1
+    rc, _ = _host_edges(sc, q, q + 0.1, 0.05, N_start=[0, -1, 0, 0])
+    assert rc == -1
+    rc, _ = _host_edges(sc, q, q + 0.1, 0.0)  # no resolution and no N
+    assert rc == -1
+    rc, got = _host_edges(sc, q[:0], q[:0], 0.05)
+    assert rc == 0
+    # N entries <= 0 take the resolution's N
+    rc, a = _host_edges(sc, q, q + 0.3, 0.05, N=[0, 0, 0, 0])
+    rc2, b = _host_edges(sc, q, q + 0.3, 0.05)
+    assert rc == 0 and rc2 == 0 and np.array_equal(a, b)
