@@ -89,6 +89,18 @@ int mrcc_scene_create_arms(int n_arms, int dof, const double* base, const double
                            const int32_t* link_sphere_count, const double* sphere_local,
                            int n_boxes, const double* boxes, mrcc_scene** out);
 
+/* Serial revolute arms of any joint geometry (the robot descriptions the mujoco and pinocchio
+ * backends load, mujoco_env.py / pinocchio_env.py, e.g. assets/models/mujoco/mujoco_ur10/ur10e.xml):
+ * T_k = T_{k-1} Rz(q_k) C_k, with C_k a fixed transform per joint that carries link k's frame
+ * (its joint axis along z) to the next.  joint_fixed: [n_arms,dof,12] = C_k row-major 3x3
+ * rotation and translation, interleaved by rows: (R00 R01 R02 t0, R10 R11 R12 t1, R20 R21 R22 t2).
+ * Spheres are given in the frame after C_k, as for mrcc_scene_create_arms; every other argument
+ * and limit as there.  A DH joint is C = [[1,0,0,a],[0,cos alpha,-sin alpha,0],[0,sin alpha,cos alpha,d]],
+ * and then both constructors give the same transforms. */
+int mrcc_scene_create_arm_chains(int n_arms, int dof, const double* base, const double* joint_fixed,
+                                 const int32_t* link_sphere_count, const double* sphere_local,
+                                 int n_boxes, const double* boxes, mrcc_scene** out);
+
 /* Modes of an arm scene (mode -> scene graph, planning_env.py:1711-1722; attachment semantics
  * rai_base_env.py:801-805, pinocchio_env.py:266-372).  Mode m: arms [0, n_active[m]) exist; arm a
  * holds a box iff has_grasp[m*n_arms+a]; the box pose is T_ee(q) * T_rel with

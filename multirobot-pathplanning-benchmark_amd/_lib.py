@@ -62,6 +62,8 @@ SIGNATURES = {
     "mrcc_scene_create_planar_boxes": (_c.c_int, [_c.c_int, _dp, _c.c_int, _dp, _dp, _c.POINTER(_vp)]),
     "mrcc_scene_create_arms": (_c.c_int, [_c.c_int, _c.c_int, _dp, _dp, _ip, _dp, _c.c_int, _dp,
                                           _c.POINTER(_vp)]),
+    "mrcc_scene_create_arm_chains": (_c.c_int, [_c.c_int, _c.c_int, _dp, _dp, _ip, _dp, _c.c_int, _dp,
+                                                _c.POINTER(_vp)]),
     "mrcc_scene_set_modes": (_c.c_int, [_vp, _c.c_int, _ip, _u8p, _dp, _dp]),
     "mrcc_scene_set_broadphase": (_c.c_int, [_vp, _c.c_int]),
     "mrcc_scene_destroy": (_c.c_int, [_vp]),

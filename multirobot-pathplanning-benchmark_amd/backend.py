@@ -60,8 +60,14 @@ class Scene:
             b, dh, sl, bx = f64(desc.base), f64(desc.dh), f64(desc.sphere_local), f64(desc.boxes)
             cnt = np.ascontiguousarray(desc.link_sphere_count, dtype=np.int32)
             keep = [b, dh, sl, bx, cnt]
-            check(L.mrcc_scene_create_arms(desc.n_arms, desc.dof, dptr(b), dptr(dh), iptr(cnt), dptr(sl),
-                                           len(bx), dptr(bx), ctypes.byref(h)))
+            if desc.joint_fixed is not None:
+                jf = f64(desc.joint_fixed)
+                keep.append(jf)
+                check(L.mrcc_scene_create_arm_chains(desc.n_arms, desc.dof, dptr(b), dptr(jf), iptr(cnt), dptr(sl),
+                                                     len(bx), dptr(bx), ctypes.byref(h)))
+            else:
+                check(L.mrcc_scene_create_arms(desc.n_arms, desc.dof, dptr(b), dptr(dh), iptr(cnt), dptr(sl),
+                                               len(bx), dptr(bx), ctypes.byref(h)))
         else:
             raise ValueError(f"unknown scene kind {desc.kind}")
         del keep

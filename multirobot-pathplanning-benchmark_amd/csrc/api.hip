@@ -4,6 +4,8 @@
 // margins), edge plans (workspace + hipGraph), the per-call entry points and the error contract
 // (status codes, thread-local message, injectable failures).
 #include <math.h>
+
+#include <cmath>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -100,7 +102,9 @@ int upload(SceneRec* sc, Packer& P) {
 // host description of an arm scene, kept so modes can be replaced
 struct ArmHost {
     int na, dof;
-    std::vector<double> base, dh;  // dh: [L*3] d a alpha
+    int is_chain;                  // 1: chain joints (C per joint), 0: DH joints
+    std::vector<double> base, dh;  // dh: [L*3] d a alpha (DH joints)
+    std::vector<double> chain;     // [L*12] C = [R | t] rows of each chain joint
     std::vector<int> counts;
     std::vector<double> sph;       // [S*4]
     std::vector<double> boxes;     // [nb*6] centre half
@@ -123,13 +127,20 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
             const int a = l / dof, k = l % dof;
             sph_lo[t] = (k + 2 < dof) ? link_first[a * dof + k + 2] : link_first[(a + 1) * dof];
         }
-    std::vector<double> dh4(4 * (size_t)L);
+    // joint table of the device: DH joints d a cos(alpha) sin(alpha); chain joints C as given
+    std::vector<double> dh4(H.is_chain ? 12 * (size_t)L : 4 * (size_t)L);
     double B = 0.0;
     for (int a = 0; a < na; ++a) {
         double reach = sqrt(H.base[12 * a + 9] * H.base[12 * a + 9] + H.base[12 * a + 10] * H.base[12 * a + 10] +
                             H.base[12 * a + 11] * H.base[12 * a + 11]);
         for (int k = 0; k < dof; ++k) {
             const int l = a * dof + k;
+            if (H.is_chain) {
+                const double* C = H.chain.data() + 12 * l;
+                for (int i = 0; i < 12; ++i) dh4[12 * l + i] = C[i];
+                reach += sqrt(C[3] * C[3] + C[7] * C[7] + C[11] * C[11]);
+                continue;
+            }
             mrcc::SinCos t = mrcc::sincos_f64(H.dh[3 * l + 2]);
             dh4[4 * l] = H.dh[3 * l];
             dh4[4 * l + 1] = H.dh[3 * l + 1];
@@ -294,6 +305,7 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     void* d = sc->dev_mem;
     ArmView& V = sc->arm;
     V.na = na; V.dof = dof; V.D = na * dof; V.L = L; V.S = S; V.nb = nb; V.nm = H.nm;
+    V.chain = H.is_chain;
     V.broadphase = bp;
     // binary32 margins: squared distances near contact are off by <= ~2^-21 sqrt(3) R B; box SAT
     // axis values by <= ~2^-21 (B + 3 H)
@@ -351,7 +363,7 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     sc->arm_nw = 0;
     int best = 0;
     for (int nw = 4; nw >= 1; nw >>= 1) {
-        const size_t b = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, nw);
+        const size_t b = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, nw);
         if (b > 65536) continue;
         const int resident = (int)(163840 / b) * nw;
         if (resident > best) {
@@ -360,8 +372,8 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
         }
     }
     if (sc->arm_nw == 0) return fail(MRCC_ERR_INVALID, "arm scene needs more than 64 KiB of LDS per wave");
-    sc->arm_lds = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, sc->arm_nw);
-    sc->arm_lds1 = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, 1);
+    sc->arm_lds = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, sc->arm_nw);
+    sc->arm_lds1 = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, 1);
     sc->arm_na = na;
     sc->arm_dof = dof;
     sc->arm_S = S;
@@ -608,25 +620,31 @@ int mrcc_scene_create_planar_boxes(int n_robots, const double* robot_half_extent
     return MRCC_OK;
 }
 
-int mrcc_scene_create_arms(int n_arms, int dof, const double* base, const double* dh,
-                           const int32_t* link_sphere_count, const double* sphere_local,
-                           int n_boxes, const double* boxes, mrcc_scene** out) {
-    CHECK_INJECT();
+static int create_arm_scene(int n_arms, int dof, const double* base, const double* dh, const double* chain,
+                            const int32_t* link_sphere_count, const double* sphere_local, int n_boxes,
+                            const double* boxes, mrcc_scene** out) {
     if (!out) return fail(MRCC_ERR_INVALID, "out is null");
     if (n_arms < 1 || n_arms > MRCC_MAX_ARMS || dof < 1 || dof > MRCC_MAX_DOF)
         return fail(MRCC_ERR_INVALID, "arms: 1..%d arms of 1..%d joints", MRCC_MAX_ARMS, MRCC_MAX_DOF);
     if (n_boxes < 0 || n_boxes > 65535) return fail(MRCC_ERR_INVALID, "arms: 0..65535 boxes");
+    if (!base || (!dh && !chain) || !link_sphere_count || (n_boxes > 0 && !boxes))
+        return fail(MRCC_ERR_INVALID, "null argument");
+    const int L = n_arms * dof;
+    if (chain)
+        for (int i = 0; i < 12 * L; ++i)
+            if (!std::isfinite(chain[i])) return fail(MRCC_ERR_INVALID, "joint transform %d is not finite", i / 12);
     SceneBox* box = new SceneBox();
     SceneRec* sc = &box->rec;
     memset(sc, 0, sizeof(SceneRec));
     sc->kind = KIND_ARMS;
     (void)hipGetDevice(&sc->device);
     ArmHost& H = box->arm_host;
-    const int L = n_arms * dof;
     H.na = n_arms;
     H.dof = dof;
+    H.is_chain = chain ? 1 : 0;
     H.base.assign(base, base + 12 * n_arms);
-    H.dh.assign(dh, dh + 3 * L);
+    if (chain) H.chain.assign(chain, chain + 12 * L);
+    else H.dh.assign(dh, dh + 3 * L);
     H.counts.assign(link_sphere_count, link_sphere_count + L);
     int S = 0;
     for (int l = 0; l < L; ++l) {
@@ -651,6 +669,22 @@ int mrcc_scene_create_arms(int n_arms, int dof, const double* base, const double
     }
     *out = (mrcc_scene*)box;
     return MRCC_OK;
+}
+
+int mrcc_scene_create_arms(int n_arms, int dof, const double* base, const double* dh,
+                           const int32_t* link_sphere_count, const double* sphere_local,
+                           int n_boxes, const double* boxes, mrcc_scene** out) {
+    CHECK_INJECT();
+    return create_arm_scene(n_arms, dof, base, dh, nullptr, link_sphere_count, sphere_local, n_boxes, boxes, out);
+}
+
+int mrcc_scene_create_arm_chains(int n_arms, int dof, const double* base, const double* joint_fixed,
+                                 const int32_t* link_sphere_count, const double* sphere_local,
+                                 int n_boxes, const double* boxes, mrcc_scene** out) {
+    CHECK_INJECT();
+    if (!joint_fixed) return fail(MRCC_ERR_INVALID, "null argument");
+    return create_arm_scene(n_arms, dof, base, nullptr, joint_fixed, link_sphere_count, sphere_local, n_boxes,
+                            boxes, out);
 }
 
 int mrcc_scene_set_modes(mrcc_scene* scene, int n_modes, const int32_t* n_active,

@@ -46,10 +46,23 @@ __device__ inline void clr_fk(const ArmView& V, const ClrLds& W, const float* q)
         for (int k = 0; k < 12; ++k) X[k] = (float)V.base[12 * lane + k];
         for (int k = 0; k < V.dof; ++k) {
             const int l = lane * V.dof + k;
-            const double* p = V.dh + 4 * l;
-            const float d = (float)p[0], a = (float)p[1], ca = (float)p[2], sa = (float)p[3];
             const float cs = W.cs[l], sn = W.sn[l];
-            const float A[3][4] = {{cs, -sn * ca, sn * sa, a * cs}, {sn, cs * ca, -cs * sa, a * sn}, {0.f, sa, ca, d}};
+            float A[3][4];
+            if (V.chain) {  // A = Rz(q) C
+                const double* C = V.dh + 12 * l;
+                for (int c = 0; c < 4; ++c) {
+                    const float c0 = (float)C[c], c1 = (float)C[4 + c];
+                    A[0][c] = cs * c0 - sn * c1;
+                    A[1][c] = sn * c0 + cs * c1;
+                    A[2][c] = (float)C[8 + c];
+                }
+            } else {
+                const double* p = V.dh + 4 * l;
+                const float d = (float)p[0], a = (float)p[1], ca = (float)p[2], sa = (float)p[3];
+                A[0][0] = cs; A[0][1] = -sn * ca; A[0][2] = sn * sa; A[0][3] = a * cs;
+                A[1][0] = sn; A[1][1] = cs * ca; A[1][2] = -cs * sa; A[1][3] = a * sn;
+                A[2][0] = 0.f; A[2][1] = sa; A[2][2] = ca; A[2][3] = d;
+            }
             float Y[12];
 #pragma unroll
             for (int r = 0; r < 3; ++r) {

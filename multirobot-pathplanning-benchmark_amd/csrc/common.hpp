@@ -112,6 +112,44 @@ __host__ __device__ inline void dh_row_f64(double& R0, double& R1, double& R2, d
     t = ty;
 }
 
+// Chain joints: X <- X * Rz(q) C, C a fixed 3 x 4 transform [R | t] (rows C0, C1, C2).  The step
+// A = Rz(q) C has rows A0c = cs C0c - sn C1c, A1c = sn C0c + cs C1c, A2c = C2c; each entry of
+// X's row r is (R0 A0c + R1 A1c) + R2 A2c (+ t for c = 3), as dh_step_f64.  For C of a DH joint,
+// [[1, 0, 0, a], [0, ca, -sa, 0], [0, sa, ca, d]], the zeros make every A entry equal to
+// dh_step_f64's (up to the sign of a zero).
+struct ChainCoef {
+    double a0, a1;
+};
+__host__ __device__ inline ChainCoef chain_coef_f64(double cs, double sn, double c0, double c1) {
+#pragma clang fp contract(off)
+    ChainCoef k;
+    k.a0 = cs * c0 - sn * c1;
+    k.a1 = sn * c0 + cs * c1;
+    return k;
+}
+
+__host__ __device__ inline void chain_step_f64(double* X, const double* C, SinCos j) {
+#pragma clang fp contract(off)
+    double A[3][4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const ChainCoef k = chain_coef_f64(j.c, j.s, C[c], C[4 + c]);
+        A[0][c] = k.a0;
+        A[1][c] = k.a1;
+        A[2][c] = C[8 + c];
+    }
+    double Y[12];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const double R0 = X[3 * r], R1 = X[3 * r + 1], R2 = X[3 * r + 2];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Y[3 * r + c] = (R0 * A[0][c] + R1 * A[1][c]) + R2 * A[2][c];
+        Y[9 + r] = ((R0 * A[0][3] + R1 * A[1][3]) + R2 * A[2][3]) + X[9 + r];
+    }
+#pragma unroll
+    for (int k = 0; k < 12; ++k) X[k] = Y[k];
+}
+
 // one row of G = X * Rel (compose_f64)
 __host__ __device__ inline void compose_row_f64(double R0, double R1, double R2, double t,
                                                 const double* Rel, double* g) {
@@ -359,7 +397,8 @@ struct ArmView {
     float Msph;   // binary32 margin of sphere tests (squared-distance units)
     float Mobb;   // binary32 margin of box SAT axis values
     const double* base;      // [na*12]
-    const double* dh;        // [L*4] d a cos(alpha) sin(alpha)
+    int chain;               // 1: joints are chain joints (dh holds C, 12 per joint)
+    const double* dh;        // [L*4] d a cos(alpha) sin(alpha), or [L*12] C of each chain joint
     const int* link_first;   // [L+1]
     const int* sph_link;     // [S] global link index
     const int* sph_lo;       // [S] first allowed partner sphere (same arm, links >= link+2)

@@ -327,10 +327,10 @@ struct ArmStaticLayout {
 __host__ __device__ inline int align16(int x) { return (x + 15) & ~15; }
 
 __host__ __device__ inline ArmStaticLayout arm_static_layout(int L, int na, int nb, int S, int NP, int nm,
-                                                           int nclu) {
+                                                           int nclu, int chain) {
     ArmStaticLayout o;
     int b = 0;
-    o.o_dh = b;      b += 8 * 4 * L;
+    o.o_dh = b;      b += 8 * (chain ? 12 : 4) * L;
     o.o_base = b;    b += 8 * 12 * na;
     o.o_sphL = b;    b += 8 * 3 * S;
     o.o_linkBC = b;  b += 8 * 3 * L;
@@ -365,8 +365,9 @@ __host__ __device__ inline int arm_wave_bytes(int D, int L, int na, int S, int g
 
 // dynamic LDS of an arm kernel launched with `nw` waves per workgroup
 __host__ __device__ inline size_t arm_lds_bytes(int D, int L, int na, int nb, int S, int NP, int nm,
-                                                int nclu, int grasp, int nw) {
-    return (size_t)arm_static_layout(L, na, nb, S, NP, nm, nclu).bytes + (size_t)nw * arm_wave_bytes(D, L, na, S, grasp);
+                                                int nclu, int grasp, int chain, int nw) {
+    return (size_t)arm_static_layout(L, na, nb, S, NP, nm, nclu, chain).bytes +
+           (size_t)nw * arm_wave_bytes(D, L, na, S, grasp);
 }
 
 struct ArmLds {
@@ -384,7 +385,8 @@ struct ArmLds {
     float4* abox;   // [2 na] box (lo, hi) around each arm's link spheres (shares memory with lst)
     float* G32;
     // per workgroup (the scene)
-    const double* dh;      // [L*4] d a cos(alpha) sin(alpha)
+    const double* dh;      // joint table: [L*4] d a cos(alpha) sin(alpha) (DH), or [L*12] the fixed
+                           // transform C (3 x 4, rows) after Rz(q) of each joint (chains)
     const double* base;    // [na*12]
     const double* sphL;    // [S*3] sphere centres in the link frame
     const double* linkBC;  // [L*3] bounding-sphere centres in the link frame
@@ -408,7 +410,7 @@ struct ArmCount {
 __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
     ArmLds W;
     unsigned char* b = (unsigned char*)base;
-    const ArmStaticLayout o = arm_static_layout(V.L, V.na, V.nb, V.S, V.NP, V.nm, V.nclu);
+    const ArmStaticLayout o = arm_static_layout(V.L, V.na, V.nb, V.S, V.NP, V.nm, V.nclu, V.chain);
     W.dh = (const double*)(b + o.o_dh);
     W.base = (const double*)(b + o.o_base);
     W.sphL = (const double*)(b + o.o_sphL);
@@ -445,10 +447,10 @@ __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
 // Stages the scene into the static LDS region (all threads of the workgroup, then a barrier).
 __device__ inline void arm_stage_scene(void* base, const ArmView& V) {
     unsigned char* b = (unsigned char*)base;
-    const ArmStaticLayout o = arm_static_layout(V.L, V.na, V.nb, V.S, V.NP, V.nm, V.nclu);
+    const ArmStaticLayout o = arm_static_layout(V.L, V.na, V.nb, V.S, V.NP, V.nm, V.nclu, V.chain);
     const int t = threadIdx.x, nt = blockDim.x;
     double* dh = (double*)(b + o.o_dh);
-    for (int i = t; i < 4 * V.L; i += nt) dh[i] = V.dh[i];
+    for (int i = t; i < (V.chain ? 12 : 4) * V.L; i += nt) dh[i] = V.dh[i];
     double* base_ = (double*)(b + o.o_base);
     for (int i = t; i < 12 * V.na; i += nt) base_[i] = V.base[i];
     double* sphL = (double*)(b + o.o_sphL);
@@ -640,8 +642,9 @@ __device__ inline float box_d2(float4 c, float4 lo, float4 hi) {
 // Evaluates the state whose binary64 joint vector is distributed over the lanes (lane k < D
 // holds coordinate k in `qv`).  All 64 lanes of the wave must call.  Returns the wave-uniform
 // result.
-// GRASP = false instantiates the evaluator for scenes without held boxes (fewer registers).
-template <bool GRASP>
+// GRASP = false instantiates the evaluator for scenes without held boxes (fewer registers);
+// CHAIN selects chain joints (V.chain = 1) over DH joints at compile time.
+template <bool GRASP, bool CHAIN>
 __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, const ArmRegs& R, double qv,
                                           int mode, unsigned mask, unsigned& rechecks, ArmCount& cnt) {
     const int lane = wave_lane();
@@ -649,19 +652,31 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
     const int na = arm_mact(V, W, mode);
     const int dof = V.dof;
     const int Dact = na * dof;
-    // 1. joint sines / cosines (binary64), finiteness; lane l also forms joint l's coefficients
-    //    of the DH step (dh_row_f64's products): [cs, sn, A01, A11, A02, A12, A03, A13]
+    // 1. joint sines / cosines (binary64), finiteness; lane l also forms rows 0 and 1 of joint l's
+    //    step A = Rz(q) C, column by column: [A00, A10, A01, A11, A02, A12, A03, A13].  DH joints:
+    //    dh_row_f64's products; chain joints: chain_coef_f64 (A0c = cs C0c - sn C1c,
+    //    A1c = sn C0c + cs C1c), which gives the DH products exactly for C of a DH joint.
     bool bad = false;
     if (lane < Dact) {
         bad = !__builtin_isfinite(qv);
         const SinCos t = sincos_f64(qv);
-        const double* p = W.dh + 4 * lane;  // d a cos(alpha) sin(alpha)
-        const double cs = t.c, sn = t.s, a = p[1], ca = p[2], sa = p[3];
+        const double cs = t.c, sn = t.s;
         double2* f = (double2*)(W.fkc + 8 * lane);
-        f[0] = make_double2(cs, sn);
-        f[1] = make_double2(-(sn * ca), cs * ca);
-        f[2] = make_double2(sn * sa, -(cs * sa));
-        f[3] = make_double2(a * cs, a * sn);
+        if (CHAIN) {
+            const double* C = W.dh + 12 * lane;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const ChainCoef k = chain_coef_f64(cs, sn, C[c], C[4 + c]);
+                f[c] = make_double2(k.a0, k.a1);
+            }
+        } else {
+            const double* p = W.dh + 4 * lane;  // d a cos(alpha) sin(alpha)
+            const double a = p[1], ca = p[2], sa = p[3];
+            f[0] = make_double2(cs, sn);
+            f[1] = make_double2(-(sn * ca), cs * ca);
+            f[2] = make_double2(sn * sa, -(cs * sa));
+            f[3] = make_double2(a * cs, a * sn);
+        }
     }
     if (__any(bad)) return true;  // non-finite coordinates are reported as collisions
     wave_sync();
@@ -674,6 +689,8 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
     {
         const int c = lane & 3, rr = (lane >> 2) % 3, ag = lane / 12;
         const int k2i = c == 1 ? 3 : (c == 2 ? 2 : 0);  // sa, ca, d in the DH table
+        // k2 = A2c: 0, sin(alpha), cos(alpha), d of a DH joint; C2c of a chain joint
+        const bool chain = CHAIN;
         for (int a0 = 0; a0 < na; a0 += 5) {
             const int a = a0 + ag;
             if (ag < 5 && a < na) {
@@ -682,13 +699,13 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                 const int l0 = a * dof;
                 // joint k's coefficients are loaded one step ahead, off the chain's critical path
                 double2 kk = ((const double2*)(W.fkc + 8 * l0))[c];
-                double k2 = c == 0 ? 0.0 : W.dh[4 * l0 + k2i];
+                double k2 = chain ? W.dh[12 * l0 + 8 + c] : (c == 0 ? 0.0 : W.dh[4 * l0 + k2i]);
                 for (int k = 0; k < dof; ++k) {
                     const int l = l0 + k;
                     const double k0 = kk.x, k1 = kk.y, kz = k2;
                     if (k + 1 < dof) {
                         kk = ((const double2*)(W.fkc + 8 * (l + 1)))[c];
-                        k2 = c == 0 ? 0.0 : W.dh[4 * (l + 1) + k2i];
+                        k2 = chain ? W.dh[12 * (l + 1) + 8 + c] : (c == 0 ? 0.0 : W.dh[4 * (l + 1) + k2i]);
                     }
                     double y;
                     {

@@ -530,7 +530,7 @@ __device__ inline void arm_stats(unsigned long long* stats, unsigned rechecks, c
 // loads of all slots overlap; the wave then evaluates the live states one after another.  The
 // ticket of the next chunk is requested before those evaluations and read after them, so the
 // atomic's round trip overlaps the work.
-template <bool GRASP>
+template <bool GRASP, bool CHAIN>
 __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView& B, const PlanWork& W,
                                            int lgG, int spt) {
     extern __shared__ double lds_arm[];
@@ -654,7 +654,7 @@ __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView
             }
             const double qv = interp_dir_f64(aj, dj, ij);  // lanes >= D: unused
             ++n_eval;
-            if (arm_state_collides<GRASP>(V, Wl, R, qv, mj, B.mask, rechecks, cnt)) {
+            if (arm_state_collides<GRASP, CHAIN>(V, Wl, R, qv, mj, B.mask, rechecks, cnt)) {
                 if (lane == 0) B.out[ej] = 0;
                 last_dead = ej;
             }
@@ -675,20 +675,31 @@ __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView
     arm_stats(W.stats, rechecks, cnt);
 }
 
-// Scenes without held boxes fit 4 waves per SIMD (<= 128 VGPRs); the grasp variant 3.
+// Scenes without held boxes fit 4 waves per SIMD (<= 128 VGPRs); the grasp variant 3.  DH joints
+// (k_arm_persistent, _grasp) and chain joints (_chain, _chain_grasp) are separate instantiations.
 __global__ void __launch_bounds__(256, 4) k_arm_persistent(ArmView V, EdgeBatchView B, PlanWork W, int lgG,
                                                           int spt) {
-    arm_persistent_body<false>(V, B, W, lgG, spt);
+    arm_persistent_body<false, false>(V, B, W, lgG, spt);
 }
 
 __global__ void __launch_bounds__(256, 3) k_arm_persistent_grasp(ArmView V, EdgeBatchView B, PlanWork W,
                                                                 int lgG, int spt) {
-    arm_persistent_body<true>(V, B, W, lgG, spt);
+    arm_persistent_body<true, false>(V, B, W, lgG, spt);
+}
+
+__global__ void __launch_bounds__(256, 4) k_arm_persistent_chain(ArmView V, EdgeBatchView B, PlanWork W,
+                                                                int lgG, int spt) {
+    arm_persistent_body<false, true>(V, B, W, lgG, spt);
+}
+
+__global__ void __launch_bounds__(256, 3) k_arm_persistent_chain_grasp(ArmView V, EdgeBatchView B, PlanWork W,
+                                                                      int lgG, int spt) {
+    arm_persistent_body<true, true>(V, B, W, lgG, spt);
 }
 
 // Work item k of n_list * wpe: edge list[k / wpe] (or k / wpe), states k % wpe, k % wpe + wpe, ...
 // With init_out (wpe == 1 only) the wave owns its edge: it writes the verdict without reading it.
-template <bool GRASP>
+template <bool GRASP, bool CHAIN>
 __global__ void __launch_bounds__(256) k_arm_direct(ArmView V, EdgeBatchView B, const int* list,
                                                     long long n_list, const long long* n_dev,
                                                     int wpe, int init_out,
@@ -725,7 +736,7 @@ __global__ void __launch_bounds__(256) k_arm_direct(ArmView V, EdgeBatchView B, 
             if (i < 0) continue;
             const double qv = arm_load_state(B, e, N, i, arm_mact(V, Wl, md) * V.dof);
             ++n_eval;
-            if (arm_state_collides<GRASP>(V, Wl, R, qv, mode, B.mask, rechecks, cnt)) {
+            if (arm_state_collides<GRASP, CHAIN>(V, Wl, R, qv, mode, B.mask, rechecks, cnt)) {
                 hit = true;
                 break;
             }
@@ -739,7 +750,7 @@ __global__ void __launch_bounds__(256) k_arm_direct(ArmView V, EdgeBatchView B, 
     arm_stats(stats, rechecks, cnt);
 }
 
-template <bool GRASP>
+template <bool GRASP, bool CHAIN>
 __global__ void __launch_bounds__(256, 3) k_arm_configs(ArmView V, const double* qv, long long n,
                                                      long long rs, long long ds, const int* modes,
                                                      unsigned mask, unsigned char* out) {
@@ -757,7 +768,7 @@ __global__ void __launch_bounds__(256, 3) k_arm_configs(ArmView V, const double*
         const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
         const int Dact = arm_mact(V, Wl, md) * V.dof;
         const double q = lane < Dact ? qv[r * rs + (long long)lane * ds] : 0.0;
-        const bool c = arm_state_collides<GRASP>(V, Wl, R, q, mode, mask, rechecks, cnt);
+        const bool c = arm_state_collides<GRASP, CHAIN>(V, Wl, R, q, mode, mask, rechecks, cnt);
         if (lane == 0) out[r] = c ? 0 : 1;
     }
 }
@@ -773,9 +784,13 @@ __global__ void k_arm_fk(ArmView V, const double* qv, long long n, long long rs,
     for (int k = 0; k < 12; ++k) X[k] = V.base[12 * a + k];
     for (int k = 0; k < V.dof; ++k) {
         const int l = a * V.dof + k;
-        const double* p = V.dh + 4 * l;
         SinCos j = sincos_f64(qv[r * rs + (long long)l * ds]);
-        dh_step_f64(X, p[0], p[1], p[2], p[3], j);
+        if (V.chain) {
+            chain_step_f64(X, V.dh + 12 * l, j);
+        } else {
+            const double* p = V.dh + 4 * l;
+            dh_step_f64(X, p[0], p[1], p[2], p[3], j);
+        }
         for (int c = 0; c < 12; ++c) out[(r * V.L + l) * 12 + c] = X[c];
     }
 }
@@ -833,10 +848,16 @@ void launch_plan_run(const SceneRec& sc, const EdgeBatchView& B, const PlanWork&
             // grid counts waves; workgroups of arm_nw waves
             const int nw = sc.arm_nw;
             const int blocks = (grid + nw - 1) / nw;
-            if (sc.arm_has_grasp)
+            if (sc.arm.chain) {
+                if (sc.arm_has_grasp)
+                    k_arm_persistent_chain_grasp<<<blocks, 64 * nw, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
+                else
+                    k_arm_persistent_chain<<<blocks, 64 * nw, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
+            } else if (sc.arm_has_grasp) {
                 k_arm_persistent_grasp<<<blocks, 64 * nw, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
-            else
+            } else {
                 k_arm_persistent<<<blocks, 64 * nw, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
+            }
             break;
         }
     }
@@ -876,10 +897,16 @@ void launch_direct(const SceneRec& sc, const EdgeBatchView& B, const int* list, 
             const int nw = grid < sc.arm_nw ? 1 : sc.arm_nw;
             const size_t lds = nw == 1 ? sc.arm_lds1 : sc.arm_lds;
             const int blocks = (grid + nw - 1) / nw;
-            if (sc.arm_has_grasp)
-                k_arm_direct<true><<<blocks, 64 * nw, lds, st>>>(sc.arm, B, list, n_list, n_dev, wpe, init_out, stats);
-            else
-                k_arm_direct<false><<<blocks, 64 * nw, lds, st>>>(sc.arm, B, list, n_list, n_dev, wpe, init_out, stats);
+#define MRCC_ARM_DIRECT(G, C) k_arm_direct<G, C><<<blocks, 64 * nw, lds, st>>>(sc.arm, B, list, n_list, n_dev, wpe, init_out, stats)
+            if (sc.arm.chain) {
+                if (sc.arm_has_grasp) MRCC_ARM_DIRECT(true, true);
+                else MRCC_ARM_DIRECT(false, true);
+            } else if (sc.arm_has_grasp) {
+                MRCC_ARM_DIRECT(true, false);
+            } else {
+                MRCC_ARM_DIRECT(false, false);
+            }
+#undef MRCC_ARM_DIRECT
             break;
         }
     }
@@ -914,10 +941,16 @@ void launch_configs(const SceneRec& sc, const double* q, long long n, long long 
             const int nw = g < sc.arm_nw ? 1 : sc.arm_nw;
             const size_t lds = nw == 1 ? sc.arm_lds1 : sc.arm_lds;
             const int blocks = (int)((g + nw - 1) / nw);
-            if (sc.arm_has_grasp)
-                k_arm_configs<true><<<blocks, 64 * nw, lds, st>>>(sc.arm, q, n, rs, ds, modes, mask, out);
-            else
-                k_arm_configs<false><<<blocks, 64 * nw, lds, st>>>(sc.arm, q, n, rs, ds, modes, mask, out);
+#define MRCC_ARM_CONFIGS(G, C) k_arm_configs<G, C><<<blocks, 64 * nw, lds, st>>>(sc.arm, q, n, rs, ds, modes, mask, out)
+            if (sc.arm.chain) {
+                if (sc.arm_has_grasp) MRCC_ARM_CONFIGS(true, true);
+                else MRCC_ARM_CONFIGS(false, true);
+            } else if (sc.arm_has_grasp) {
+                MRCC_ARM_CONFIGS(true, false);
+            } else {
+                MRCC_ARM_CONFIGS(false, false);
+            }
+#undef MRCC_ARM_CONFIGS
             break;
         }
     }

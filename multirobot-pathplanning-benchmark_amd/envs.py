@@ -5,8 +5,14 @@
   resolution 0.01 and tolerance 0.01.
 * ``mrcc.planar_disks`` (C1), ``mrcc.planar_boxes`` (C2), ``mrcc.quad_arms`` (C3),
   ``mrcc.mixed_arms`` (C5) — the benchmark scenes of this engine (seed 0).
+* ``mujoco.four_ur10`` — the reference's four-UR10e mujoco world (mujoco_env.py:1160-1204):
+  robots, start pose, resolution 0.01 and tolerance 0.01, on the spherized model that
+  ``mjcf.load_mjcf_arms`` builds from the world's MJCF file.  The file is not part of this
+  package: pass its path, or set ``MRCC_MJCF_FOUR_UR10``.
 """
 from __future__ import annotations
+
+import os
 
 import numpy as np
 
@@ -60,3 +66,21 @@ class QuadArmsEnv(MrccEnvironment):
 class MixedArmsEnv(MrccEnvironment):
     def __init__(self, seed: int = 0, n_modes: int = 64):
         super().__init__(scenes.mixed_arms(seed, n_modes=n_modes), collision_resolution=0.05)
+
+
+@register("mujoco.four_ur10")
+class FourUR10Env(MrccEnvironment):
+    """mujoco_env.py:1160-1204: four UR10e arms (6 joints each) around a box on a table."""
+
+    def __init__(self, path: str = None, spheres_per_link: int = 8):
+        from .mjcf import load_mjcf_arms
+
+        path = path or os.environ.get("MRCC_MJCF_FOUR_UR10")
+        if not path or not os.path.exists(path):
+            raise FileNotFoundError("mujoco.four_ur10 needs the MJCF world mujoco_4_ur10_world_closer.xml "
+                                    "(argument `path` or MRCC_MJCF_FOUR_UR10)")
+        desc = load_mjcf_arms(path, tolerance=0.01, spheres_per_link=spheres_per_link)
+        home = np.array([0, -2, 1.0, -1.0, -1.57, 1.0])
+        start = NpConfiguration.from_list([home for _ in range(4)])
+        super().__init__(desc, robots=["ur10_1", "ur10_2", "ur10_3", "ur10_4"], start_pos=start,
+                         collision_resolution=0.01, collision_tolerance=0.01)

@@ -46,6 +46,9 @@ class SceneDesc:
     sphere_local: np.ndarray = field(default_factory=lambda: np.zeros((0, 4)))
     boxes: np.ndarray = field(default_factory=lambda: np.zeros((0, 6)))
     modes: Optional[dict] = None               # n_active, has_grasp, grasp_rel, grasp_half
+    # arms of any joint geometry: [n_arms, dof, 12] fixed transform C after Rz(q) of each joint
+    # (rows R00 R01 R02 t0, R10 R11 R12 t1, R20 R21 R22 t2); when set, dh is unused
+    joint_fixed: Optional[np.ndarray] = None
 
     @property
     def dim(self) -> int:

@@ -36,6 +36,8 @@ struct or_scene {
     int na, dof, S, nb;
     double base[MAX_ARMS][12];
     double dh[MAX_ARMS][MAX_DOF][5]; /* d a cos(alpha) sin(alpha) alpha */
+    int is_chain;           /* 1: joints are T <- T Rz(q) C with C below */
+    double chain[MAX_ARMS][MAX_DOF][12]; /* C rows: R00 R01 R02 t0, R10 .. t1, R20 .. t2 */
     int* sph_link;          /* [S] global link index arm*dof+link */
     double* sph;            /* [S,4] local xyz r */
     int link_first[MAX_ARMS * MAX_DOF + 1];
@@ -371,6 +373,21 @@ or_scene* or_scene_arms(int n_arms, int dof, const double* base, const double* d
     return s;
 }
 
+or_scene* or_scene_arm_chains(int n_arms, int dof, const double* base, const double* joint_fixed,
+                              const int32_t* link_sphere_count, const double* sphere_local,
+                              int n_boxes, const double* boxes) {
+    if (n_arms < 1 || n_arms > MAX_ARMS || dof < 1 || dof > MAX_DOF) return NULL;
+    /* the DH slots are unused: build with alpha = 0, then switch the joints to C */
+    double* dh0 = (double*)calloc(3 * (size_t)n_arms * dof, sizeof(double));
+    or_scene* s = or_scene_arms(n_arms, dof, base, dh0, link_sphere_count, sphere_local, n_boxes, boxes);
+    free(dh0);
+    if (!s) return NULL;
+    s->is_chain = 1;
+    for (int a = 0; a < n_arms; ++a)
+        for (int k = 0; k < dof; ++k) memcpy(s->chain[a][k], joint_fixed + 12 * (a * dof + k), sizeof(double) * 12);
+    return s;
+}
+
 int or_scene_set_modes(or_scene* s, int n_modes, const int32_t* n_active,
                        const uint8_t* has_grasp, const double* grasp_rel,
                        const double* grasp_half) {
@@ -387,6 +404,27 @@ int or_scene_set_modes(or_scene* s, int n_modes, const int32_t* n_active,
     memcpy(s->m_rel, grasp_rel, sizeof(double) * 12 * nma);
     memcpy(s->m_half, grasp_half, sizeof(double) * 3 * nma);
     return 0;
+}
+
+/* T = T * Rz(q) C for one chain joint: A = Rz(q) C, rows A0c = cs C0c - sn C1c,
+ * A1c = sn C0c + cs C1c, A2c = C2c, then the entries of T * A as dh_step. */
+static void chain_step(double X[12], const double C[12], double q) {
+    double sn, cs;
+    or_sincos(q, &sn, &cs);
+    double A[3][4];
+    for (int c = 0; c < 4; ++c) {
+        A[0][c] = cs * C[c] - sn * C[4 + c];
+        A[1][c] = sn * C[c] + cs * C[4 + c];
+        A[2][c] = C[8 + c];
+    }
+    double Y[12];
+    for (int r = 0; r < 3; ++r) {
+        const double* R = X + 3 * r;
+        for (int c = 0; c < 3; ++c)
+            Y[3 * r + c] = (R[0] * A[0][c] + R[1] * A[1][c]) + R[2] * A[2][c];
+        Y[9 + r] = ((R[0] * A[0][3] + R[1] * A[1][3]) + R[2] * A[2][3]) + X[9 + r];
+    }
+    memcpy(X, Y, sizeof(Y));
 }
 
 /* T = T * A(q) for one DH joint; X = [R row-major | t] */
@@ -414,7 +452,8 @@ int or_arm_fk(const or_scene* s, const double* q, double* out) {
         double X[12];
         memcpy(X, s->base[a], sizeof(X));
         for (int k = 0; k < s->dof; ++k) {
-            dh_step(X, s->dh[a][k], q[a * s->dof + k]);
+            if (s->is_chain) chain_step(X, s->chain[a][k], q[a * s->dof + k]);
+            else dh_step(X, s->dh[a][k], q[a * s->dof + k]);
             memcpy(out + 12 * (a * s->dof + k), X, sizeof(X));
         }
     }

@@ -16,6 +16,8 @@
  * libraries (rai / pinocchio / VAMP; no in-tree implementation):
  *   or_planar_config_free     2D oriented boxes, closed separating-axis test
  *   or_arm_fk / or_arm_config_free   DH arms, link spheres, AABB obstacles, grasped boxes
+ *   or_scene_arm_chains       arms of any joint geometry (T <- T Rz(q) C), as the mujoco /
+ *                             pinocchio robot descriptions give them
  *   or_clearance_cost / _grad        summed squared penetration and its joint gradient
  * and the planners' cost / distance functions beside the path (core/configuration.py):
  *   or_config_cost            :101-219 numba reductions, :303-329 _batch_dist, :437-510 batch_config_cost
@@ -50,6 +52,10 @@ or_scene* or_scene_planar(int n_robots, const double* robot_half, int n_obs,
 or_scene* or_scene_arms(int n_arms, int dof, const double* base, const double* dh,
                         const int32_t* link_sphere_count, const double* sphere_local,
                         int n_boxes, const double* boxes);
+/* arms whose joints are T <- T Rz(q) C, C [dof*n_arms,12] rows (R00 R01 R02 t0, ..., t2) */
+or_scene* or_scene_arm_chains(int n_arms, int dof, const double* base, const double* joint_fixed,
+                              const int32_t* link_sphere_count, const double* sphere_local,
+                              int n_boxes, const double* boxes);
 int or_scene_set_modes(or_scene* s, int n_modes, const int32_t* n_active,
                        const uint8_t* has_grasp, const double* grasp_rel,
                        const double* grasp_half);

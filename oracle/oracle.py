@@ -47,6 +47,8 @@ def lib():
         L.or_scene_planar.restype = ctypes.c_void_p
         L.or_scene_arms.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _ip, _dp, ctypes.c_int, _dp]
         L.or_scene_arms.restype = ctypes.c_void_p
+        L.or_scene_arm_chains.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _ip, _dp, ctypes.c_int, _dp]
+        L.or_scene_arm_chains.restype = ctypes.c_void_p
         L.or_scene_set_modes.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _u8p, _dp, _dp]
         L.or_scene_free.argtypes = [ctypes.c_void_p]
         L.or_scene_dim.argtypes = [ctypes.c_void_p]
@@ -155,7 +157,12 @@ class Scene:
             sl, psl = _d(desc.sphere_local)
             bx, pbx = _d(desc.boxes)
             self._keep += [b, dh, cnt, sl, bx]
-            self.h = L.or_scene_arms(desc.n_arms, desc.dof, pb, pdh, pcnt, psl, len(desc.boxes), pbx)
+            if getattr(desc, "joint_fixed", None) is not None:
+                jf, pjf = _d(desc.joint_fixed)
+                self._keep.append(jf)
+                self.h = L.or_scene_arm_chains(desc.n_arms, desc.dof, pb, pjf, pcnt, psl, len(desc.boxes), pbx)
+            else:
+                self.h = L.or_scene_arms(desc.n_arms, desc.dof, pb, pdh, pcnt, psl, len(desc.boxes), pbx)
             if desc.modes is not None:
                 m = desc.modes
                 na, pna = _i(m["n_active"])
