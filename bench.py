@@ -493,7 +493,62 @@ def secondary_workloads(args, backend, scenes, dev):
     out["c4_clearance"] = clearance_workload(args, backend, scenes, dev, threads)
     out["config_cost"] = cost_workload(args, dev, threads)
     out["rrt_call_pattern"] = call_pattern_workload(dev)
+    ur = ur10_workload(args, backend, scenes, dev, threads)
+    if ur is not None:
+        out["mujoco_four_ur10"] = ur
     return out
+
+
+def ur10_workload(args, backend, scenes, dev, threads):
+    """The reference's four-UR10e mujoco world (mujoco_env.py:1160-1204) on its spherized model
+    (tests/golden/mj_four_ur10.npz, built by mjcf.load_mjcf_arms from the world's MJCF files):
+    65,536 edges around the start pose at the environment's resolution 0.01, batched persistent
+    launch against the CPU oracle."""
+    import torch
+    from oracle import oracle as O
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "mj_four_ur10.npz")
+    if not os.path.exists(path):
+        return None
+    z = np.load(path)
+    n_arms, dof = int(z["n_arms"]), int(z["dof"])
+    desc = scenes.SceneDesc(kind="arms", limits=z["limits"], n_arms=n_arms, dof=dof, base=z["base"],
+                            dh=np.zeros((n_arms, dof, 3)), link_sphere_count=z["link_sphere_count"],
+                            sphere_local=z["sphere_local"], boxes=z["boxes"], joint_fixed=z["joint_fixed"])
+    sc = backend.Scene(desc)
+    rng = np.random.default_rng(7)
+    n, res = 65536, 0.01
+    home = np.tile([0, -2, 1.0, -1.0, -1.57, 1.0], n_arms)
+    q1 = np.clip(home + rng.normal(0, 0.3, (n, desc.dim)), desc.limits[0], desc.limits[1])
+    step = rng.normal(0, 1, (n, desc.dim))
+    step *= (np.exp(rng.normal(np.log(0.1), 0.7, n)) / np.max(np.abs(step), axis=1))[:, None]
+    q2 = np.clip(q1 + step, desc.limits[0], desc.limits[1])
+    n_states, N = logical_states(q1, q2, res)
+    a, b = torch.from_numpy(q1).to(dev), torch.from_numpy(q2).to(dev)
+    o_ = torch.empty(n, dtype=torch.uint8, device=dev)
+    plan = backend.EdgePlan(sc, sc.edge_batch(a, b, o_, res), backend.make_tune(use_graph=1))
+    plan.run()
+    torch.cuda.synchronize()
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plan.run()
+    torch.cuda.synchronize()
+    gpu_rate = n_states / ((time.perf_counter() - t0) / reps)
+    got = o_.cpu().numpy().astype(bool)
+    ent = {"edges": n, "edge_states": n_states, "mean_N": float(np.mean(N)), "gpu_batched": gpu_rate,
+           "unit": "edge-states/s", "free_frac": float(got.mean()), "spheres": int(len(desc.sphere_local))}
+    if not args.skip_cpu:
+        o = O.Scene(desc)
+        k = 8192
+        t0 = time.perf_counter()
+        ref = o.check_edges(q1[:k], q2[:k], res, threads=threads)
+        dt = time.perf_counter() - t0
+        ent["cpu_oracle"] = {"value": int(np.sum(N[:k] - 2)) / dt, "cores": threads, "edges": k}
+        ent["mismatches"] = int(np.sum(ref != got[:k]))
+        ent["checks_compared"] = int(np.sum(N[:k] - 2))
+    log(f"mujoco_four_ur10: {ent}")
+    return ent
 
 
 def call_pattern_workload(dev, iters=200, k=30):
