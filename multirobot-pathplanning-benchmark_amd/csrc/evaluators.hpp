@@ -833,13 +833,21 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             if (cand) W.cand[lanes_below(m)] = y;  // the pair's sphere ranges
             wave_sync();
             const int n = __popcll(m);
+            // the next candidate's entry is read from LDS while this one is tested
+            int yn = W.cand[0];
             for (int c = 0; c < n; ++c) {
-                const int yc = W.cand[c];
+                const int yc = yn;
+                if (c + 1 < n) yn = W.cand[c + 1];
                 const int fl = yc & 511, nl = (yc >> 9) & 15, fm = (yc >> 13) & 511, nm = (yc >> 22) & 15;
                 const int i = fl + (lane >> 3), j = fm + (lane & 7);
+                const bool act = (lane >> 3) < nl && (lane & 7) < nm;
+                float4 ci = make_float4(0.f, 0.f, 0.f, 0.f), cj = ci;
+                if (act) {
+                    ci = W.sph[i];
+                    cj = W.sph[j];
+                }
                 bool band = false;
-                if ((lane >> 3) < nl && (lane & 7) < nm) {
-                    const float4 ci = W.sph[i], cj = W.sph[j];
+                if (act) {
                     const float dx = ci.x - cj.x, dy = ci.y - cj.y, dz = ci.z - cj.z;
                     const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
                     const float Rs = ci.w + cj.w;
