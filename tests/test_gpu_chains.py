@@ -121,3 +121,23 @@ def test_ur10_clearance_matches_oracle(ur10):
         assert np.linalg.norm(g[r] - fd) <= 1e-3 * np.linalg.norm(fd) + 1e-6, r
         checked += 1
     assert checked >= 3
+
+
+@pytest.mark.parametrize("bp", [True, False])
+def test_ur10_modes_and_held_boxes(B, bp):
+    """Chain joints with modes: fewer active arms and boxes held at the end-effector (the grasp
+    instantiation of the chain kernels), configurations and edges against the oracle."""
+    desc, _ = fixture_desc()
+    rng = np.random.default_rng(6)
+    desc.modes = scenes.grasp_modes(rng, 64, desc.n_arms)
+    sc, o = B.Scene(desc), O.Scene(desc)
+    sc.set_broadphase(bp)
+    home = np.tile([0, -2, 1.0, -1.0, -1.57, 1.0], desc.n_arms)
+    Q = np.clip(home + rng.normal(0, 0.4, size=(6000, desc.dim)), desc.limits[0], desc.limits[1])
+    modes = rng.integers(0, 64, 6000).astype(np.int32)
+    ref = o.check_configs(Q, modes=modes)
+    assert 0.05 < ref.mean() < 0.95
+    assert np.array_equal(gpu_configs(sc, Q, modes=modes), ref)
+    q2 = np.clip(Q + rng.normal(0, 0.15, Q.shape), desc.limits[0], desc.limits[1])
+    refe = o.check_edges(Q[:3000], q2[:3000], 0.01, modes=modes[:3000])
+    assert np.array_equal(gpu_edges(sc, Q[:3000], q2[:3000], 0.01, modes=modes[:3000]), refe)
