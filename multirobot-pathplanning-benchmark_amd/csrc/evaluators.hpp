@@ -1001,8 +1001,76 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                 if (__any(hit)) return true;
             }
         }
+    } else if (all_arms) {
+        // Brute force with every arm active.  Blocks of 16 tests only compare against the bounds
+        // (T - M: certain contact; T + M: near contact); a block in which some lane is near contact
+        // without a certain one is tested again with the band bookkeeping (rare).
+        // 4. spheres vs boxes
+        const float4* gbox = (const float4*)V.box32;  // uniform index: scalar loads
+        for (int base = 0; base < S_act; base += 64) {
+            const int s = base + lane;
+            const bool act = s < S_act;
+            const float4 c = W.sph[act ? s : 0];
+            const float T = c.w * c.w;
+            const float Tp = act ? T + M : -1.0f, Tm = act ? T - M : -1.0f;  // -1: never (d2 >= 0)
+            for (int b0 = 0; b0 < V.nb; b0 += 16) {
+                const int b1 = b0 + 16 < V.nb ? b0 + 16 : V.nb;
+                bool near = false;
+#pragma unroll 4
+                for (int b = b0; b < b1; ++b) {
+                    const float d2 = box_d2(c, gbox[2 * b], gbox[2 * b + 1]);
+                    near |= d2 <= Tp;
+                    hit |= d2 < Tm;
+                }
+                if (act) cnt.sb += b1 - b0;
+                if (__any(hit)) return true;
+                if (__any(near)) {
+                    for (int b = b0; b < b1; ++b) {
+                        const float d2 = box_d2(c, gbox[2 * b], gbox[2 * b + 1]);
+                        arm_pend(W, d2 <= Tp, 0, s, b, np);  // no certain contact in this block
+                        if (np > ARM_PEND_MAX - 64 && arm_flush(V, W, np, rechecks)) return true;
+                    }
+                }
+            }
+        }
+        // 5. sphere pairs: same arm links >= 2 apart, and different arms; lane i tests j >= lo(i)
+        for (int base = 0; base < S_act; base += 64) {
+            const int i = base + lane;
+            const bool act = i < S_act;
+            const float4 ci = W.sph[act ? i : 0];
+            const int lo = act ? V.sph_lo[i] : 0x7fffffff;
+            const int jstart = V.sph_lo[base];
+            for (int j0 = jstart; j0 < S_act; j0 += 16) {
+                const int j1 = j0 + 16 < S_act ? j0 + 16 : S_act;
+                bool near = false;
+#pragma unroll 4
+                for (int j = j0; j < j1; ++j) {
+                    const float4 cj = W.sph[j];
+                    const float dx = ci.x - cj.x, dy = ci.y - cj.y, dz = ci.z - cj.z;
+                    const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    const float Rs = ci.w + cj.w;
+                    const float T = Rs * Rs;
+                    const bool valid = j >= lo;
+                    near |= valid && d2 <= T + M;
+                    hit |= valid && d2 < T - M;
+                }
+                if (act && lo < j1) cnt.sp += j1 - (lo > j0 ? lo : j0);
+                if (__any(hit)) return true;
+                if (__any(near)) {
+                    for (int j = j0; j < j1; ++j) {
+                        const float4 cj = W.sph[j];
+                        const float dx = ci.x - cj.x, dy = ci.y - cj.y, dz = ci.z - cj.z;
+                        const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                        const float Rs = ci.w + cj.w;
+                        const float T = Rs * Rs;
+                        arm_pend(W, j >= lo && d2 <= T + M, 1, i, j, np);  // no certain contact here
+                        if (np > ARM_PEND_MAX - 64 && arm_flush(V, W, np, rechecks)) return true;
+                    }
+                }
+            }
+        }
     } else {
-        // 4. spheres vs boxes (brute force)
+        // 4. spheres vs boxes (brute force, some arms masked out)
         for (int base = 0; base < S_act; base += 64) {
             const int s = base + lane;
             const bool act = s < S_act && ((mask >> (W.sphLink[s < S_act ? s : 0] >> 8)) & 1u);
@@ -1022,7 +1090,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             }
             if (__any(hit)) return true;
         }
-        // 5. sphere pairs: same arm links >= 2 apart, and different arms (brute force)
+        // 5. sphere pairs: same arm links >= 2 apart, and different arms (brute force, masked)
         for (int base = 0; base < S_act; base += 64) {
             const int i = base + lane;
             const bool act = i < S_act && ((mask >> (W.sphLink[i < S_act ? i : 0] >> 8)) & 1u);
