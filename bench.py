@@ -287,6 +287,13 @@ def main():
                          f"FP64 oracle, OpenMP over edges, {cpu_t:.1f} s"}
         mism = int(np.sum(ref != gpu_free[:k]))
         checked += ks
+        # the same oracle on one thread, over a 2 s sample of the same edges
+        k1 = max(1, min(k, int(k * 2.0 / (cpu_t / reps) / threads)))
+        t0 = time.perf_counter()
+        o.check_edges(q1[:k1], q2[:k1], res, threads=1)
+        t1 = time.perf_counter() - t0
+        cpu["one_thread"] = {"value": int(np.sum(N[:k1] - 2)) / t1, "cores": 1, "edges": k1}
+        log(f"cpu oracle, one thread: {cpu['one_thread']['value']:.4g} edge-states/s ({k1} edges)")
         # every configuration of the workload
         refc = o.check_configs(Qc, threads=threads)
         mism += int(np.sum(refc != cfg_free))
