@@ -278,9 +278,11 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
         return true;
     };
     int n_pairs = 0;
+    // self pairs by link gap (2, 3, ...): links two apart touch most often, and the wave stops at
+    // the first contact it finds
     for (int a = 0; a < na; ++a) {
-        for (int l = a * dof; l < (a + 1) * dof; ++l)
-            for (int m = l + 2; m < (a + 1) * dof; ++m) n_pairs += add_pair(l, m);
+        for (int gap = 2; gap < dof; ++gap)
+            for (int l = a * dof; l + gap < (a + 1) * dof; ++l) n_pairs += add_pair(l, l + gap);
         lpair_sets[a + 1] = n_pairs;
     }
     for (int b = 1; b < na; ++b)
