@@ -644,9 +644,12 @@ __device__ inline float box_d2(float4 c, float4 lo, float4 hi) {
 // result.
 // GRASP = false instantiates the evaluator for scenes without held boxes (fewer registers);
 // CHAIN selects chain joints (V.chain = 1) over DH joints at compile time.
+// `lazy` (wave-uniform): place the spheres beyond the first 64 only when a candidate pair needs
+// them (pays when most states collide; the caller decides from its recent states).
 template <bool GRASP, bool CHAIN>
 __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, const ArmRegs& R, double qv,
-                                          int mode, unsigned mask, unsigned& rechecks, ArmCount& cnt) {
+                                          int mode, unsigned mask, unsigned& rechecks, ArmCount& cnt,
+                                          bool lazy = false) {
     const int lane = wave_lane();
     if (mode < 0 || mode >= V.nm) mode = 0;
     const int na = arm_mact(V, W, mode);
@@ -736,10 +739,15 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
         }
     }
     wave_sync();
-    // 3. sphere centres and link bounding spheres: binary64, rounded to binary32
+    // 3. sphere centres and link bounding spheres: binary64, rounded to binary32.  Lazily (link
+    //    broad phase), only the first 64 spheres are placed here (they hold every sphere of arm 0:
+    //    at most 8 links of 8); the others are placed when the first candidate pair that reads
+    //    them comes up (arm 0's candidates come first, and most random states collide within one
+    //    arm), or at the end of the self pairs.
     const int lastFC = __builtin_amdgcn_readlane(R.linkFC, Dact - 1);
     const int S_act = (lastFC & 0xffff) + (lastFC >> 16);
-    for (int s = lane; s < S_act; s += 64) {
+    const int S_first = lazy && V.broadphase && S_act > 64 ? 64 : S_act;
+    for (int s = lane; s < S_first; s += 64) {
         double c[3];
         const double* l = W.sphL + 3 * s;
         xform_point_f64(W.X + 12 * (W.sphLink[s] & 255), l[0], l[1], l[2], c);
@@ -767,11 +775,23 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
         //     are self collisions); then the segment of an arm pair, only when the boxes around
         //     the two arms' link spheres are not separated by Marm.
         int p0 = 0, p_end = W.sets[na];
+        bool placed = S_first == S_act;  // every sphere placed
+        auto place_rest = [&]() {
+            placed = true;
+            for (int s = 64 + lane; s < S_act; s += 64) {
+                double c[3];
+                const double* l = W.sphL + 3 * s;
+                xform_point_f64(W.X + 12 * (W.sphLink[s] & 255), l[0], l[1], l[2], c);
+                W.sph[s] = make_float4((float)c[0], (float)c[1], (float)c[2], W.sphR[s]);
+            }
+            wave_sync();
+        };
         int sg = 0;
         bool segs_known = false;
         unsigned long long segs = 0ull;
         for (;;) {
             if (p0 >= p_end) {
+                if (!placed) place_rest();
                 if (!segs_known) {
                     segs_known = true;
                     if (lane < na) {
@@ -833,34 +853,47 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             if (cand) W.cand[lanes_below(m)] = y;  // the pair's sphere ranges
             wave_sync();
             const int n = __popcll(m);
+            // spheres not placed yet: the candidates before the first one that reads spheres beyond
+            // the first 64 are tested first, then those spheres are placed, then the others
+            int c_hi = n;
+            if (!placed) {
+                const unsigned long long h = __ballot(cand && ((y >> 13) & 511) + ((y >> 22) & 15) > 64);
+                if (h) c_hi = __popcll(m & ((h & (0ull - h)) - 1ull));
+            }
             // the next candidate's entry is read from LDS while this one is tested
             int yn = W.cand[0];
-            for (int c = 0; c < n; ++c) {
-                const int yc = yn;
-                if (c + 1 < n) yn = W.cand[c + 1];
-                const int fl = yc & 511, nl = (yc >> 9) & 15, fm = (yc >> 13) & 511, nm = (yc >> 22) & 15;
-                const int i = fl + (lane >> 3), j = fm + (lane & 7);
-                const bool act = (lane >> 3) < nl && (lane & 7) < nm;
-                float4 ci = make_float4(0.f, 0.f, 0.f, 0.f), cj = ci;
-                if (act) {
-                    ci = W.sph[i];
-                    cj = W.sph[j];
-                }
-                bool band = false;
-                if (act) {
-                    const float dx = ci.x - cj.x, dy = ci.y - cj.y, dz = ci.z - cj.z;
-                    const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    const float Rs = ci.w + cj.w;
-                    const float T = Rs * Rs;
-                    ++cnt.sp;
-                    if (d2 <= T + M) {
-                        if (d2 < T - M) hit = true;
-                        else band = true;
+            int c = 0;
+            for (;;) {
+                for (; c < c_hi; ++c) {
+                    const int yc = yn;
+                    if (c + 1 < n) yn = W.cand[c + 1];
+                    const int fl = yc & 511, nl = (yc >> 9) & 15, fm = (yc >> 13) & 511, nm = (yc >> 22) & 15;
+                    const int i = fl + (lane >> 3), j = fm + (lane & 7);
+                    const bool act = (lane >> 3) < nl && (lane & 7) < nm;
+                    float4 ci = make_float4(0.f, 0.f, 0.f, 0.f), cj = ci;
+                    if (act) {
+                        ci = W.sph[i];
+                        cj = W.sph[j];
                     }
+                    bool band = false;
+                    if (act) {
+                        const float dx = ci.x - cj.x, dy = ci.y - cj.y, dz = ci.z - cj.z;
+                        const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                        const float Rs = ci.w + cj.w;
+                        const float T = Rs * Rs;
+                        ++cnt.sp;
+                        if (d2 <= T + M) {
+                            if (d2 < T - M) hit = true;
+                            else band = true;
+                        }
+                    }
+                    arm_pend(W, band, 1, i, j, np);
+                    if (__any(hit)) return true;
+                    if (np > ARM_PEND_MAX - 64 && arm_flush(V, W, np, rechecks)) return true;
                 }
-                arm_pend(W, band, 1, i, j, np);
-                if (__any(hit)) return true;
-                if (np > ARM_PEND_MAX - 64 && arm_flush(V, W, np, rechecks)) return true;
+                if (c_hi == n) break;
+                place_rest();
+                c_hi = n;
             }
             wave_sync();
             if (__any(hit)) return true;

@@ -544,6 +544,8 @@ __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView
     ArmCount cnt = {0, 0, 0, 0};
     int p = 0;  // pass hint (wave-uniform)
     int q = 0;  // queues exhausted so far
+    // outcomes of the wave's last 16 states (1: collision); mostly colliding -> lazy placement
+    unsigned recent = 0xffffu;
     const long long nchunks = (T + spt - 1) / spt;
     const int home = blockIdx.x & 7;
     long long ps_lo = W.pass_start[0], ps_hi = W.pass_start[1];  // pass p's slots (scalar)
@@ -654,7 +656,10 @@ __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView
             }
             const double qv = interp_dir_f64(aj, dj, ij);  // lanes >= D: unused
             ++n_eval;
-            if (arm_state_collides<GRASP, CHAIN>(V, Wl, R, qv, mj, B.mask, rechecks, cnt)) {
+            const bool col = arm_state_collides<GRASP, CHAIN>(V, Wl, R, qv, mj, B.mask, rechecks, cnt,
+                                                              __builtin_popcount(recent & 0xffffu) >= 8);
+            recent = (recent << 1) | (col ? 1u : 0u);
+            if (col) {
                 if (lane == 0) B.out[ej] = 0;
                 last_dead = ej;
             }
