@@ -558,6 +558,7 @@ __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView
         // (the current pass's bounds are in scalar registers), rank -> edge (one load), then every
         // per-edge word in one round of loads
         int e = -1, i = -1, N = 0, mode = 0, Dact = 0;
+        int kl = 0;  // the slot's place among its edge's slots of the pass
         int pl = p;
         bool dead = false;
         if (lane < n_here) {
@@ -571,6 +572,7 @@ __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView
             const long long off = sl - lo;
             const long long r = off >> lgG;
             const int k = (int)(off & ((1 << lgG) - 1));
+            kl = k;
             const int ee = (off >= 0 && r < B.n) ? W.order[r] : -1;
             if (ee >= 0 && ee < B.n) {
                 const int S = W.eS[ee], Mv = W.eM[ee], Nv = W.eN[ee], w0 = W.ew0[ee];
@@ -611,19 +613,37 @@ __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView
         }
         n_slots += n_here;
         n_dead += __popcll(__ballot(dead));
-        // The live slots are evaluated in turn; the inputs of the next one (its edge's row and
-        // step, binary64) are loaded before the current one is evaluated, and the next chunk's
-        // ticket is requested after the first slot's loads, so none of these round trips sits on
-        // the evaluation path.
-        unsigned long long live = __ballot(e >= 0);
+        // The live slots are evaluated in turn, k-major: the first slot of every edge of the chunk,
+        // then the second, ...  Most states of random edges collide, so an edge usually dies at
+        // its first state; in this order the state after it belongs to another edge, and the inputs
+        // loaded for it ahead of time (its edge's row and step, binary64) are used.  The slots of
+        // edges that died in this chunk are skipped.  The next chunk's ticket is requested after
+        // the first slot's loads, so none of these round trips sits on the evaluation path.
+        const int G = 1 << lgG;
+        int kc = 0;
+        unsigned long long cur = __ballot(e >= 0 && kl == 0);
+        unsigned long long deadm = 0;  // lanes whose edge was found in collision in this chunk
+        auto next_slot = [&]() -> int {
+            for (;;) {
+                n_dead += __popcll(cur & deadm);
+                cur &= ~deadm;
+                if (cur) {
+                    const int j = __builtin_ctzll(cur);
+                    cur &= cur - 1ull;
+                    return j;
+                }
+                if (++kc >= G) return -1;
+                cur = __ballot(e >= 0 && kl == kc);
+            }
+        };
+        int jn = next_slot();
         int en = -1, in_ = 0, mn = 0;
         double an = 0.0, dn = 0.0;
-        if (live) {
-            const int j = __builtin_ctzll(live);
-            en = __builtin_amdgcn_readlane(e, j);
-            in_ = __builtin_amdgcn_readlane(i, j);
-            mn = __builtin_amdgcn_readlane(mode, j);
-            const int Dn = __builtin_amdgcn_readlane(Dact, j);
+        if (jn >= 0) {
+            en = __builtin_amdgcn_readlane(e, jn);
+            in_ = __builtin_amdgcn_readlane(i, jn);
+            mn = __builtin_amdgcn_readlane(mode, jn);
+            const int Dn = __builtin_amdgcn_readlane(Dact, jn);
             if (lane < Dn) {
                 an = row1(B, en)[lane * B.ds];
                 dn = W.dir[(long long)en * V.D + lane];
@@ -632,17 +652,15 @@ __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView
         const int x = (home + q) & 7;
         unsigned long long tk = 0;
         if (lane == 0) tk = atomicAdd(W.ticket + 32 * x, 1ull);
-        int last_dead = -1;  // edge found in collision by this wave in this chunk
-        while (live) {
-            const int ej = en, ij = in_, mj = mn;
+        while (jn >= 0) {
+            const int jc = jn, ej = en, ij = in_, mj = mn;
             const double aj = an, dj = dn;
-            live &= live - 1;
-            if (live) {
-                const int j = __builtin_ctzll(live);
-                en = __builtin_amdgcn_readlane(e, j);
-                in_ = __builtin_amdgcn_readlane(i, j);
-                mn = __builtin_amdgcn_readlane(mode, j);
-                const int Dn = __builtin_amdgcn_readlane(Dact, j);
+            jn = next_slot();
+            if (jn >= 0) {
+                en = __builtin_amdgcn_readlane(e, jn);
+                in_ = __builtin_amdgcn_readlane(i, jn);
+                mn = __builtin_amdgcn_readlane(mode, jn);
+                const int Dn = __builtin_amdgcn_readlane(Dact, jn);
                 an = 0.0;
                 dn = 0.0;
                 if (lane < Dn) {
@@ -650,7 +668,7 @@ __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView
                     dn = W.dir[(long long)en * V.D + lane];
                 }
             }
-            if (ej == last_dead) {  // a later state of an edge this chunk already rejected
+            if ((deadm >> jc) & 1ull) {  // picked before its edge was found in collision
                 ++n_dead;
                 continue;
             }
@@ -661,7 +679,7 @@ __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView
             recent = (recent << 1) | (col ? 1u : 0u);
             if (col) {
                 if (lane == 0) B.out[ej] = 0;
-                last_dead = ej;
+                deadm |= __ballot(e == ej);
             }
         }
         const unsigned long long k = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(tk >> 32)) << 32) |
