@@ -549,20 +549,26 @@ __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView
     const long long nchunks = (T + spt - 1) / spt;
     const int home = blockIdx.x & 7;
     long long ps_lo = W.pass_start[0], ps_hi = W.pass_start[1];  // pass p's slots (scalar)
+    // A wave takes `nb` tickets of one counter at a time (chunks c, c + 8, ...; lane l resolves
+    // slot l % spt of the batch's chunk l / spt).  One ticket while the chunks hold states to
+    // evaluate; after a batch without one (the edges of its slots already collided), twice as many
+    // next time, up to 64 slots: such batches cost their round trips whatever their size.
+    const int nb_max = 64 / spt;
+    const int lj = lane / spt, lo_slot = lane - lj * spt;
+    int nb = 1;
     long long c = claim_chunk(W, q, nchunks);
     while (c >= 0) {
-        const long long s0 = c * spt;
-        const long long rem = T - s0;
-        const int n_here = (int)(rem < spt ? rem : spt);  // >= 1: c < nchunks
-        // lane-parallel resolution of the chunk's slots: slot -> (pass, rank, k) by arithmetic
+        const long long cj = c + 8 * (long long)lj;
+        const long long sl = cj * spt + lo_slot;
+        const bool valid = lj < nb && cj < nchunks && sl < T;
+        // lane-parallel resolution of the batch's slots: slot -> (pass, rank, k) by arithmetic
         // (the current pass's bounds are in scalar registers), rank -> edge (one load), then every
         // per-edge word in one round of loads
         int e = -1, i = -1, N = 0, mode = 0, Dact = 0;
         int kl = 0;  // the slot's place among its edge's slots of the pass
         int pl = p;
         bool dead = false;
-        if (lane < n_here) {
-            const long long sl = s0 + lane;
+        if (valid) {
             long long lo = ps_lo;
             if (sl < ps_lo || sl >= ps_hi) {  // another pass: walk from the hint
                 if (W.pass_start[pl] > sl) pl = 0;
@@ -603,15 +609,16 @@ __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView
                 }
             }
         }
+        const unsigned long long vm = __ballot(valid);  // lane 0 is valid: c < nchunks
         {
-            const int pn = __builtin_amdgcn_readlane(pl, n_here - 1);  // the chunk's largest pass
+            const int pn = __builtin_amdgcn_readlane(pl, 63 - __builtin_clzll(vm));  // the batch's largest pass
             if (pn != p) {
                 p = pn;
                 ps_lo = W.pass_start[p];
                 ps_hi = W.pass_start[p + 1];
             }
         }
-        n_slots += n_here;
+        n_slots += __popcll(vm);
         n_dead += __popcll(__ballot(dead));
         // The live slots are evaluated in turn, k-major: the first slot of every edge of the chunk,
         // then the second, ...  Most states of random edges collide, so an edge usually dies at
@@ -649,9 +656,10 @@ __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView
                 dn = W.dir[(long long)en * V.D + lane];
             }
         }
+        const int nb_next = jn >= 0 ? 1 : (2 * nb < nb_max ? 2 * nb : nb_max);
         const int x = (home + q) & 7;
         unsigned long long tk = 0;
-        if (lane == 0) tk = atomicAdd(W.ticket + 32 * x, 1ull);
+        if (lane == 0) tk = atomicAdd(W.ticket + 32 * x, (unsigned long long)nb_next);
         while (jn >= 0) {
             const int jc = jn, ej = en, ij = in_, mj = mn;
             const double aj = an, dj = dn;
@@ -685,9 +693,11 @@ __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView
         const unsigned long long k = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(tk >> 32)) << 32) |
                                      (unsigned)__builtin_amdgcn_readfirstlane((unsigned)tk);
         c = (long long)x + 8 * (long long)k;
+        nb = nb_next;
         if (c >= nchunks) {  // counter x is exhausted: continue on the others
             ++q;
             c = claim_chunk(W, q, nchunks);
+            nb = 1;
         }
     }
     if (W.stats && lane == 0) {
