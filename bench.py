@@ -133,6 +133,8 @@ def main():
                     help="also run one device copy of 256 MiB (calibrates FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--cost-only", action="store_true",
                     help="only the configuration-cost workload (for profiling runs)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline batch: warmup + timed graph replays (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
@@ -200,6 +202,10 @@ def main():
     value = n_states / (wall / args.steps)
     log(f"batched persistent: {ms_step:.3f} ms/step (events), {value:.4g} edge-states/s (wall), "
         f"{gpu_free.mean():.3f} edges free")
+    if args.headline_only:
+        print(json.dumps({"headline_only": True, "ms_per_step": ms_step, "value": value,
+                          "edges_free": float(gpu_free.mean())}), flush=True)
+        return
 
     # stats of one run (separate plan with counters)
     splan = backend.EdgePlan(sc, batch, backend.make_tune(use_graph=0, collect_stats=1))
