@@ -318,6 +318,10 @@ def test_c3_queue_shapes(c3, B, spt, g, w):
     desc, sc, o, rng = c3
     a = near_home(rng, desc, 1500, sigma=0.7)
     b = np.clip(a + rng.normal(0, 0.4, a.shape), desc.limits[0], desc.limits[1])
+    # plus uniform lognormal edges: almost all collide at their first state, so the later passes
+    # hold chunks without live states and waves take several tickets at once
+    u1, u2 = scenes.lognormal_edges(rng, desc, 1000, 0.05)
+    a, b = np.concatenate([a, u1]), np.concatenate([b, u2])
     ref = o.check_edges(a, b, 0.05)
     t = B.make_tune(slots_per_pass=g, slots_per_ticket=spt, waves_per_cu=w, use_graph=0, collect_stats=1)
     assert np.array_equal(gpu_edges(sc, a, b, 0.05, tune=t), ref)
