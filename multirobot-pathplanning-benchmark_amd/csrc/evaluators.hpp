@@ -870,23 +870,18 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                     const int fl = yc & 511, nl = (yc >> 9) & 15, fm = (yc >> 13) & 511, nm = (yc >> 22) & 15;
                     const int i = fl + (lane >> 3), j = fm + (lane & 7);
                     const bool act = (lane >> 3) < nl && (lane & 7) < nm;
-                    float4 ci = make_float4(0.f, 0.f, 0.f, 0.f), cj = ci;
-                    if (act) {
-                        ci = W.sph[i];
-                        cj = W.sph[j];
-                    }
-                    bool band = false;
-                    if (act) {
-                        const float dx = ci.x - cj.x, dy = ci.y - cj.y, dz = ci.z - cj.z;
-                        const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                        const float Rs = ci.w + cj.w;
-                        const float T = Rs * Rs;
-                        ++cnt.sp;
-                        if (d2 <= T + M) {
-                            if (d2 < T - M) hit = true;
-                            else band = true;
-                        }
-                    }
+                    // branch-free: lanes past the pair's spheres read the links' first spheres
+                    // and their outcome is masked (no exec-mask regions in the hot loop)
+                    const float4 ci = W.sph[act ? i : fl], cj = W.sph[act ? j : fm];
+                    const float dx = ci.x - cj.x, dy = ci.y - cj.y, dz = ci.z - cj.z;
+                    const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    const float Rs = ci.w + cj.w;
+                    const float T = Rs * Rs;
+                    cnt.sp += act ? 1u : 0u;
+                    const bool near_c = act && d2 <= T + M;
+                    const bool deep = d2 < T - M;
+                    hit = hit || (near_c && deep);
+                    const bool band = near_c && !deep;
                     arm_pend(W, band, 1, i, j, np);
                     if (__any(hit)) return true;
                     if (np > ARM_PEND_MAX - 64 && arm_flush(V, W, np, rechecks)) return true;
