@@ -832,22 +832,20 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                 p0 = sk & 4095;
                 p_end = p0 + ((sk >> 12) & 255);
             }
+            // branch-free: lanes past the round's pairs read its first pair, outcome masked
             const int pi = p0 + lane;
+            const bool pin = pi < p_end;
+            const int2 pr = arm_pair(V, W, pin ? pi : p0);  // p0 < p_end here
             p0 += 64;
-            bool cand = false;
-            int y = 0;
-            if (pi < p_end) {
-                const int2 pr = arm_pair(V, W, pi);
-                const int l = (pr.x >> 8) & 255, mm = pr.x & 255;
-                y = pr.y;
-                if (all_arms || (((mask >> ((pr.x >> 20) & 15)) & 1u) && ((mask >> ((pr.x >> 16) & 15)) & 1u))) {
-                    const float4 A = W.linkC[l], Bc = W.linkC[mm];
-                    const float dx = A.x - Bc.x, dy = A.y - Bc.y, dz = A.z - Bc.z;
-                    const float Rs = A.w + Bc.w;
-                    cand = fmaf(dz, dz, fmaf(dy, dy, dx * dx)) <= Rs * Rs + Mc;
-                    ++cnt.lp;
-                }
-            }
+            const int l = (pr.x >> 8) & 255, mm = pr.x & 255;
+            const int y = pr.y;
+            const bool inc =
+                pin && (all_arms || (((mask >> ((pr.x >> 20) & 15)) & 1u) && ((mask >> ((pr.x >> 16) & 15)) & 1u)));
+            const float4 A = W.linkC[l], Bc = W.linkC[mm];
+            const float dx = A.x - Bc.x, dy = A.y - Bc.y, dz = A.z - Bc.z;
+            const float Rs = A.w + Bc.w;
+            const bool cand = inc && fmaf(dz, dz, fmaf(dy, dy, dx * dx)) <= Rs * Rs + Mc;
+            cnt.lp += inc ? 1u : 0u;
             const unsigned long long m = __ballot(cand);
             if (m == 0ull) continue;
             if (cand) W.cand[lanes_below(m)] = y;  // the pair's sphere ranges
@@ -932,20 +930,18 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                 const int nA = __popcll(pm);
                 for (int a0 = 0; a0 < nA; a0 += 8) {
                     const int g = lane >> 3, j = lane & 7;
-                    bool bcand = false;
-                    int lb = 0;
-                    if (a0 + g < nA) {
-                        const int code2 = W.cand[a0 + g];
-                        const int l = code2 & 255, c2 = code2 >> 8;
-                        const int first = __float_as_int(W.clu[2 * c2].w), count = __float_as_int(W.clu[2 * c2 + 1].w);
-                        if (j < count) {
-                            const int b = first + j;
-                            const float4 Lc = W.linkC[l];
-                            bcand = box_d2(Lc, W.box[2 * b], W.box[2 * b + 1]) <= Lc.w * Lc.w + Mc;
-                            lb = l | (b << 8);
-                            ++cnt.lb;
-                        }
-                    }
+                    // branch-free: idle lanes read valid entries (the group's first pair, box 0)
+                    // and their outcome is masked
+                    const bool ga = a0 + g < nA;
+                    const int code2 = W.cand[ga ? a0 + g : a0];
+                    const int l = code2 & 255, c2 = code2 >> 8;
+                    const int first = __float_as_int(W.clu[2 * c2].w), count = __float_as_int(W.clu[2 * c2 + 1].w);
+                    const bool act = ga && j < count;
+                    const int b = act ? first + j : 0;
+                    const float4 Lc = W.linkC[l];
+                    const bool bcand = act && box_d2(Lc, W.box[2 * b], W.box[2 * b + 1]) <= Lc.w * Lc.w + Mc;
+                    const int lb = l | (b << 8);
+                    cnt.lb += act ? 1u : 0u;
                     const unsigned long long bm = __ballot(bcand);
                     if (bm == 0ull) continue;
                     if (bcand) W.lbc[lanes_below(bm)] = lb;
@@ -953,25 +949,20 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                     const int nB = __popcll(bm);
                     for (int c0 = 0; c0 < nB; c0 += 8) {
                         const int g2 = lane >> 3, j2 = lane & 7;
-                        bool band = false;
-                        int sph = 0, bb = 0;
-                        if (c0 + g2 < nB) {
-                            const int v = W.lbc[c0 + g2];
-                            const int l = v & 255;
-                            const int fc = W.linkFC[l];
-                            if (j2 < (fc >> 16)) {
-                                bb = v >> 8;
-                                sph = (fc & 0xffff) + j2;
-                                const float4 c = W.sph[sph];
-                                const float d2 = box_d2(c, W.box[2 * bb], W.box[2 * bb + 1]);
-                                const float T = c.w * c.w;
-                                ++cnt.sb;
-                                if (d2 <= T + M) {
-                                    if (d2 < T - M) hit = true;
-                                    else band = true;
-                                }
-                            }
-                        }
+                        const bool gb = c0 + g2 < nB;
+                        const int v = W.lbc[gb ? c0 + g2 : c0];
+                        const int fc = W.linkFC[v & 255];
+                        const bool act = gb && j2 < (fc >> 16);
+                        const int bb = v >> 8;
+                        const int sph = (fc & 0xffff) + (act ? j2 : 0);
+                        const float4 c = W.sph[sph];
+                        const float d2 = box_d2(c, W.box[2 * bb], W.box[2 * bb + 1]);
+                        const float T = c.w * c.w;
+                        cnt.sb += act ? 1u : 0u;
+                        const bool near_c = act && d2 <= T + M;
+                        const bool deep = d2 < T - M;
+                        hit = hit || (near_c && deep);
+                        const bool band = near_c && !deep;
                         arm_pend(W, band, 0, sph, bb, np);
                         if (np > ARM_PEND_MAX - 64 && arm_flush(V, W, np, rechecks)) return true;
                     }
@@ -993,14 +984,11 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             const float Tl = Lc.w * Lc.w;
             for (int b0 = 0; b0 < V.nb; b0 += 64) {
                 const int b = b0 + lane;
-                bool cand = false;
-                if (b < V.nb) {
-                    float4 lo, hi;
-                    lo = W.boxG[2 * b];
-                    hi = W.boxG[2 * b + 1];
-                    cand = box_d2(Lc, lo, hi) <= Tl + Mc;
-                    ++cnt.lb;
-                }
+                const bool bin = b < V.nb;
+                const int bc = bin ? b : 0;
+                const float4 lo = W.boxG[2 * bc], hi = W.boxG[2 * bc + 1];
+                const bool cand = bin && box_d2(Lc, lo, hi) <= Tl + Mc;
+                cnt.lb += bin ? 1u : 0u;
                 const unsigned long long m = __ballot(cand);
                 if (m == 0ull) continue;
                 if (cand) W.cand[lanes_below(m)] = b;
@@ -1008,20 +996,17 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                 const int n = __popcll(m);
                 for (int c0 = 0; c0 < n; c0 += 8) {
                     const int k = c0 + (lane >> 3), j = lane & 7;
-                    bool band = false;
-                    int s = 0, bb = 0;
-                    if (k < n && j < ns) {
-                        bb = W.cand[k];
-                        s = f + j;
-                        const float4 c = W.sph[s];
-                        const float d2 = box_d2(c, W.boxG[2 * bb], W.boxG[2 * bb + 1]);
-                        const float T = c.w * c.w;
-                        ++cnt.sb;
-                        if (d2 <= T + M) {
-                            if (d2 < T - M) hit = true;
-                            else band = true;
-                        }
-                    }
+                    const bool act = k < n && j < ns;
+                    const int bb = W.cand[act ? k : c0];
+                    const int s = f + (act ? j : 0);
+                    const float4 c = W.sph[s];
+                    const float d2 = box_d2(c, W.boxG[2 * bb], W.boxG[2 * bb + 1]);
+                    const float T = c.w * c.w;
+                    cnt.sb += act ? 1u : 0u;
+                    const bool near_c = act && d2 <= T + M;
+                    const bool deep = d2 < T - M;
+                    hit = hit || (near_c && deep);
+                    const bool band = near_c && !deep;
                     arm_pend(W, band, 0, s, bb, np);
                     if (np > ARM_PEND_MAX - 64 && arm_flush(V, W, np, rechecks)) return true;
                 }
@@ -1108,12 +1093,10 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             for (int b = 0; b < V.nb; ++b) {
                 const float d2 = box_d2(c, gbox[2 * b], gbox[2 * b + 1]);
                 cnt.sb += act;
-                bool band = false;
-                if (act && d2 <= T + M) {
-                    if (d2 < T - M) hit = true;
-                    else band = true;
-                }
-                arm_pend(W, band, 0, s, b, np);
+                const bool near_c = act && d2 <= T + M;
+                const bool deep = d2 < T - M;
+                hit = hit || (near_c && deep);
+                arm_pend(W, near_c && !deep, 0, s, b, np);
                 if (np > ARM_PEND_MAX - 64 && arm_flush(V, W, np, rechecks)) return true;
             }
             if (__any(hit)) return true;
@@ -1135,12 +1118,10 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                 const float Rs = ci.w + cj.w;
                 const float T = Rs * Rs;
                 cnt.sp += (j >= lo);
-                bool band = false;
-                if (j >= lo && d2 <= T + M) {
-                    if (d2 < T - M) hit = true;
-                    else band = true;
-                }
-                arm_pend(W, band, 1, i, j, np);
+                const bool near_c = j >= lo && d2 <= T + M;
+                const bool deep = d2 < T - M;
+                hit = hit || (near_c && deep);
+                arm_pend(W, near_c && !deep, 1, i, j, np);
                 if (np > ARM_PEND_MAX - 64 && arm_flush(V, W, np, rechecks)) return true;
                 if (++nj == 16) {
                     nj = 0;
