@@ -525,11 +525,12 @@ __device__ inline void arm_stats(unsigned long long* stats, unsigned rechecks, c
     stat_add(stats, 7, cnt.sp);
 }
 
-// Persistent arm kernel.  A wave claims a chunk of `spt` (<= 64) consecutive slots; lane j
-// resolves slot j of the chunk (slot -> edge, state, mode; edge already dead?) so the dependent
-// loads of all slots overlap; the wave then evaluates the live states one after another.  The
-// ticket of the next chunk is requested before those evaluations and read after them, so the
-// atomic's round trip overlaps the work.
+// Persistent arm kernel.  A wave claims a chunk of `spt` (<= 64) consecutive slots (after chunks
+// without live states, up to 64 / spt chunks of one counter at once); lane j resolves slot j
+// (slot -> edge, state, mode; edge already dead?) so the dependent loads of all slots overlap;
+// the wave then evaluates the live states one after another, k-major.  The ticket of the next
+// chunk is requested before those evaluations and read after them, so the atomic's round trip
+// overlaps the work.
 template <bool GRASP, bool CHAIN>
 __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView& B, const PlanWork& W,
                                            int lgG, int spt) {
