@@ -109,9 +109,14 @@ __device__ inline float clr_pairs(const ArmView& V, const ClrLds& W, float m, bo
             const float ex = ci.x - fminf(fmaxf(ci.x, bx[0]), bx[4]);
             const float ey = ci.y - fminf(fmaxf(ci.y, bx[1]), bx[5]);
             const float ez = ci.z - fminf(fmaxf(ci.z, bx[2]), bx[6]);
-            const float d = sqrtf(ex * ex + ey * ey + ez * ez);
-            const float pen = m + ci.w - d;
-            if (act && pen > 0.f) {
+            const float e2 = ex * ex + ey * ey + ez * ez;
+            const float R = m + ci.w;
+            // the square root only for pairs within reach: d < R implies e2 < R^2 (1 + 2^-21),
+            // so the bound below passes every pair with pen > 0, whose terms are unchanged
+            if (!(act && e2 <= R * R * 1.0001f)) continue;
+            const float d = sqrtf(e2);
+            const float pen = R - d;
+            if (pen > 0.f) {
                 cost += pen * pen;
                 if (grad && d > 0.f) {
                     const float g = -2.f * pen / d;
@@ -126,9 +131,12 @@ __device__ inline float clr_pairs(const ArmView& V, const ClrLds& W, float m, bo
         for (int j = V.sph_lo[base]; j < S; ++j) {
             const float4 cj = W.sph[j];
             const float ex = ci.x - cj.x, ey = ci.y - cj.y, ez = ci.z - cj.z;
-            const float d = sqrtf(ex * ex + ey * ey + ez * ez);
-            const float pen = m + ci.w + cj.w - d;
-            if (j >= lo && pen > 0.f) {
+            const float e2 = ex * ex + ey * ey + ez * ez;
+            const float R = m + ci.w + cj.w;
+            if (!(j >= lo && e2 <= R * R * 1.0001f)) continue;  // as above
+            const float d = sqrtf(e2);
+            const float pen = R - d;
+            if (pen > 0.f) {
                 cost += pen * pen;
                 if (grad && d > 0.f) {
                     const float g = -2.f * pen / d;
