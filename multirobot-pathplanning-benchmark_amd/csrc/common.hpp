@@ -66,11 +66,13 @@ __host__ __device__ inline SinCos sincos_f64(double x) {
     double cs = (1.0 - 0.5 * r2) + (r2 * r2) * pc;
     double qd = kd - 4.0 * __builtin_floor(kd * 0.25);
     int q = (int)qd & 3;
+    // quadrant q: (s, c) = (sn, cs), (cs, -sn), (-sn, -cs), (-cs, sn), as selects (q differs
+    // between lanes: branches would run every quadrant's path under lane masks)
+    const bool odd = (q & 1) != 0;
+    const double s0 = odd ? cs : sn, c0 = odd ? sn : cs;
     SinCos o;
-    if (q == 0) { o.s = sn; o.c = cs; }
-    else if (q == 1) { o.s = cs; o.c = -sn; }
-    else if (q == 2) { o.s = -sn; o.c = -cs; }
-    else { o.s = -cs; o.c = sn; }
+    o.s = (q & 2) ? -s0 : s0;
+    o.c = ((q + 1) & 2) ? -c0 : c0;
     return o;
 }
 
