@@ -133,6 +133,8 @@ def main():
                     help="also run one device copy of 256 MiB (calibrates FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--cost-only", action="store_true",
                     help="only the configuration-cost workload (for profiling runs)")
+    ap.add_argument("--clearance-only", action="store_true",
+                    help="only the clearance cost + gradient workload (for profiling runs)")
     ap.add_argument("--ur10-only", action="store_true",
                     help="only the four-UR10e mujoco-world workload (for profiling runs)")
     ap.add_argument("--headline-only", action="store_true",
@@ -151,6 +153,10 @@ def main():
     assert torch.cuda.is_available(), "bench needs a HIP device"
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
+    if args.clearance_only:
+        ent = clearance_workload(args, backend, scenes, dev, len(os.sched_getaffinity(0)))
+        print(json.dumps({"clearance_only": True, **ent}), flush=True)
+        return
     if args.ur10_only:
         ent = ur10_workload(args, backend, scenes, dev, len(os.sched_getaffinity(0)))
         print(json.dumps({"ur10_only": True, **(ent or {"error": "fixture missing"})}), flush=True)
