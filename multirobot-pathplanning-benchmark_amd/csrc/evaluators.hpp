@@ -799,12 +799,17 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                         float4 hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
 #pragma unroll
                         for (int k = 0; k < MRCC_MAX_DOF; ++k) {
+                            if (k >= dof) break;  // wave-uniform
+                            // selects, not a lane-masked block
                             const int l = lane * dof + k;
-                            if (k < dof && (W.linkFC[l] >> 16) > 0) {
-                                const float4 c = W.linkC[l];
-                                lo.x = fminf(lo.x, c.x - c.w); lo.y = fminf(lo.y, c.y - c.w); lo.z = fminf(lo.z, c.z - c.w);
-                                hi.x = fmaxf(hi.x, c.x + c.w); hi.y = fmaxf(hi.y, c.y + c.w); hi.z = fmaxf(hi.z, c.z + c.w);
-                            }
+                            const float4 c = W.linkC[l];
+                            const bool use = (W.linkFC[l] >> 16) > 0;
+                            lo.x = use ? fminf(lo.x, c.x - c.w) : lo.x;
+                            lo.y = use ? fminf(lo.y, c.y - c.w) : lo.y;
+                            lo.z = use ? fminf(lo.z, c.z - c.w) : lo.z;
+                            hi.x = use ? fmaxf(hi.x, c.x + c.w) : hi.x;
+                            hi.y = use ? fmaxf(hi.y, c.y + c.w) : hi.y;
+                            hi.z = use ? fmaxf(hi.z, c.z + c.w) : hi.z;
                         }
                         W.abox[2 * lane] = lo;
                         W.abox[2 * lane + 1] = hi;
