@@ -105,7 +105,9 @@ int mrcc_scene_create_arm_chains(int n_arms, int dof, const double* base, const 
  * rai_base_env.py:801-805, pinocchio_env.py:266-372).  Mode m: arms [0, n_active[m]) exist; arm a
  * holds a box iff has_grasp[m*n_arms+a]; the box pose is T_ee(q) * T_rel with
  * grasp_rel[(m*n_arms+a)*12 ..] (row-major R then t) and half extents grasp_half[(m*n_arms+a)*3..].
- * A held box does not collide with its own arm.  Mode 0 exists by default (all arms, no grasps). */
+ * A held box does not collide with the spheres of its holding link (the last link of its arm); it
+ * is tested against every other sphere, its own arm's included, the obstacle boxes and the other
+ * held boxes.  Mode 0 exists by default (all arms, no grasps). */
 int mrcc_scene_set_modes(mrcc_scene* scene, int n_modes, const int32_t* n_active,
                          const uint8_t* has_grasp, const double* grasp_rel,
                          const double* grasp_half);

@@ -1164,14 +1164,19 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             }
         }
         if (__any(hit)) return true;
-        // 6b. held box vs spheres of the other arms (in the mask)
+        // 6b. held box vs the spheres of every arm in the mask, except those of the holding link
+        //     (the last link of arm a): the attachment deactivates only the (object, holding
+        //     link) pair (pinocchio_env.py:266-272; rai attach + setContact(-1),
+        //     rai_base_env.py:801-805)
         for (int a = 0; a < na; ++a) {
             if (!((gmask >> a) & 1u)) continue;
             const float* G = W.G32 + 16 * a;
+            const int hold = a * dof + dof - 1;
             for (int base = 0; base < S_act; base += 64) {
                 const int s = base + lane;
-                const int as = s < S_act ? W.sphLink[s] >> 8 : a;
-                const bool act = s < S_act && as != a && ((mask >> as) & 1u);
+                const int sl = s < S_act ? W.sphLink[s] : hold | (a << 8);
+                const int as = sl >> 8;
+                const bool act = s < S_act && (sl & 255) != hold && ((mask >> as) & 1u);
                 bool band = false;
                 if (act) {
                     const float4 c = W.sph[s];

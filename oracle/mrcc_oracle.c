@@ -572,6 +572,15 @@ static int arm_free(const or_scene* s, const double* q, int mode, uint32_t mask)
                 if (obb3_overlap(Bx, s->boxch + 6 * b + 3, G[a], h)) { free_ = 0; break; }
             }
         }
+        /* held box vs the spheres of its own arm, except those of the holding (last) link: the
+         * attachment deactivates only the (object, holding link) pair (pinocchio_env.py:266-272;
+         * rai attach + setContact(-1), rai_base_env.py:801-805) */
+        if (free_ && s->m_grasp[mode * s->na + a]) {
+            const double* h = s->m_half + 3 * (mode * s->na + a);
+            const int s_hold = s->link_first[a * dof + dof - 1];
+            for (int t = s0; t < s_hold; ++t)
+                if (sph_obb(C + 3 * t, s->sph[4 * t + 3], G[a], h)) { free_ = 0; break; }
+        }
     }
     for (int a = 0; a < na && free_; ++a) {
         if (!((mask >> a) & 1u)) continue;
