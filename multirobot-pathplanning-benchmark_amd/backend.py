@@ -104,9 +104,18 @@ class Scene:
 
     # -------------------------------------------------------------------------------------------
     # device-tensor entry points (inputs already resident)
+    def _check_rows(self, q, name="q"):
+        """A float64 HIP tensor [n, D] of this scene (the kernels read D coordinates per row)."""
+        if q.dtype != _torch().float64 or not q.is_cuda or q.dim() != 2 or q.shape[1] != self.D:
+            raise ValueError(f"{name}: need a float64 HIP tensor [n, {self.D}], got {q.dtype} "
+                             f"{tuple(q.shape)} on {q.device}")
+
     def check_configs_device(self, q, out, modes=None, robot_mask: int = ALL_ROBOTS, stream=None):
         """q: float64 device tensor [n, D] (rows contiguous) or any 2-D strided view."""
+        self._check_rows(q)
         n = q.shape[0]
+        if out.shape[0] != n or (modes is not None and modes.shape[0] != n):
+            raise ValueError("out / modes need one entry per configuration")
         check(lib().mrcc_check_configs(self.handle, ctypes.c_void_p(q.data_ptr()), n, q.stride(0), q.stride(1),
                                        ctypes.c_void_p(modes.data_ptr()) if modes is not None else None,
                                        robot_mask, ctypes.c_void_p(out.data_ptr()),
@@ -115,7 +124,17 @@ class Scene:
     def edge_batch(self, q1, q2, out, resolution, N=None, N_start=None, N_max=None,
                    include_endpoints=False, modes=None, robot_mask=ALL_ROBOTS, idx1=None, idx2=None):
         """Builds the C batch descriptor over device tensors (kept alive by the caller)."""
-        assert q1.stride() == q2.stride(), "q1 and q2 must share the layout"
+        self._check_rows(q1, "q1")
+        self._check_rows(q2, "q2")
+        if q1.stride() != q2.stride():
+            raise ValueError("q1 and q2 must share the layout")
+        n = out.shape[0]
+        if idx1 is None and q1.shape[0] < n or idx2 is None and q2.shape[0] < n:
+            raise ValueError("q1 / q2 need a row per edge (or idx1 / idx2)")
+        for t, nm in ((N, "N"), (N_start, "N_start"), (N_max, "N_max"), (modes, "modes"), (idx1, "idx1"),
+                      (idx2, "idx2")):
+            if t is not None and (t.dtype != _torch().int32 or t.shape[0] != n):
+                raise ValueError(f"{nm}: need int32 [n_edges]")
         p = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
         b = EdgeBatch()
         b.q1, b.q2 = p(q1), p(q2)
@@ -216,13 +235,25 @@ class MrccEnvironment(BaseProblem):
         BaseProblem.__init__(self)
 
     # ---- modes ----------------------------------------------------------------------------
-    @staticmethod
-    def mode_id(mode) -> int:
+    def mode_id(self, mode) -> int:
+        """Device mode id of a mode: an int as is; a Mode by its ``mrcc_mode_id``.  A Mode without
+        one is mode 0 only in a scene with a single mode; otherwise ValueError (a wrong scene graph
+        would give wrong answers silently)."""
         if mode is None:
             return 0
         if isinstance(mode, (int, np.integer)):
-            return int(mode)
-        return int(getattr(mode, "mrcc_mode_id", 0))
+            m = int(mode)
+        else:
+            m = getattr(mode, "mrcc_mode_id", None)
+            if m is None:
+                if self.scene.n_modes > 1:
+                    raise ValueError("mode has no device mode id (mrcc_mode_id); register it with "
+                                     "register_mode")
+                return 0
+            m = int(m)
+        if not 0 <= m < self.scene.n_modes:
+            raise ValueError(f"mode id {m} outside [0, {self.scene.n_modes})")
+        return m
 
     def set_to_mode(self, mode: Mode) -> None:
         self._current_mode = mode
@@ -239,10 +270,11 @@ class MrccEnvironment(BaseProblem):
         return m
 
     # ---- per call --------------------------------------------------------------------------
-    @staticmethod
-    def _flat(q) -> np.ndarray:
-        v = q.state() if isinstance(q, Configuration) else q
-        return np.ascontiguousarray(v, dtype=np.float64)
+    def _flat(self, q) -> np.ndarray:
+        v = np.ascontiguousarray(q.state() if isinstance(q, Configuration) else q, dtype=np.float64)
+        if v.shape != (self.scene.D,):
+            raise ValueError(f"configuration of shape {v.shape}, the scene has D = {self.scene.D}")
+        return v
 
     def is_collision_free(self, q: Optional[Configuration], mode: Optional[Mode]) -> bool:
         """abstract_env.py:255-276 contract; ValueError when q is None (:256-257)."""
@@ -286,7 +318,11 @@ class MrccEnvironment(BaseProblem):
         torch = _torch()
         Q = np.ascontiguousarray(np.stack([self._flat(q) for q in qs]) if not isinstance(qs, np.ndarray) else qs,
                                  dtype=np.float64)
+        if Q.ndim != 2 or Q.shape[1] != self.scene.D:
+            raise ValueError(f"configurations: need [n, {self.scene.D}], got {Q.shape}")
         n = Q.shape[0]
+        if modes is not None and len(modes) != n:
+            raise ValueError("modes need one entry per configuration")
         if n == 0:
             return np.zeros(0, dtype=bool)
         qd = torch.from_numpy(Q).cuda()
@@ -308,7 +344,12 @@ class MrccEnvironment(BaseProblem):
         """
         if len(qs) == 0:
             return True
-        modes = None if mode is None or not isinstance(mode, (list, tuple)) else mode
+        if mode is None:
+            modes = None
+        elif isinstance(mode, (list, tuple)):
+            modes = mode
+        else:
+            modes = [mode] * len(qs)
         return bool(np.all(self.batch_is_collision_free(qs, modes)))
 
     def batch_is_edge_collision_free(self, q1s, q2s, modes=None, resolution=None, include_endpoints=False,
@@ -321,7 +362,11 @@ class MrccEnvironment(BaseProblem):
                                  dtype=np.float64)
         B = np.ascontiguousarray(q2s if isinstance(q2s, np.ndarray) else np.stack([self._flat(q) for q in q2s]),
                                  dtype=np.float64)
+        if A.ndim != 2 or A.shape != B.shape or A.shape[1] != self.scene.D:
+            raise ValueError(f"q1s, q2s: need two [n, {self.scene.D}] arrays, got {A.shape} and {B.shape}")
         n = A.shape[0]
+        if modes is not None and len(modes) != n:
+            raise ValueError("modes need one entry per edge")
         if n == 0:
             return np.zeros(0, dtype=bool)
         iv = lambda v: None if v is None else torch.from_numpy(np.ascontiguousarray(v, dtype=np.int32)).cuda()

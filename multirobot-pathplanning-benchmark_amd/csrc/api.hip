@@ -292,7 +292,35 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
                 for (int m = b * dof; m < (b + 1) * dof; ++m) n_pairs += add_pair(l, m);
             lpair_sets[na + 1 + b * (b - 1) / 2 + a] = first | ((n_pairs - first) << 12) | (a << 20) | (b << 24);
         }
+    // the static LDS region of the arm kernels, laid out as arm_static_layout: a workgroup stages it
+    // with one contiguous copy
+    const int NP = n_pairs, nm = H.nm;
+    const ArmStaticLayout lay = arm_static_layout(L, na, nb, S, NP, nm, nclu, H.is_chain);
+    std::vector<unsigned char> img(lay.bytes, 0);
+    auto put = [&](int off, const void* src, size_t bytes) {
+        if (bytes) memcpy(img.data() + off, src, bytes);
+    };
+    put(lay.o_dh, dh4.data(), 8 * dh4.size());
+    put(lay.o_base, H.base.data(), 8 * 12 * (size_t)na);
+    for (int t = 0; t < S; ++t) put(lay.o_sphL + 24 * t, &H.sph[4 * t], 24);
+    put(lay.o_linkBC, link_bc.data(), 8 * link_bc.size());
+    if (nclu > 0) put(lay.o_box, box32.data(), 4 * box32.size());
+    put(lay.o_clu, clu32.data(), 32 * (size_t)nclu);
+    put(lay.o_sphR, sph_r32.data(), 4 * (size_t)S);
+    for (int t = 0; t < S; ++t) {
+        const int v = sph_link[t] | ((sph_link[t] / dof) << 8);  // link | arm << 8
+        put(lay.o_sphLink + 4 * t, &v, 4);
+    }
+    put(lay.o_linkR, link_R.data(), 4 * (size_t)L);
+    for (int l = 0; l < L; ++l) {
+        const int v = link_first[l] | ((link_first[l + 1] - link_first[l]) << 16);  // first | count << 16
+        put(lay.o_linkFC + 4 * l, &v, 4);
+    }
+    if (nm <= MRCC_MODES_LDS_MAX) put(lay.o_mact, H.m_active.data(), 4 * (size_t)nm);
+    put(lay.o_sets, lpair_sets.data(), 4 * lpair_sets.size());
+    if (NP <= ARM_PAIRS_LDS_MAX) put(lay.o_pairs, lpairs2.data(), 4 * lpairs2.size());
     Packer P;
+    size_t o_img = P.add(img);
     size_t o_lbc = P.add(link_bc), o_lR = P.add(link_R), o_lp = P.add(lpairs2), o_lpc = P.add(lpair_sets);
     size_t o_base = P.add(H.base), o_dh = P.add(dh4), o_lf = P.add(link_first), o_sl = P.add(sph_link),
            o_slo = P.add(sph_lo), o_s64 = P.add(H.sph), o_sr = P.add(sph_r32), o_b32 = P.add(box32),
@@ -360,6 +388,24 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     sc->n_robots = na;
     V.grasp = 0;
     for (unsigned char g : H.m_grasp) V.grasp |= (g != 0);
+    {
+        ArmK& K = sc->armk;
+        memset(&K, 0, sizeof(K));
+        K.na = na; K.dof = dof; K.D = V.D; K.L = L; K.S = S; K.nb = nb; K.nm = nm; K.NP = NP; K.nclu = nclu;
+        K.chain = V.chain;
+        K.broadphase = bp;
+        K.grasp = V.grasp;
+        K.img_bytes = lay.bytes;
+        K.Msph = V.Msph; K.Mobb = V.Mobb; K.Mcull = V.Mcull; K.Marm = V.Marm;
+        for (int k = 0; k < 3; ++k) {
+            K.bb_lo[k] = V.bb_lo[k];
+            K.bb_hi[k] = V.bb_hi[k];
+        }
+        K.g = (const unsigned char*)d;
+        K.o_img = (int)o_img; K.o_box32 = (int)o_b32; K.o_box64 = (int)o_b64; K.o_boxch32 = (int)o_bch32;
+        K.o_boxch64 = (int)o_bch64; K.o_sph64 = (int)o_s64; K.o_sphlo = (int)o_slo; K.o_pairs = (int)o_lp;
+        K.o_mact = (int)o_ma; K.o_mgrasp = (int)o_mg; K.o_mrel = (int)o_mr; K.o_mhalf = (int)o_mh;
+    }
     // waves per workgroup: the most resident waves per CU (160 KiB of LDS) within 64 KiB per
     // workgroup; the scene is staged once per workgroup
     sc->arm_nw = 0;
@@ -374,6 +420,14 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
         }
     }
     if (sc->arm_nw == 0) return fail(MRCC_ERR_INVALID, "arm scene needs more than 64 KiB of LDS per wave");
+    // the per-call kernel: one workgroup of as many waves (up to 16) as 64 KiB of LDS holds
+    sc->arm_nw_edge = 1;
+    for (int nw = 16; nw >= 1; --nw)
+        if (arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, nw) <= 65536) {
+            sc->arm_nw_edge = nw;
+            break;
+        }
+    sc->arm_lds_edge = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, sc->arm_nw_edge);
     sc->arm_lds = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, sc->arm_nw);
     sc->arm_lds1 = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, 1);
     sc->arm_na = na;
@@ -408,6 +462,10 @@ EdgeBatchView view_of(const mrcc_edge_batch* b) {
     v.modes = b->mode_ids;
     v.mask = b->robot_mask;
     v.out = b->out;
+    v.N0 = 0;
+    v.Ns0 = 0;
+    v.Nm0 = -1;
+    v.md0 = 0;
     return v;
 }
 
@@ -415,6 +473,7 @@ int check_batch(const SceneRec* sc, const mrcc_edge_batch* b) {
     if (!b) return fail(MRCC_ERR_INVALID, "null edge batch");
     if (b->n_edges < 0) return fail(MRCC_ERR_INVALID, "n_edges < 0");
     if (b->n_edges == 0) return MRCC_OK;  // nothing is read or written
+    if (b->n_edges > 0x7fffffffLL) return fail(MRCC_ERR_INVALID, "at most 2^31 - 1 edges per batch");
     if (!b->q1 || !b->q2 || !b->out) return fail(MRCC_ERR_INVALID, "null edge batch buffer");
     if (!b->N && !(b->resolution > 0.0)) return fail(MRCC_ERR_INVALID, "resolution must be > 0");
     if (sc->D > MRCC_MAX_D) return fail(MRCC_ERR_INVALID, "D > %d", MRCC_MAX_D);
@@ -716,6 +775,7 @@ int mrcc_scene_set_broadphase(mrcc_scene* scene, int enabled) {
     SceneRec* sc = rec_of(scene);
     if (sc->kind != KIND_ARMS) return fail(MRCC_ERR_UNSUPPORTED, "broad phase applies to arm scenes");
     sc->arm.broadphase = enabled ? 1 : 0;
+    sc->armk.broadphase = sc->arm.broadphase;
     return MRCC_OK;
 }
 
@@ -793,8 +853,7 @@ int mrcc_edge_plan_create(const mrcc_scene* scene, const mrcc_edge_batch* batch,
     const int D = sc->D;
     Packer P;  // sizes only
     auto sz = [&](size_t bytes) { size_t off = (P.buf.size() + 255) & ~size_t(255); P.buf.resize(off + bytes); return off; };
-    size_t o_eN = sz(4 * n), o_eM = sz(4 * n), o_eS = sz(4 * n), o_w0 = sz(4 * n), o_full = sz(n),
-           o_order = sz(4 * n), o_hist = sz(4 * (MRCC_MAX_PASSES + 1)), o_cur = sz(4 * (MRCC_MAX_PASSES + 1)),
+    size_t o_rec = sz(16 * n), o_order = sz(4 * n), o_hist = sz(4 * (MRCC_MAX_PASSES + 1)), o_cur = sz(4 * (MRCC_MAX_PASSES + 1)),
            o_ps = sz(8 * (MRCC_MAX_PASSES + 1)), o_meta = sz(8 * 8), o_long = sz(4 * n), o_tick = sz(8 * 32 * 8),
            o_stats = sz(8 * 8), o_dir = sz(8 * (size_t)n * D);
     if (hipMalloc(&p->work_mem, P.buf.size()) != hipSuccess) {
@@ -802,11 +861,7 @@ int mrcc_edge_plan_create(const mrcc_scene* scene, const mrcc_edge_batch* batch,
         return fail(MRCC_ERR_NOMEM, "workspace of %zu bytes", P.buf.size());
     }
     unsigned char* w = (unsigned char*)p->work_mem;
-    p->W.eN = (int*)(w + o_eN);
-    p->W.eM = (int*)(w + o_eM);
-    p->W.eS = (int*)(w + o_eS);
-    p->W.ew0 = (int*)(w + o_w0);
-    p->W.efull = w + o_full;
+    p->W.erec = (int4*)(w + o_rec);
     p->W.order = (int*)(w + o_order);
     p->W.hist = (int*)(w + o_hist);
     p->W.cursor = (int*)(w + o_cur);
@@ -933,37 +988,45 @@ static int ensure_stage(SceneRec* sc) {
     return MRCC_OK;
 }
 
-// Stages (optionally) q1 / q2 and the scalar arguments of one edge; returns the 1-edge view.
-// Layout of the staging block: q1[MAX_D] q2[MAX_D] int32{N, N_start, N_max, mode} out.
-static int stage_edge(SceneRec* sc, const double* q1h, const double* q2h, int32_t N, int32_t N_start,
-                      int32_t N_max, int32_t mode_id, hipStream_t st, EdgeBatchView* v) {
+// Mode ids of host-side arguments: arm scenes hold modes [0, n_modes); the other kinds ignore the
+// mode, as the reference's abstract environment does (abstract_env.py:255-276).
+static int check_mode(const SceneRec* sc, int32_t mode_id) {
+    if (sc->kind == KIND_ARMS && (mode_id < 0 || mode_id >= sc->arm.nm))
+        return fail(MRCC_ERR_INVALID, "mode id %d outside [0, %d)", mode_id, sc->arm.nm);
+    return MRCC_OK;
+}
+
+// One-edge view with the scalar arguments by value (no staging of them)
+static EdgeBatchView one_edge_view(const SceneRec* sc, const double* q1, const double* q2, double resolution,
+                                   int32_t N, int32_t N_start, int32_t N_max, int32_t include_endpoints,
+                                   int32_t mode_id, uint32_t robot_mask, uint8_t* out) {
+    EdgeBatchView v;
+    memset(&v, 0, sizeof(v));
+    v.q1 = q1;
+    v.q2 = q2;
+    v.rs = sc->D;
+    v.ds = 1;
+    v.n = 1;
+    v.res = resolution;
+    v.incl = include_endpoints;
+    v.mask = robot_mask;
+    v.out = out;
+    v.N0 = N > 0 ? N : 0;
+    v.Ns0 = N_start;
+    v.Nm0 = N_max < 0 ? -1 : N_max;
+    v.md0 = mode_id;
+    return v;
+}
+
+// Copies one or two host rows into the scene's pinned and device staging block (synchronous
+// callers only: the previous call has finished reading it).  Layout: q1[MAX_D] q2[MAX_D] ... out.
+static int stage_rows(SceneRec* sc, const double* q1h, const double* q2h, hipStream_t st) {
     int rc = ensure_stage(sc);
     if (rc != MRCC_OK) return rc;
-    HIP_OK(hipStreamSynchronize(st));  // the previous call may still read the block
     double* h = sc->stage_host;
-    size_t bytes = 8 * (2 * MRCC_MAX_D) + 16;
     if (q1h) memcpy(h, q1h, 8 * (size_t)sc->D);
     if (q2h) memcpy(h + MRCC_MAX_D, q2h, 8 * (size_t)sc->D);
-    int32_t* ih = (int32_t*)(h + 2 * MRCC_MAX_D);
-    ih[0] = N;
-    ih[1] = N_start;
-    ih[2] = N_max;
-    ih[3] = mode_id;
-    sc->stage_out_host[0] = 1;
-    bytes += 8;
-    HIP_OK(hipMemcpyAsync(sc->stage_dev, h, bytes, hipMemcpyHostToDevice, st));
-    const int32_t* id = (const int32_t*)(sc->stage_dev + 2 * MRCC_MAX_D);
-    memset(v, 0, sizeof(*v));
-    v->q1 = sc->stage_dev;
-    v->q2 = sc->stage_dev + MRCC_MAX_D;
-    v->rs = sc->D;
-    v->ds = 1;
-    v->n = 1;
-    v->N = N > 0 ? id : nullptr;
-    v->N_start = id + 1;
-    v->N_max = id + 2;
-    v->modes = id + 3;
-    v->out = sc->stage_out_dev;
+    HIP_OK(hipMemcpyAsync(sc->stage_dev, h, 8 * (2 * MRCC_MAX_D), hipMemcpyHostToDevice, st));
     return MRCC_OK;
 }
 
@@ -974,25 +1037,13 @@ int mrcc_check_edge(const mrcc_scene* scene, const double* q1, const double* q2,
     if (!scene || !q1 || !q2 || !out) return fail(MRCC_ERR_INVALID, "null argument");
     if (N <= 0 && !(resolution > 0.0)) return fail(MRCC_ERR_INVALID, "resolution must be > 0");
     if (N_start < 0) return fail(MRCC_ERR_INVALID, "N_start < 0");
-    SceneRec* sc = (SceneRec*)rec_of(scene);
-    hipStream_t st = (hipStream_t)stream;
-    EdgeBatchView v;
-    memset(&v, 0, sizeof(v));
-    if (N > 0 || N_start != 0 || N_max >= 0 || mode_id != 0) {
-        // scalar arguments travel through the scene's staging block
-        int rc = stage_edge(sc, nullptr, nullptr, N, N_start, N_max, mode_id, st, &v);
-        if (rc != MRCC_OK) return rc;
-    }
-    v.q1 = q1;
-    v.q2 = q2;
-    v.rs = sc->D;
-    v.ds = 1;
-    v.n = 1;
-    v.res = resolution;
-    v.incl = include_endpoints;
-    v.mask = robot_mask;
-    v.out = out;
-    launch_direct(*sc, v, nullptr, 1, nullptr, 1, 1, 1, nullptr, st);
+    const SceneRec* sc = rec_of(scene);
+    int rc = check_mode(sc, mode_id);
+    if (rc != MRCC_OK) return rc;
+    // every argument travels by value: nothing host-side is shared with an earlier asynchronous call
+    const EdgeBatchView v = one_edge_view(sc, q1, q2, resolution, N, N_start, N_max, include_endpoints,
+                                          mode_id, robot_mask, out);
+    launch_edge1(*sc, v, (hipStream_t)stream);
     HIP_OK(hipGetLastError());
     return MRCC_OK;
 }
@@ -1006,6 +1057,8 @@ int mrcc_check_edge_host(const mrcc_scene* scene, const double* q1, const double
     if (N <= 0 && !(resolution > 0.0)) return fail(MRCC_ERR_INVALID, "resolution must be > 0");
     if (N_start < 0) return fail(MRCC_ERR_INVALID, "N_start < 0");
     SceneRec* sc = (SceneRec*)rec_of(scene);
+    int rc = check_mode(sc, mode_id);
+    if (rc != MRCC_OK) return rc;
     int Nh = N;
     if (Nh <= 0) {  // the same binary64 rule as the device (abstract_env.py:321-323)
         double m = fabs(q1[0] - q2[0]);
@@ -1015,13 +1068,11 @@ int mrcc_check_edge_host(const mrcc_scene* scene, const double* q1, const double
     if (N_start > Nh) return fail(MRCC_ERR_INVALID, "N_start %d > N %d (the reference asserts)", N_start, Nh);
     if (Nh > MRCC_MAX_EDGE_STATES) return fail(MRCC_ERR_INVALID, "edge of %d states (at most %d)", Nh, MRCC_MAX_EDGE_STATES);
     hipStream_t st = nullptr;
-    EdgeBatchView v;
-    int rc = stage_edge(sc, q1, q2, N, N_start, N_max, mode_id, st, &v);
+    rc = stage_rows(sc, q1, q2, st);
     if (rc != MRCC_OK) return rc;
-    v.res = resolution;
-    v.incl = include_endpoints;
-    v.mask = robot_mask;
-    launch_direct(*sc, v, nullptr, 1, nullptr, 1, 0, 1, nullptr, st);
+    const EdgeBatchView v = one_edge_view(sc, sc->stage_dev, sc->stage_dev + MRCC_MAX_D, resolution, N, N_start,
+                                          N_max, include_endpoints, mode_id, robot_mask, sc->stage_out_dev);
+    launch_edge1(*sc, v, st);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(sc->stage_out_host, sc->stage_out_dev, 1, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
@@ -1073,6 +1124,10 @@ int mrcc_check_edges_host(const mrcc_scene* scene, const double* q1, const doubl
         }
         const int ns = N_start ? N_start[i] : 0;
         if (ns < 0) return fail(MRCC_ERR_INVALID, "edge %lld: N_start < 0", (long long)i);
+        if (mode_ids) {
+            const int rm = check_mode(sc, mode_ids[i]);
+            if (rm != MRCC_OK) return rm;
+        }
         if (ns > Nh) return fail(MRCC_ERR_INVALID, "edge %lld: N_start %d > N %d (the reference asserts)", (long long)i, ns, Nh);
         if (Nh > MRCC_MAX_EDGE_STATES)
             return fail(MRCC_ERR_INVALID, "edge %lld of %d states (at most %d)", (long long)i, Nh, MRCC_MAX_EDGE_STATES);
@@ -1100,6 +1155,7 @@ int mrcc_check_edges_host(const mrcc_scene* scene, const double* q1, const doubl
     unsigned char* d = sc->batch_dev;
     EdgeBatchView v;
     memset(&v, 0, sizeof(v));
+    v.Nm0 = -1;  // N_max None unless given
     v.q1 = (const double*)d;
     v.q2 = (const double*)(d + rows);
     v.rs = D;
@@ -1128,11 +1184,17 @@ int mrcc_check_config_host(const mrcc_scene* scene, const double* q, int32_t mod
     CHECK_INJECT();
     if (!scene || !q || !result) return fail(MRCC_ERR_INVALID, "null argument");
     SceneRec* sc = (SceneRec*)rec_of(scene);
-    hipStream_t st = nullptr;
-    EdgeBatchView v;
-    int rc = stage_edge(sc, q, nullptr, 0, 0, -1, mode_id, st, &v);
+    int rc = check_mode(sc, mode_id);
     if (rc != MRCC_OK) return rc;
-    launch_configs(*sc, v.q1, 1, sc->D, 1, v.modes, robot_mask, v.out, 1, st);
+    hipStream_t st = nullptr;
+    rc = stage_rows(sc, q, nullptr, st);
+    if (rc != MRCC_OK) return rc;
+    // the mode id travels in the staging block's int area (after the two rows)
+    int32_t* ih = (int32_t*)(sc->stage_host + 2 * MRCC_MAX_D);
+    ih[0] = mode_id;
+    const int32_t* md = (const int32_t*)(sc->stage_dev + 2 * MRCC_MAX_D);
+    HIP_OK(hipMemcpyAsync((void*)md, ih, 4, hipMemcpyHostToDevice, st));
+    launch_configs(*sc, sc->stage_dev, 1, sc->D, 1, md, robot_mask, sc->stage_out_dev, 1, st);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(sc->stage_out_host, sc->stage_out_dev, 1, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
@@ -1149,12 +1211,11 @@ int mrcc_check_edges_per_call(const mrcc_scene* scene, const mrcc_edge_batch* ba
     EdgeBatchView v = view_of(batch);
     hipStream_t st = (hipStream_t)stream;
     for (long long e = 0; e < batch->n_edges; ++e) {
-        // one launch per edge: the batch view offset to edge e
+        // one launch per edge (the plain per-call kernel): the batch view offset to edge e
         EdgeBatchView u = v;
-        const long long r1 = e, r2 = e;
-        if (!v.idx1) u.q1 = v.q1 + r1 * v.rs;
+        if (!v.idx1) u.q1 = v.q1 + e * v.rs;
         else u.idx1 = v.idx1 + e;
-        if (!v.idx2) u.q2 = v.q2 + r2 * v.rs;
+        if (!v.idx2) u.q2 = v.q2 + e * v.rs;
         else u.idx2 = v.idx2 + e;
         if (v.N) u.N = v.N + e;
         if (v.N_start) u.N_start = v.N_start + e;
@@ -1162,7 +1223,7 @@ int mrcc_check_edges_per_call(const mrcc_scene* scene, const mrcc_edge_batch* ba
         if (v.modes) u.modes = v.modes + e;
         u.out = v.out + e;
         u.n = 1;
-        launch_direct(*sc, u, nullptr, 1, nullptr, 1, 1, 1, nullptr, st);
+        launch_edge1(*sc, u, st);
     }
     HIP_OK(hipGetLastError());
     return MRCC_OK;

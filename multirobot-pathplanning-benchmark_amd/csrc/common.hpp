@@ -433,7 +433,39 @@ struct ArmView {
     const float* clu32;      // [nclu*8] cluster bounds: lo xyz, first box (int bits), hi xyz, count
 };
 
-// batch of edges as seen by kernels
+// Arm scene as the edge and configuration kernels see it: scalars, and one base pointer into the
+// scene's device allocation with 32-bit byte offsets of the arrays read from global memory (fewer
+// scalar registers than one 64-bit pointer per array).  The tables the evaluator reads from LDS are
+// one contiguous image in the allocation, laid out as the static LDS region (arm_static_layout),
+// which a workgroup copies with 16-byte loads.
+struct ArmK {
+    int na, dof, D, L, S, nb, nm, NP, nclu;
+    int chain;       // 1: chain joints
+    int broadphase;  // 1: link bounding spheres cull sphere tests
+    int grasp;       // 1: some mode holds a box
+    int img_bytes;   // size of the static LDS image (multiple of 16)
+    float Msph, Mobb, Mcull, Marm;
+    float bb_lo[3], bb_hi[3];
+    const unsigned char* g;
+    int o_img, o_box32, o_box64, o_boxch32, o_boxch64, o_sph64, o_sphlo, o_pairs, o_mact, o_mgrasp,
+        o_mrel, o_mhalf;
+    __device__ const float4* img() const { return (const float4*)(g + o_img); }
+    __device__ const float* box32() const { return (const float*)(g + o_box32); }       // [nb*8]
+    __device__ const double* box64() const { return (const double*)(g + o_box64); }     // [nb*6] lo hi
+    __device__ const float* boxch32() const { return (const float*)(g + o_boxch32); }   // [nb*6]
+    __device__ const double* boxch64() const { return (const double*)(g + o_boxch64); } // [nb*6]
+    __device__ const double* sph64() const { return (const double*)(g + o_sph64); }     // [S*4]
+    __device__ const int* sph_lo() const { return (const int*)(g + o_sphlo); }          // [S]
+    __device__ const int2* lpairs2() const { return (const int2*)(g + o_pairs); }       // [NP]
+    __device__ const int* m_active() const { return (const int*)(g + o_mact); }         // [nm]
+    __device__ const unsigned char* m_grasp() const { return g + o_mgrasp; }            // [nm*na]
+    __device__ const double* m_rel() const { return (const double*)(g + o_mrel); }      // [nm*na*12]
+    __device__ const double* m_half() const { return (const double*)(g + o_mhalf); }    // [nm*na*3]
+};
+
+// batch of edges as seen by kernels.  The scalars N0, Ns0, Nm0, md0 stand for the arrays N, N_start,
+// N_max, modes when those are null (N0 <= 0: N from the resolution), so a one-edge call passes its
+// arguments by value.
 struct EdgeBatchView {
     const double* q1;
     const double* q2;
@@ -449,15 +481,49 @@ struct EdgeBatchView {
     const int* modes;
     unsigned mask;
     unsigned char* out;
+    int N0, Ns0, Nm0, md0;
 };
+
+__host__ __device__ inline int batch_N_start(const EdgeBatchView& B, long long e) {
+    return B.N_start ? B.N_start[e] : B.Ns0;
+}
+__host__ __device__ inline int batch_N_max(const EdgeBatchView& B, long long e) {
+    return B.N_max ? B.N_max[e] : B.Nm0;
+}
+__host__ __device__ inline int batch_mode(const EdgeBatchView& B, long long e) {
+    return B.modes ? B.modes[e] : B.md0;
+}
+
+// What the persistent arm kernels read of a batch and its plan, in one argument: the fields of
+// EdgeBatchView and PlanWork they use, under the same names (fewer scalar registers than both).
+struct ArmQueue {
+    const double* q1;
+    const int* idx1;
+    const int* modes;
+    unsigned char* out;
+    const int4* erec;
+    const int* order;
+    const long long* pass_start;
+    unsigned long long* ticket;
+    const double* dir;
+    unsigned long long* stats;
+    const long long* meta;
+    long long rs;
+    int ds, n, md0, incl;
+    unsigned mask;
+};
+
+__host__ __device__ inline int batch_mode(const ArmQueue& B, long long e) {
+    return B.modes ? B.modes[e] : B.md0;
+}
+
+// Per-edge record of a plan: N, slot count M, passes S, window start | full << 31.
+__host__ __device__ inline int erec_w0(int4 r) { return r.w & 0x7fffffff; }
+__host__ __device__ inline int erec_full(int4 r) { return (int)((unsigned)r.w >> 31); }
 
 // workspace of a batched plan
 struct PlanWork {
-    int* eN;           // [n] state count
-    int* eM;           // [n] slots
-    int* eS;           // [n] passes
-    int* ew0;          // [n] window start
-    unsigned char* efull;  // [n]
+    int4* erec;        // [n] per-edge record (N, M, S, w0 | full << 31)
     int* order;        // [n] edges longest-first
     int* hist;         // [MAX_PASSES+1]
     int* cursor;       // [MAX_PASSES+1]

@@ -11,10 +11,13 @@ struct SceneRec {
     int device;
     DiskView disk;
     PlanarView planar;
-    ArmView arm;
+    ArmView arm;          // every array of an arm scene (clearance and kinematics kernels)
+    ArmK armk;            // the edge and configuration kernels' view of the same scene
     size_t arm_lds;       // dynamic LDS bytes of the arm kernels (arm_nw waves per workgroup)
     size_t arm_lds1;      // dynamic LDS bytes of an arm kernel with one wave per workgroup
     int arm_nw;           // waves per workgroup of the arm kernels
+    int arm_nw_edge;      // waves of the per-call kernel's one workgroup
+    size_t arm_lds_edge;  // its dynamic LDS bytes
     void* dev_mem;        // one allocation holding every array the views point to
     size_t dev_bytes;
     // host copies needed by host code
@@ -39,6 +42,8 @@ void launch_direct(const SceneRec& sc, const EdgeBatchView& B, const int* list, 
                    const long long* n_dev, int wpe, int init_out, int max_grid,
                    unsigned long long* stats, hipStream_t st);
 void launch_fill_u8(unsigned char* p, long long n, unsigned char v, hipStream_t st);
+// plain per-call form: edge 0 of B in one workgroup (k_arm_edge / k_lane_edge)
+void launch_edge1(const SceneRec& sc, const EdgeBatchView& B, hipStream_t st);
 void launch_configs(const SceneRec& sc, const double* q, long long n, long long rs, long long ds,
                     const int* modes, unsigned mask, unsigned char* out, int max_grid,
                     hipStream_t st);

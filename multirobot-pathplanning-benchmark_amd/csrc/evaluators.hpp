@@ -5,6 +5,8 @@
 // FK and split the pair set).  Every evaluator returns true when the state is in collision.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 
 using namespace mrcc;
@@ -402,12 +404,25 @@ struct ArmLds {
     const int* sets;       // self-pair counts and arm-pair segments of the pair list
 };
 
-// tests executed by one lane (broad phase link-box / sphere-box / link-pair / sphere-pair)
-struct ArmCount {
-    unsigned lb, sb, lp, sp;
+// A counter that counts nothing: kernels built without statistics keep no registers for them.
+struct NoCount {
+    __device__ NoCount& operator+=(unsigned) { return *this; }
+    __device__ NoCount& operator++() { return *this; }
 };
 
-__device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
+// tests executed by one lane (broad phase link-box / sphere-box / link-pair / sphere-pair)
+template <bool STATS>
+struct ArmCountT {
+    unsigned lb, sb, lp, sp;
+};
+template <>
+struct ArmCountT<false> {
+    NoCount lb, sb, lp, sp;
+};
+template <bool STATS>
+using RecheckT = typename std::conditional<STATS, unsigned, NoCount>::type;
+
+__device__ inline ArmLds arm_lds(void* base, const ArmK& V) {
     ArmLds W;
     unsigned char* b = (unsigned char*)base;
     const ArmStaticLayout o = arm_static_layout(V.L, V.na, V.nb, V.S, V.NP, V.nm, V.nclu, V.chain);
@@ -416,7 +431,7 @@ __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
     W.sphL = (const double*)(b + o.o_sphL);
     W.linkBC = (const double*)(b + o.o_linkBC);
     W.box = (const float4*)(b + o.o_box);
-    W.boxG = (const float4*)V.box32;
+    W.boxG = (const float4*)V.box32();
     W.sphR = (const float*)(b + o.o_sphR);
     W.sphLink = (const int*)(b + o.o_sphLink);
     W.clu = (const float4*)(b + o.o_clu);
@@ -444,50 +459,24 @@ __device__ inline ArmLds arm_lds(void* base, const ArmView& V) {
     return W;
 }
 
-// Stages the scene into the static LDS region (all threads of the workgroup, then a barrier).
-__device__ inline void arm_stage_scene(void* base, const ArmView& V) {
-    unsigned char* b = (unsigned char*)base;
-    const ArmStaticLayout o = arm_static_layout(V.L, V.na, V.nb, V.S, V.NP, V.nm, V.nclu, V.chain);
-    const int t = threadIdx.x, nt = blockDim.x;
-    double* dh = (double*)(b + o.o_dh);
-    for (int i = t; i < (V.chain ? 12 : 4) * V.L; i += nt) dh[i] = V.dh[i];
-    double* base_ = (double*)(b + o.o_base);
-    for (int i = t; i < 12 * V.na; i += nt) base_[i] = V.base[i];
-    double* sphL = (double*)(b + o.o_sphL);
-    for (int i = t; i < V.S; i += nt) {
-        sphL[3 * i] = V.sph64[4 * i];
-        sphL[3 * i + 1] = V.sph64[4 * i + 1];
-        sphL[3 * i + 2] = V.sph64[4 * i + 2];
-        ((float*)(b + o.o_sphR))[i] = V.sph_r32[i];
-        ((int*)(b + o.o_sphLink))[i] = V.sph_link[i] | ((V.sph_link[i] / V.dof) << 8);  // link | arm << 8
-    }
-    double* bc = (double*)(b + o.o_linkBC);
-    for (int i = t; i < 3 * V.L; i += nt) bc[i] = V.link_bc64[i];
-    for (int i = t; i < V.L; i += nt) {
-        ((float*)(b + o.o_linkR))[i] = V.link_R32[i];
-        ((int*)(b + o.o_linkFC))[i] = V.link_first[i] | ((V.link_first[i + 1] - V.link_first[i]) << 16);
-    }
-    if (V.nclu > 0) {
-        float4* box = (float4*)(b + o.o_box);
-        const float4* src = (const float4*)V.box32;
-        for (int i = t; i < 2 * V.nb; i += nt) box[i] = src[i];
-    }
-    {
-        float4* cl = (float4*)(b + o.o_clu);
-        const float4* src = (const float4*)V.clu32;
-        for (int i = t; i < 2 * V.nclu; i += nt) cl[i] = src[i];
-    }
-    if (V.nm <= MRCC_MODES_LDS_MAX) {
-        int* ma = (int*)(b + o.o_mact);
-        for (int i = t; i < V.nm; i += nt) ma[i] = V.m_active[i];
-    }
-    {
-        int* st = (int*)(b + o.o_sets);
-        for (int i = t; i < V.na + 1 + V.na * (V.na - 1) / 2; i += nt) st[i] = V.lpair_sets[i];
-    }
-    if (V.NP <= ARM_PAIRS_LDS_MAX) {
-        int2* pr = (int2*)(b + o.o_pairs);
-        for (int i = t; i < V.NP; i += nt) pr[i] = V.lpairs2[i];
+// Stages the scene into the static LDS region: a copy of the scene's LDS image (all threads of the
+// workgroup, 16-byte loads, four per thread issued before their stores), then a barrier.
+__device__ inline void arm_stage_scene(void* base, const ArmK& V) {
+    float4* dst = (float4*)base;
+    const float4* src = V.img();
+    const int n16 = V.img_bytes >> 4;
+    const int nt = blockDim.x;
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int i0 = threadIdx.x; i0 < n16; i0 += 4 * nt) {
+        const int i1 = i0 + nt, i2 = i1 + nt, i3 = i2 + nt;
+        const float4 a = src[i0];
+        const float4 b = i1 < n16 ? src[i1] : z;
+        const float4 c = i2 < n16 ? src[i2] : z;
+        const float4 d = i3 < n16 ? src[i3] : z;
+        dst[i0] = a;
+        if (i1 < n16) dst[i1] = b;
+        if (i2 < n16) dst[i2] = c;
+        if (i3 < n16) dst[i3] = d;
     }
     __syncthreads();
 }
@@ -549,7 +538,8 @@ __device__ inline void arm_pend(const ArmLds& W, bool band, int kind, int i, int
 
 // Rechecks the queued pairs in binary64 (same predicates as the oracle); empties the queue.
 // Returns the wave-uniform "some pair collides".
-__device__ inline bool arm_flush(const ArmView& V, const ArmLds& W, int& np, unsigned& rechecks) {
+template <class RC>
+__device__ inline bool arm_flush(const ArmK& V, const ArmLds& W, int& np, RC& rechecks) {
     wave_sync();
     bool hit = false;
     for (int k = wave_lane(); k < np; k += 64) {
@@ -558,11 +548,11 @@ __device__ inline bool arm_flush(const ArmView& V, const ArmLds& W, int& np, uns
         double a[3];
         xform_point_f64(W.X + 12 * (W.sphLink[i] & 255), W.sphL[3 * i], W.sphL[3 * i + 1], W.sphL[3 * i + 2], a);
         if (e.x == 0) {
-            hit |= sph_box_f64(a, V.sph64[4 * i + 3], V.box64 + 6 * j, V.box64 + 6 * j + 3);
+            hit |= sph_box_f64(a, V.sph64()[4 * i + 3], V.box64() + 6 * j, V.box64() + 6 * j + 3);
         } else {
             double b[3];
             xform_point_f64(W.X + 12 * (W.sphLink[j] & 255), W.sphL[3 * j], W.sphL[3 * j + 1], W.sphL[3 * j + 2], b);
-            hit |= sph_sph_f64(a, V.sph64[4 * i + 3], b, V.sph64[4 * j + 3]);
+            hit |= sph_sph_f64(a, V.sph64()[4 * i + 3], b, V.sph64()[4 * j + 3]);
         }
     }
     if (wave_lane() == 0) rechecks += np;
@@ -573,8 +563,8 @@ __device__ inline bool arm_flush(const ArmView& V, const ArmLds& W, int& np, uns
 
 // Rechecks queued held-box pairs in binary64: kind 2 held box i - obstacle box j, 3 sphere i -
 // held box j, 4 held boxes i, j.  Empties the queue; returns the wave-uniform "some pair collides".
-__device__ inline bool arm_flush_held(const ArmView& V, const ArmLds& W, int mode, int& np,
-                                      unsigned& rechecks) {
+template <class RC>
+__device__ inline bool arm_flush_held(const ArmK& V, const ArmLds& W, int mode, int& np, RC& rechecks) {
     wave_sync();
     bool hit = false;
     for (int k = wave_lane(); k < np; k += 64) {
@@ -582,18 +572,18 @@ __device__ inline bool arm_flush_held(const ArmView& V, const ArmLds& W, int mod
         const int i = e.y >> 16, j = e.y & 0xffff;
         if (e.x == 2) {
             double A[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
-            const double* bc = V.boxch64 + 6 * j;
+            const double* bc = V.boxch64() + 6 * j;
             A[9] = bc[0];
             A[10] = bc[1];
             A[11] = bc[2];
-            hit |= obb3_overlap_f64(A, bc + 3, W.G64 + 12 * i, V.m_half + 3 * (mode * V.na + i));
+            hit |= obb3_overlap_f64(A, bc + 3, W.G64 + 12 * i, V.m_half() + 3 * (mode * V.na + i));
         } else if (e.x == 3) {
             double c[3];
             xform_point_f64(W.X + 12 * (W.sphLink[i] & 255), W.sphL[3 * i], W.sphL[3 * i + 1], W.sphL[3 * i + 2], c);
-            hit |= sph_obb_f64(c, V.sph64[4 * i + 3], W.G64 + 12 * j, V.m_half + 3 * (mode * V.na + j));
+            hit |= sph_obb_f64(c, V.sph64()[4 * i + 3], W.G64 + 12 * j, V.m_half() + 3 * (mode * V.na + j));
         } else {
-            hit |= obb3_overlap_f64(W.G64 + 12 * i, V.m_half + 3 * (mode * V.na + i), W.G64 + 12 * j,
-                                    V.m_half + 3 * (mode * V.na + j));
+            hit |= obb3_overlap_f64(W.G64 + 12 * i, V.m_half() + 3 * (mode * V.na + i), W.G64 + 12 * j,
+                                    V.m_half() + 3 * (mode * V.na + j));
         }
     }
     if (wave_lane() == 0) rechecks += np;
@@ -608,16 +598,16 @@ typedef __attribute__((address_space(3))) const int lds_cint;
 typedef __attribute__((address_space(3))) const unsigned long long lds_cu64;
 
 // link pair pi (LDS table for up to ARM_PAIRS_LDS_MAX pairs)
-__device__ inline int2 arm_pair(const ArmView& V, const ArmLds& W, int pi) {
+__device__ inline int2 arm_pair(const ArmK& V, const ArmLds& W, int pi) {
     if (V.NP <= ARM_PAIRS_LDS_MAX) {
         const unsigned long long v = ((lds_cu64*)W.pairs)[pi];
         return make_int2((int)(unsigned)v, (int)(unsigned)(v >> 32));
     }
-    return V.lpairs2[pi];
+    return V.lpairs2()[pi];
 }
-__device__ inline int arm_mact(const ArmView& V, const ArmLds& W, int md) {
+__device__ inline int arm_mact(const ArmK& V, const ArmLds& W, int md) {
     if (V.nm <= MRCC_MODES_LDS_MAX) return ((lds_cint*)W.mact)[md];
-    return V.m_active[md];
+    return V.m_active()[md];
 }
 
 // Per-lane register loaded once per kernel: the lane's link (first sphere, sphere count).
@@ -625,7 +615,7 @@ struct ArmRegs {
     int linkFC;
 };
 
-__device__ inline ArmRegs arm_regs(const ArmView& V, const ArmLds& W) {
+__device__ inline ArmRegs arm_regs(const ArmK& V, const ArmLds& W) {
     ArmRegs R;
     const int lane = wave_lane();
     R.linkFC = lane < V.L ? W.linkFC[lane] : 0;
@@ -646,12 +636,14 @@ __device__ inline float box_d2(float4 c, float4 lo, float4 hi) {
 // CHAIN selects chain joints (V.chain = 1) over DH joints at compile time.
 // `lazy` (wave-uniform): place the spheres beyond the first 64 only when a candidate pair needs
 // them (pays when most states collide; the caller decides from its recent states).
-template <bool GRASP, bool CHAIN>
-__device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, const ArmRegs& R, double qv,
-                                          int mode, unsigned mask, unsigned& rechecks, ArmCount& cnt,
-                                          bool lazy = false) {
+// STATS = false compiles the test counters away.  A mode outside [0, nm) is reported as a
+// collision (callers reject such modes before they get here).
+template <bool GRASP, bool CHAIN, bool STATS>
+__device__ inline bool arm_state_collides(const ArmK& V, const ArmLds& W, const ArmRegs& R, double qv,
+                                          int mode, unsigned mask, RecheckT<STATS>& rechecks,
+                                          ArmCountT<STATS>& cnt, bool lazy = false) {
     const int lane = wave_lane();
-    if (mode < 0 || mode >= V.nm) mode = 0;
+    if (mode < 0 || mode >= V.nm) return true;
     const int na = arm_mact(V, W, mode);
     const int dof = V.dof;
     const int Dact = na * dof;
@@ -723,8 +715,8 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                     R2 = quad_bcast_f64(y, 2);
                     t = quad_bcast_f64(y, 3);
                 }
-                if (GRASP && c == 0 && V.m_grasp[mode * V.na + a]) {
-                    const double* Rel = V.m_rel + 12 * (mode * V.na + a);
+                if (GRASP && c == 0 && V.m_grasp()[mode * V.na + a]) {
+                    const double* Rel = V.m_rel() + 12 * (mode * V.na + a);
                     double g[4];
                     compose_row_f64(R0, R1, R2, t, Rel, g);
                     for (int cc = 0; cc < 3; ++cc) {
@@ -733,7 +725,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                     }
                     W.G64[12 * a + 9 + rr] = g[3];
                     W.G32[16 * a + 9 + rr] = (float)g[3];
-                    W.G32[16 * a + 12 + rr] = (float)V.m_half[3 * (mode * V.na + a) + rr];
+                    W.G32[16 * a + 12 + rr] = (float)V.m_half()[3 * (mode * V.na + a) + rr];
                 }
             }
         }
@@ -1024,7 +1016,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
         // (T - M: certain contact; T + M: near contact); a block in which some lane is near contact
         // without a certain one is tested again with the band bookkeeping (rare).
         // 4. spheres vs boxes
-        const float4* gbox = (const float4*)V.box32;  // uniform index: scalar loads
+        const float4* gbox = (const float4*)V.box32();  // uniform index: scalar loads
         for (int base = 0; base < S_act; base += 64) {
             const int s = base + lane;
             const bool act = s < S_act;
@@ -1056,8 +1048,8 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             const int i = base + lane;
             const bool act = i < S_act;
             const float4 ci = W.sph[act ? i : 0];
-            const int lo = act ? V.sph_lo[i] : 0x7fffffff;
-            const int jstart = V.sph_lo[base];
+            const int lo = act ? V.sph_lo()[i] : 0x7fffffff;
+            const int jstart = V.sph_lo()[base];
             for (int j0 = jstart; j0 < S_act; j0 += 16) {
                 const int j1 = j0 + 16 < S_act ? j0 + 16 : S_act;
                 bool near = false;
@@ -1094,7 +1086,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             const bool act = s < S_act && ((mask >> (W.sphLink[s < S_act ? s : 0] >> 8)) & 1u);
             const float4 c = W.sph[act ? s : 0];
             const float T = c.w * c.w;
-            const float4* gbox = (const float4*)V.box32;  // uniform index: scalar loads
+            const float4* gbox = (const float4*)V.box32();  // uniform index: scalar loads
             for (int b = 0; b < V.nb; ++b) {
                 const float d2 = box_d2(c, gbox[2 * b], gbox[2 * b + 1]);
                 cnt.sb += act;
@@ -1111,8 +1103,8 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
             const int i = base + lane;
             const bool act = i < S_act && ((mask >> (W.sphLink[i < S_act ? i : 0] >> 8)) & 1u);
             const float4 ci = W.sph[act ? i : 0];
-            const int lo = act ? V.sph_lo[i] : 0x7fffffff;
-            const int jstart = V.sph_lo[base];
+            const int lo = act ? V.sph_lo()[i] : 0x7fffffff;
+            const int jstart = V.sph_lo()[base];
             int nj = 0;
             for (int j = jstart; j < S_act; ++j) {
                 const int aj = W.sphLink[j] >> 8;
@@ -1141,7 +1133,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
     if constexpr (!GRASP) return false;
     unsigned gmask = 0;
     for (int a = 0; a < na; ++a)
-        if (V.m_grasp[mode * V.na + a] && ((mask >> a) & 1u)) gmask |= 1u << a;
+        if (V.m_grasp()[mode * V.na + a] && ((mask >> a) & 1u)) gmask |= 1u << a;
     if (gmask) {
         const float Mb = V.Mobb;
         // 6a. held box vs obstacle boxes
@@ -1153,7 +1145,7 @@ __device__ inline bool arm_state_collides(const ArmView& V, const ArmLds& W, con
                 bool band = false;
                 if (b < V.nb) {
                     float A[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
-                    const float* bc = V.boxch32 + 6 * b;
+                    const float* bc = V.boxch32() + 6 * b;
                     A[9] = bc[0]; A[10] = bc[1]; A[11] = bc[2];
                     const int r = obb3_overlap_f32(A, bc + 3, G, G + 12, Mb);
                     if (r == 1) hit = true;

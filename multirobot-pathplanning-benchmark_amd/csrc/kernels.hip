@@ -25,6 +25,10 @@ __device__ inline const double* row1(const EdgeBatchView& B, long long e) {
     long long r = B.idx1 ? (long long)B.idx1[e] : e;
     return B.q1 + r * B.rs;
 }
+__device__ inline const double* row1(const ArmQueue& B, long long e) {
+    long long r = B.idx1 ? (long long)B.idx1[e] : e;
+    return B.q1 + r * B.rs;
+}
 __device__ inline const double* row2(const EdgeBatchView& B, long long e) {
     long long r = B.idx2 ? (long long)B.idx2[e] : e;
     return B.q2 + r * B.rs;
@@ -32,6 +36,7 @@ __device__ inline const double* row2(const EdgeBatchView& B, long long e) {
 
 __device__ inline int edge_N_dev(const EdgeBatchView& B, long long e, int D) {
     if (B.N) return B.N[e];
+    if (B.N0 > 0) return B.N0;
     const double* a = row1(B, e);
     const double* b = row2(B, e);
     double m = __builtin_fabs(a[0] - b[0]);
@@ -88,8 +93,9 @@ __device__ inline void prep_load_rows(const EdgeBatchView& B, long long e, int D
     }
 }
 
+// nm > 0: mode ids outside [0, nm) make the edge an error (verdict 0, counted in meta[2])
 __global__ void __launch_bounds__(256) k_plan_prep(EdgeBatchView B, PlanWork W, int D, int lgG,
-                                                   int want_dir, int lg_lpe) {
+                                                   int want_dir, int lg_lpe, int nm_modes) {
     __shared__ int h[PLAN_LDS_BINS];
     __shared__ int smax;
     for (int i = threadIdx.x; i < PLAN_LDS_BINS; i += blockDim.x) h[i] = 0;
@@ -101,7 +107,7 @@ __global__ void __launch_bounds__(256) k_plan_prep(EdgeBatchView B, PlanWork W, 
     const long long per_grid = (long long)gridDim.x * (blockDim.x >> lg_lpe);
     // the lanes of a group share e, so they run the same iterations and their shuffles stay within
     // active lanes
-    const bool need_rows = !B.N || (want_dir && W.dir);
+    const bool need_rows = (!B.N && B.N0 <= 0) || (want_dir && W.dir);
     const long long e_first = (long long)blockIdx.x * (blockDim.x >> lg_lpe) + (threadIdx.x >> lg_lpe);
     double an[PREP_MAXC], bn[PREP_MAXC];  // rows of the next edge
     if (need_rows && e_first < B.n) prep_load_rows(B, e_first, D, sub, lg_lpe, an, bn);
@@ -114,8 +120,8 @@ __global__ void __launch_bounds__(256) k_plan_prep(EdgeBatchView B, PlanWork W, 
         }
         if (need_rows && e + per_grid < B.n) prep_load_rows(B, e + per_grid, D, sub, lg_lpe, an, bn);
         int N;
-        if (B.N) {
-            N = B.N[e];
+        if (B.N || B.N0 > 0) {
+            N = B.N ? B.N[e] : B.N0;
         } else {
             // the sequential rule (edge_N_dev): m = |a0 - b0|, then m = v > m ? v : m, so a NaN
             // in coordinate 0 sticks and a NaN elsewhere is skipped.  Lane 0 starts from
@@ -148,9 +154,16 @@ __global__ void __launch_bounds__(256) k_plan_prep(EdgeBatchView B, PlanWork W, 
             }
         }
         if (sub != 0) continue;
-        const int ns = B.N_start ? B.N_start[e] : 0;
-        const int nm = B.N_max ? B.N_max[e] : -1;
+        const int ns = batch_N_start(B, e);
+        const int nm = batch_N_max(B, e);
         EdgeWin w = edge_window(N, ns, nm, B.incl);
+        if (nm_modes > 0) {
+            const int md = batch_mode(B, e);
+            if (md < 0 || md >= nm_modes) {
+                w.err = 1;
+                w.M = 0;
+            }
+        }
         int M = w.M;
         long long S = ((long long)M + (1 << lgG) - 1) >> lgG;
         if (w.err) atomicAdd((unsigned long long*)&W.meta[2], 1ull);
@@ -159,11 +172,7 @@ __global__ void __launch_bounds__(256) k_plan_prep(EdgeBatchView B, PlanWork W, 
             W.long_list[slot] = (int)e;
             S = 0;
         }
-        W.eN[e] = N;
-        W.eM[e] = M;
-        W.eS[e] = (int)S;
-        W.ew0[e] = w.w0;
-        W.efull[e] = (unsigned char)w.full;
+        W.erec[e] = make_int4(N, M, (int)S, w.w0 | (w.full << 31));
         B.out[e] = w.err ? 0 : 1;
         if (S < PLAN_LDS_BINS) atomicAdd(&h[S], 1);
         else atomicAdd(&W.hist[S], 1);
@@ -261,7 +270,7 @@ __global__ void __launch_bounds__(256) k_plan_scatter(PlanWork W, long long n) {
 #pragma unroll
         for (int k = 0; k < SCATTER_EPT; ++k) {
             const long long e = t0 + (long long)k * blockDim.x + threadIdx.x;
-            S[k] = e < n ? W.eS[e] : 0;
+            S[k] = e < n ? W.erec[e].z : 0;
             r[k] = (S[k] > 0 && S[k] < PLAN_LDS_BINS) ? atomicAdd(&cnt[S[k]], 1) : 0;
         }
         __syncthreads();
@@ -284,7 +293,8 @@ __global__ void __launch_bounds__(256) k_plan_scatter(PlanWork W, long long n) {
 // chunks in increasing order, so all counters advance through the passes together; when its
 // counter is exhausted it takes chunks from the others.  Returns the chunk, or -1 when every
 // counter is exhausted.  Must be called by all lanes of the wave (wave-uniform result).
-__device__ inline long long claim_chunk(const PlanWork& W, int& q, long long nchunks) {
+template <class WV>
+__device__ inline long long claim_chunk(const WV& W, int& q, long long nchunks) {
     const int home = blockIdx.x & 7;
     while (q < 8) {
         const int x = (home + q) & 7;
@@ -321,18 +331,19 @@ __device__ inline SlotRef slot_ref(const PlanWork& W, int& p, long long s, int l
         ref.e = -1;
         return ref;
     }
-    const int S = W.eS[ref.e];
-    ref.local = k * S + p;
-    if (ref.local >= W.eM[ref.e]) ref.e = -1;
+    const int4 rec = W.erec[ref.e];
+    ref.local = k * rec.z + p;
+    if (ref.local >= rec.y) ref.e = -1;
     return ref;
 }
 
 __device__ inline EdgeWin win_of(const PlanWork& W, int e, int incl) {
+    const int4 rec = W.erec[e];
     EdgeWin w;
-    w.N = W.eN[e];
-    w.w0 = W.ew0[e];
-    w.M = W.eM[e];
-    w.full = W.efull[e];
+    w.N = rec.x;
+    w.w0 = erec_w0(rec);
+    w.M = rec.y;
+    w.full = erec_full(rec);
     w.first = incl ? 0 : 1;
     w.err = 0;
     return w;
@@ -450,9 +461,7 @@ __global__ void __launch_bounds__(64) k_lane_direct(View S, EdgeBatchView B, con
         const long long e = list ? (long long)list[li] : li;
         if (init_out && lane == 0) B.out[e] = 1;  // only with wpe == 1
         const int N = edge_N_dev(B, e, D);
-        const int ns = B.N_start ? B.N_start[e] : 0;
-        const int nm = B.N_max ? B.N_max[e] : -1;
-        EdgeWin w = edge_window(N, ns, nm, B.incl);
+        EdgeWin w = edge_window(N, batch_N_start(B, e), batch_N_max(B, e), B.incl);
         if (w.err) { B.out[e] = 0; continue; }
         const double* a = row1(B, e);
         const double* b = row2(B, e);
@@ -516,13 +525,17 @@ __device__ inline double arm_load_state(const EdgeBatchView& B, long long e, int
     return qv;
 }
 
-__device__ inline void arm_stats(unsigned long long* stats, unsigned rechecks, const ArmCount& cnt) {
-    if (!stats) return;
-    stat_add(stats, 1, rechecks);
-    stat_add(stats, 4, cnt.lb);
-    stat_add(stats, 5, cnt.sb);
-    stat_add(stats, 6, cnt.lp);
-    stat_add(stats, 7, cnt.sp);
+template <bool STATS>
+__device__ inline void arm_stats(unsigned long long* stats, const RecheckT<STATS>& rechecks,
+                                 const ArmCountT<STATS>& cnt) {
+    if constexpr (STATS) {
+        if (!stats) return;
+        stat_add(stats, 1, rechecks);
+        stat_add(stats, 4, cnt.lb);
+        stat_add(stats, 5, cnt.sb);
+        stat_add(stats, 6, cnt.lp);
+        stat_add(stats, 7, cnt.sp);
+    }
 }
 
 // Persistent arm kernel.  A wave claims a chunk of `spt` (<= 64) consecutive slots (after chunks
@@ -531,18 +544,19 @@ __device__ inline void arm_stats(unsigned long long* stats, unsigned rechecks, c
 // the wave then evaluates the live states one after another, k-major.  The ticket of the next
 // chunk is requested before those evaluations and read after them, so the atomic's round trip
 // overlaps the work.
-template <bool GRASP, bool CHAIN>
-__device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView& B, const PlanWork& W,
-                                           int lgG, int spt) {
+template <bool GRASP, bool CHAIN, bool STATS>
+__device__ inline void arm_persistent_body(const ArmK& V, const ArmQueue& B, const ArmQueue& W, int lgG,
+                                           int spt) {
     extern __shared__ double lds_arm[];
     arm_stage_scene(lds_arm, V);
     const ArmLds Wl = arm_lds(lds_arm, V);
     const ArmRegs R = arm_regs(V, Wl);
     const int lane = wave_lane();
     const long long T = W.meta[0];
-    unsigned long long n_eval = 0, n_dead = 0, n_slots = 0;
-    unsigned rechecks = 0;
-    ArmCount cnt = {0, 0, 0, 0};
+    // statistics (compiled away without STATS)
+    typename std::conditional<STATS, unsigned long long, NoCount>::type n_eval{}, n_dead{}, n_slots{};
+    RecheckT<STATS> rechecks{};
+    ArmCountT<STATS> cnt{};
     int p = 0;  // pass hint (wave-uniform)
     int q = 0;  // queues exhausted so far
     // outcomes of the wave's last 16 states (1: collision); mostly colliding -> lazy placement
@@ -582,10 +596,11 @@ __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView
             kl = k;
             const int ee = (off >= 0 && r < B.n) ? W.order[r] : -1;
             if (ee >= 0 && ee < B.n) {
-                const int S = W.eS[ee], Mv = W.eM[ee], Nv = W.eN[ee], w0 = W.ew0[ee];
-                const int full = W.efull[ee];
+                const int4 rec = W.erec[ee];
+                const int S = rec.z, Mv = rec.y, Nv = rec.x, w0 = erec_w0(rec);
+                const int full = erec_full(rec);
                 const unsigned char o = B.out[ee];
-                const int md0 = B.modes ? B.modes[ee] : 0;
+                const int md0 = batch_mode(B, ee);
                 const int local = k * S + pl;
                 if (local < Mv) {
                     if (o == 0) {
@@ -683,8 +698,8 @@ __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView
             }
             const double qv = interp_dir_f64(aj, dj, ij);  // lanes >= D: unused
             ++n_eval;
-            const bool col = arm_state_collides<GRASP, CHAIN>(V, Wl, R, qv, mj, B.mask, rechecks, cnt,
-                                                              __builtin_popcount(recent & 0xffffu) >= 8);
+            const bool col = arm_state_collides<GRASP, CHAIN, STATS>(V, Wl, R, qv, mj, B.mask, rechecks, cnt,
+                                                                     __builtin_popcount(recent & 0xffffu) >= 8);
             recent = (recent << 1) | (col ? 1u : 0u);
             if (col) {
                 if (lane == 0) B.out[ej] = 0;
@@ -701,40 +716,29 @@ __device__ inline void arm_persistent_body(const ArmView& V, const EdgeBatchView
             nb = 1;
         }
     }
-    if (W.stats && lane == 0) {
-        stat_add(W.stats, 0, n_eval);
-        stat_add(W.stats, 2, n_dead);
-        stat_add(W.stats, 3, n_slots);
+    if constexpr (STATS) {
+        if (W.stats && lane == 0) {
+            stat_add(W.stats, 0, n_eval);
+            stat_add(W.stats, 2, n_dead);
+            stat_add(W.stats, 3, n_slots);
+        }
     }
-    arm_stats(W.stats, rechecks, cnt);
+    arm_stats<STATS>(W.stats, rechecks, cnt);
 }
 
-// Scenes without held boxes fit 4 waves per SIMD (<= 128 VGPRs); the grasp variant 3.  DH joints
-// (k_arm_persistent, _grasp) and chain joints (_chain, _chain_grasp) are separate instantiations.
-__global__ void __launch_bounds__(256, 4) k_arm_persistent(ArmView V, EdgeBatchView B, PlanWork W, int lgG,
-                                                          int spt) {
-    arm_persistent_body<false, false>(V, B, W, lgG, spt);
-}
-
-__global__ void __launch_bounds__(256, 3) k_arm_persistent_grasp(ArmView V, EdgeBatchView B, PlanWork W,
-                                                                int lgG, int spt) {
-    arm_persistent_body<true, false>(V, B, W, lgG, spt);
-}
-
-__global__ void __launch_bounds__(256, 4) k_arm_persistent_chain(ArmView V, EdgeBatchView B, PlanWork W,
-                                                                int lgG, int spt) {
-    arm_persistent_body<false, true>(V, B, W, lgG, spt);
-}
-
-__global__ void __launch_bounds__(256, 3) k_arm_persistent_chain_grasp(ArmView V, EdgeBatchView B, PlanWork W,
-                                                                      int lgG, int spt) {
-    arm_persistent_body<true, true>(V, B, W, lgG, spt);
+// Scenes without held boxes fit 4 waves per SIMD (<= 128 VGPRs); the grasp variant 3.  DH and chain
+// joints, held boxes and statistics are template parameters.
+template <bool GRASP, bool CHAIN, bool STATS>
+__global__ void __launch_bounds__(256, GRASP ? 3 : 4) k_arm_persistent(ArmK V, ArmQueue Q, int lgG, int spt) {
+    arm_persistent_body<GRASP, CHAIN, STATS>(V, Q, Q, lgG, spt);
 }
 
 // Work item k of n_list * wpe: edge list[k / wpe] (or k / wpe), states k % wpe, k % wpe + wpe, ...
 // With init_out (wpe == 1 only) the wave owns its edge: it writes the verdict without reading it.
+// A mode id outside [0, nm) makes the edge's verdict 0 (the error contract of mujoco_env.py:797-807,
+// "exception => collision"); entry points with host-side mode ids reject them before launching.
 template <bool GRASP, bool CHAIN>
-__global__ void __launch_bounds__(256) k_arm_direct(ArmView V, EdgeBatchView B, const int* list,
+__global__ void __launch_bounds__(256) k_arm_direct(ArmK V, EdgeBatchView B, const int* list,
                                                     long long n_list, const long long* n_dev,
                                                     int wpe, int init_out,
                                                     unsigned long long* stats) {
@@ -745,7 +749,7 @@ __global__ void __launch_bounds__(256) k_arm_direct(ArmView V, EdgeBatchView B, 
     const int nw = blockDim.x >> 6;
     unsigned long long n_eval = 0;
     unsigned rechecks = 0;
-    ArmCount cnt = {0, 0, 0, 0};
+    ArmCountT<true> cnt = {0, 0, 0, 0};
     if (n_dev) n_list = *n_dev;
     const long long nitems = n_list * wpe;
     const long long w0 = (long long)blockIdx.x * nw + (threadIdx.x >> 6);
@@ -753,13 +757,11 @@ __global__ void __launch_bounds__(256) k_arm_direct(ArmView V, EdgeBatchView B, 
         const long long li = it / wpe;
         const int u = (int)(it % wpe);
         const long long e = list ? (long long)list[li] : li;
-        const int mode = B.modes ? B.modes[e] : 0;
+        const int mode = batch_mode(B, e);
         const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
         const int N = edge_N_dev(B, e, V.D);
-        const int ns = B.N_start ? B.N_start[e] : 0;
-        const int nm = B.N_max ? B.N_max[e] : -1;
-        EdgeWin w = edge_window(N, ns, nm, B.incl);
-        if (w.err) {
+        EdgeWin w = edge_window(N, batch_N_start(B, e), batch_N_max(B, e), B.incl);
+        if (w.err || md != mode) {
             if (wave_lane() == 0) B.out[e] = 0;
             continue;
         }
@@ -770,7 +772,7 @@ __global__ void __launch_bounds__(256) k_arm_direct(ArmView V, EdgeBatchView B, 
             if (i < 0) continue;
             const double qv = arm_load_state(B, e, N, i, arm_mact(V, Wl, md) * V.dof);
             ++n_eval;
-            if (arm_state_collides<GRASP, CHAIN>(V, Wl, R, qv, mode, B.mask, rechecks, cnt)) {
+            if (arm_state_collides<GRASP, CHAIN, true>(V, Wl, R, qv, mode, B.mask, rechecks, cnt)) {
                 hit = true;
                 break;
             }
@@ -781,11 +783,11 @@ __global__ void __launch_bounds__(256) k_arm_direct(ArmView V, EdgeBatchView B, 
         }
     }
     if (stats && wave_lane() == 0) stat_add(stats, 0, n_eval);
-    arm_stats(stats, rechecks, cnt);
+    arm_stats<true>(stats, rechecks, cnt);
 }
 
 template <bool GRASP, bool CHAIN>
-__global__ void __launch_bounds__(256, 3) k_arm_configs(ArmView V, const double* qv, long long n,
+__global__ void __launch_bounds__(256, 3) k_arm_configs(ArmK V, const double* qv, long long n,
                                                      long long rs, long long ds, const int* modes,
                                                      unsigned mask, unsigned char* out) {
     extern __shared__ double lds_arm[];
@@ -794,17 +796,93 @@ __global__ void __launch_bounds__(256, 3) k_arm_configs(ArmView V, const double*
     const ArmRegs R = arm_regs(V, Wl);
     const int nw = blockDim.x >> 6;
     const int lane = wave_lane();
-    unsigned rechecks = 0;
-    ArmCount cnt = {0, 0, 0, 0};
+    NoCount rechecks;
+    ArmCountT<false> cnt;
     for (long long r = (long long)blockIdx.x * nw + (threadIdx.x >> 6); r < n;
          r += (long long)gridDim.x * nw) {
         const int mode = modes ? modes[r] : 0;
         const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
         const int Dact = arm_mact(V, Wl, md) * V.dof;
         const double q = lane < Dact ? qv[r * rs + (long long)lane * ds] : 0.0;
-        const bool c = arm_state_collides<GRASP, CHAIN>(V, Wl, R, q, mode, mask, rechecks, cnt);
+        // a mode outside [0, nm): verdict 0 (arm_state_collides reports it as a collision)
+        const bool c = arm_state_collides<GRASP, CHAIN, false>(V, Wl, R, q, mode, mask, rechecks, cnt);
         if (lane == 0) out[r] = c ? 0 : 1;
     }
+}
+
+// Plain per-call form: one launch checks edge 0 of B with one workgroup.  Its waves split the
+// edge's states (wave w: slots w, w + nw, ...) and share the verdict through LDS: a wave that finds a
+// collision raises the flag, the others stop before their next state, and thread 0 writes the
+// verdict once every wave is done, so the output needs no initialisation.
+template <bool GRASP, bool CHAIN>
+__global__ void __launch_bounds__(1024) k_arm_edge(ArmK V, EdgeBatchView B) {
+    extern __shared__ double lds_arm[];
+    __shared__ int s_hit;
+    if (threadIdx.x == 0) s_hit = 0;
+    arm_stage_scene(lds_arm, V);  // ends with a barrier
+    const ArmLds Wl = arm_lds(lds_arm, V);
+    const ArmRegs R = arm_regs(V, Wl);
+    const int nw = blockDim.x >> 6, wv = threadIdx.x >> 6;
+    const int mode = batch_mode(B, 0);
+    const bool bad_mode = mode < 0 || mode >= V.nm;
+    const int N = edge_N_dev(B, 0, V.D);
+    const EdgeWin w = edge_window(N, batch_N_start(B, 0), batch_N_max(B, 0), B.incl);
+    NoCount rechecks;
+    ArmCountT<false> cnt;
+    if (!w.err && !bad_mode) {
+        const int Dact = arm_mact(V, Wl, mode) * V.dof;
+        volatile int* hit = &s_hit;
+        for (int local = wv; local < w.M; local += nw) {
+            if (*hit) break;  // wave-uniform: one LDS word
+            const int i = edge_state_index(w, local, B.incl);
+            if (i < 0) continue;
+            const double qv = arm_load_state(B, 0, N, i, Dact);
+            if (arm_state_collides<GRASP, CHAIN, false>(V, Wl, R, qv, mode, B.mask, rechecks, cnt)) {
+                if (wave_lane() == 0) *hit = 1;
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) B.out[0] = (w.err || bad_mode || s_hit) ? 0 : 1;
+}
+
+// Plain per-call form of the lane-per-state scenes: one workgroup of 4 waves, one state per lane,
+// 256 states per round; the verdict through LDS as in k_arm_edge.
+template <class View, int T1, int T2>
+__global__ void __launch_bounds__(256) k_lane_edge(View S, EdgeBatchView B) {
+    constexpr int MAXD = LaneDim<View, T1, T2>::value;
+    __shared__ int s_hit;
+    if (threadIdx.x == 0) s_hit = 0;
+    __syncthreads();
+    const int D = S.D;
+    const int N = edge_N_dev(B, 0, D);
+    const EdgeWin w = edge_window(N, batch_N_start(B, 0), batch_N_max(B, 0), B.incl);
+    unsigned rechecks = 0;
+    if (!w.err) {
+        const double* a = row1(B, 0);
+        const double* b = row2(B, 0);
+        volatile int* hit = &s_hit;
+        for (int base = 0; base < w.M; base += blockDim.x) {
+            if (*hit) break;  // the same LDS word for every lane
+            const int local = base + (int)threadIdx.x;
+            bool coll = false;
+            if (local < w.M) {
+                const int i = edge_state_index(w, local, B.incl);
+                if (i >= 0) {
+                    double q[MAXD];
+#pragma unroll
+                    for (int k = 0; k < MAXD; ++k)
+                        if (k < D) q[k] = interp_f64(a[k * B.ds], b[k * B.ds], N, i);
+                    coll = lane_collides<View, T1, T2>(S, q, B.mask, rechecks);
+                }
+            }
+            if (__any(coll)) *hit = 1;
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) B.out[0] = (w.err || s_hit) ? 0 : 1;
 }
 
 // binary64 forward kinematics of every link, out[r][L][12]
@@ -858,7 +936,7 @@ void launch_plan_build(const SceneRec& sc, const EdgeBatchView& B, const PlanWor
     if (gp > 1024) gp = 1024;
     int gs = grid_for(B.n, 256 * SCATTER_EPT);
     if (gs > 1024) gs = 1024;
-    k_plan_prep<<<gp, 256, 0, st>>>(B, W, sc.D, lgG, want_dir, lg_lpe);
+    k_plan_prep<<<gp, 256, 0, st>>>(B, W, sc.D, lgG, want_dir, lg_lpe, sc.kind == KIND_ARMS ? sc.arm.nm : 0);
     k_plan_scan<<<1, 1024, 0, st>>>(W, lgG);
     k_plan_scatter<<<gs, 256, 0, st>>>(W, B.n);
 }
@@ -882,16 +960,39 @@ void launch_plan_run(const SceneRec& sc, const EdgeBatchView& B, const PlanWork&
             // grid counts waves; workgroups of arm_nw waves
             const int nw = sc.arm_nw;
             const int blocks = (grid + nw - 1) / nw;
+            const bool stats = W.stats != nullptr;
+            ArmQueue Q;
+            Q.q1 = B.q1;
+            Q.idx1 = B.idx1;
+            Q.modes = B.modes;
+            Q.out = B.out;
+            Q.erec = W.erec;
+            Q.order = W.order;
+            Q.pass_start = W.pass_start;
+            Q.ticket = W.ticket;
+            Q.dir = W.dir;
+            Q.stats = W.stats;
+            Q.meta = W.meta;
+            Q.rs = B.rs;
+            Q.ds = (int)B.ds;
+            Q.n = (int)B.n;
+            Q.md0 = B.md0;
+            Q.incl = B.incl;
+            Q.mask = B.mask;
+#define MRCC_ARM_PERSISTENT(G, C)                                                                     \
+    do {                                                                                          \
+        if (stats) k_arm_persistent<G, C, true><<<blocks, 64 * nw, sc.arm_lds, st>>>(sc.armk, Q, lgG, spt);  \
+        else k_arm_persistent<G, C, false><<<blocks, 64 * nw, sc.arm_lds, st>>>(sc.armk, Q, lgG, spt);      \
+    } while (0)
             if (sc.arm.chain) {
-                if (sc.arm_has_grasp)
-                    k_arm_persistent_chain_grasp<<<blocks, 64 * nw, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
-                else
-                    k_arm_persistent_chain<<<blocks, 64 * nw, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
+                if (sc.arm_has_grasp) MRCC_ARM_PERSISTENT(true, true);
+                else MRCC_ARM_PERSISTENT(false, true);
             } else if (sc.arm_has_grasp) {
-                k_arm_persistent_grasp<<<blocks, 64 * nw, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
+                MRCC_ARM_PERSISTENT(true, false);
             } else {
-                k_arm_persistent<<<blocks, 64 * nw, sc.arm_lds, st>>>(sc.arm, B, W, lgG, spt);
+                MRCC_ARM_PERSISTENT(false, false);
             }
+#undef MRCC_ARM_PERSISTENT
             break;
         }
     }
@@ -931,7 +1032,7 @@ void launch_direct(const SceneRec& sc, const EdgeBatchView& B, const int* list, 
             const int nw = grid < sc.arm_nw ? 1 : sc.arm_nw;
             const size_t lds = nw == 1 ? sc.arm_lds1 : sc.arm_lds;
             const int blocks = (grid + nw - 1) / nw;
-#define MRCC_ARM_DIRECT(G, C) k_arm_direct<G, C><<<blocks, 64 * nw, lds, st>>>(sc.arm, B, list, n_list, n_dev, wpe, init_out, stats)
+#define MRCC_ARM_DIRECT(G, C) k_arm_direct<G, C><<<blocks, 64 * nw, lds, st>>>(sc.armk, B, list, n_list, n_dev, wpe, init_out, stats)
             if (sc.arm.chain) {
                 if (sc.arm_has_grasp) MRCC_ARM_DIRECT(true, true);
                 else MRCC_ARM_DIRECT(false, true);
@@ -975,7 +1076,7 @@ void launch_configs(const SceneRec& sc, const double* q, long long n, long long 
             const int nw = g < sc.arm_nw ? 1 : sc.arm_nw;
             const size_t lds = nw == 1 ? sc.arm_lds1 : sc.arm_lds;
             const int blocks = (int)((g + nw - 1) / nw);
-#define MRCC_ARM_CONFIGS(G, C) k_arm_configs<G, C><<<blocks, 64 * nw, lds, st>>>(sc.arm, q, n, rs, ds, modes, mask, out)
+#define MRCC_ARM_CONFIGS(G, C) k_arm_configs<G, C><<<blocks, 64 * nw, lds, st>>>(sc.armk, q, n, rs, ds, modes, mask, out)
             if (sc.arm.chain) {
                 if (sc.arm_has_grasp) MRCC_ARM_CONFIGS(true, true);
                 else MRCC_ARM_CONFIGS(false, true);
@@ -985,6 +1086,38 @@ void launch_configs(const SceneRec& sc, const double* q, long long n, long long 
                 MRCC_ARM_CONFIGS(false, false);
             }
 #undef MRCC_ARM_CONFIGS
+            break;
+        }
+    }
+}
+
+void launch_edge1(const SceneRec& sc, const EdgeBatchView& B, hipStream_t st) {
+    switch (sc.kind) {
+        case KIND_DISKS:
+            if (sc.disk.rd == 2 && sc.disk.nr == 2)
+                k_lane_edge<DiskView, 2, 2><<<1, 256, 0, st>>>(sc.disk, B);
+            else
+                k_lane_edge<DiskView, 0, 0><<<1, 256, 0, st>>>(sc.disk, B);
+            break;
+        case KIND_PLANAR:
+            if (sc.planar.nr == 4)
+                k_lane_edge<PlanarView, 4, 0><<<1, 256, 0, st>>>(sc.planar, B);
+            else
+                k_lane_edge<PlanarView, 0, 0><<<1, 256, 0, st>>>(sc.planar, B);
+            break;
+        case KIND_ARMS: {
+            const int nw = sc.arm_nw_edge;
+            const size_t lds = sc.arm_lds_edge;
+#define MRCC_ARM_EDGE(G, C) k_arm_edge<G, C><<<1, 64 * nw, lds, st>>>(sc.armk, B)
+            if (sc.arm.chain) {
+                if (sc.arm_has_grasp) MRCC_ARM_EDGE(true, true);
+                else MRCC_ARM_EDGE(false, true);
+            } else if (sc.arm_has_grasp) {
+                MRCC_ARM_EDGE(true, false);
+            } else {
+                MRCC_ARM_EDGE(false, false);
+            }
+#undef MRCC_ARM_EDGE
             break;
         }
     }
