@@ -211,6 +211,18 @@ int mrcc_check_edges_host(const mrcc_scene* scene, const double* q1, const doubl
 /* The plain per-call form applied to every edge of a batch: n_edges launches of the
  * mrcc_check_edge kernel (host loop, no Python in between). */
 int mrcc_check_edges_per_call(const mrcc_scene* scene, const mrcc_edge_batch* batch, void* stream);
+/* The batched check of mrcc_check_edges without any device allocation.  No counterpart in the
+ * reference: it exists for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The plan workspace is a file-scope device
+ * array of 16 MiB per GPU (kept until the process ends): (24 + 8 D) bytes per edge plus about 264
+ * KiB of tables, so about 65,000 edges of 28 coordinates; a larger batch is rejected with
+ * MRCC_ERR_INVALID.  Stream-ordered and asynchronous (nothing synchronises, nothing is allocated):
+ * edges the reference would reject (N_start > N, more than 2^24 states, an unknown mode id) get
+ * verdict 0, and when n_invalid (device int64) is given their count is written there.  tune: NULL
+ * or a persistent shape without statistics or timing (use_graph is ignored).  Every call on one GPU
+ * shares that GPU's workspace: calls that overlap on one GPU (other streams or threads) race. */
+int mrcc_check_edges_noalloc(const mrcc_scene* scene, const mrcc_edge_batch* batch,
+                             const mrcc_tune* tune, int64_t* n_invalid, void* stream);
 
 /* ---- host helpers (no device work) ------------------------------------------------------- */
 /* generate_binary_search_indices(N) (planning_env.py:34-51) into out[N]. */

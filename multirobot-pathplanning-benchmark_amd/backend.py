@@ -153,6 +153,14 @@ class Scene:
         check(lib().mrcc_check_edges(self.handle, ctypes.byref(batch), ctypes.byref(tune) if tune else None,
                                      stream if stream is not None else _stream_handle()))
 
+    def check_edges_noalloc_device(self, batch: EdgeBatch, n_invalid=None, tune: Optional[Tune] = None,
+                                   stream=None):
+        """mrcc_check_edges_noalloc: the batched check on the library's per-GPU workspace (no device
+        allocation, asynchronous); n_invalid: optional int64 device tensor [1]."""
+        check(lib().mrcc_check_edges_noalloc(self.handle, ctypes.byref(batch), ctypes.byref(tune) if tune else None,
+                                             ctypes.c_void_p(n_invalid.data_ptr()) if n_invalid is not None else None,
+                                             stream if stream is not None else _stream_handle()))
+
     def check_edges_per_call_device(self, batch: EdgeBatch, stream=None):
         check(lib().mrcc_check_edges_per_call(self.handle, ctypes.byref(batch),
                                               stream if stream is not None else _stream_handle()))

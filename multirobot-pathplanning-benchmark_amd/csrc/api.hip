@@ -420,9 +420,9 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
         }
     }
     if (sc->arm_nw == 0) return fail(MRCC_ERR_INVALID, "arm scene needs more than 64 KiB of LDS per wave");
-    // the per-call kernel: one workgroup of as many waves (up to 16) as 64 KiB of LDS holds
+    // the per-call kernel: one workgroup of as many waves (up to 8) as 64 KiB of LDS holds
     sc->arm_nw_edge = 1;
-    for (int nw = 16; nw >= 1; --nw)
+    for (int nw = 8; nw >= 1; --nw)
         if (arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, nw) <= 65536) {
             sc->arm_nw_edge = nw;
             break;
@@ -482,6 +482,82 @@ int check_batch(const SceneRec* sc, const mrcc_edge_batch* b) {
 
 }  // namespace
 
+// ---- shared with the no-allocation entry point (noalloc.hip) ----------------------------------
+namespace mrcc_internal {
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+int injected() { return ::injected(); }
+int device_cus() { return ::device_cus(); }
+const SceneRec* scene_rec(const mrcc_scene* s) { return rec_of(s); }
+int check_batch(const SceneRec* sc, const mrcc_edge_batch* b) { return ::check_batch(sc, b); }
+EdgeBatchView batch_view(const mrcc_edge_batch* b) { return view_of(b); }
+
+int tune_defaults(const SceneRec* sc, const mrcc_tune* tune, mrcc_tune* t) {
+    memset(t, 0, sizeof(*t));
+    if (tune) *t = *tune;
+    const bool arms = sc->kind == KIND_ARMS;
+    if (t->slots_per_pass <= 0) t->slots_per_pass = arms ? 4 : 8;
+    if (t->slots_per_ticket <= 0) t->slots_per_ticket = arms ? 16 : 4;
+    if (t->waves_per_cu <= 0) t->waves_per_cu = 16;
+    if (!tune) {
+        t->sorted = 1;
+        t->persistent = 1;
+        t->use_graph = 1;
+    }
+    if ((t->slots_per_pass & (t->slots_per_pass - 1)) || t->slots_per_pass > 64)
+        return ::fail(MRCC_ERR_INVALID, "slots_per_pass must be a power of two <= 64");
+    if (arms && t->slots_per_ticket > 64)  // one lane resolves one slot of a claimed chunk
+        return ::fail(MRCC_ERR_INVALID, "arm scenes: slots_per_ticket <= 64");
+    return MRCC_OK;
+}
+
+int plan_grid(const SceneRec* sc, const mrcc_tune& t) {
+    const int cus = ::device_cus();
+    return sc->kind == KIND_ARMS ? t.waves_per_cu * cus : std::max(1, t.waves_per_cu * cus / 4);
+}
+
+size_t plan_layout(long long n_edges, int D, unsigned char* base, PlanWork* W) {
+    const long long n = std::max<long long>(n_edges, 1);
+    size_t top = 0;
+    auto sz = [&](size_t bytes) {
+        const size_t off = (top + 255) & ~size_t(255);
+        top = off + bytes;
+        return off;
+    };
+    const size_t o_rec = sz(16 * n), o_order = sz(4 * n), o_hist = sz(4 * (MRCC_MAX_PASSES + 1)),
+                 o_cur = sz(4 * (MRCC_MAX_PASSES + 1)), o_ps = sz(8 * (MRCC_MAX_PASSES + 1)), o_meta = sz(8 * 8),
+                 o_long = sz(4 * n), o_tick = sz(8 * 32 * 8), o_stats = sz(8 * 8), o_dir = sz(8 * (size_t)n * D);
+    if (W) {
+        W->erec = (int4*)(base + o_rec);
+        W->order = (int*)(base + o_order);
+        W->hist = (int*)(base + o_hist);
+        W->cursor = (int*)(base + o_cur);
+        W->pass_start = (long long*)(base + o_ps);
+        W->meta = (long long*)(base + o_meta);
+        W->long_list = (int*)(base + o_long);
+        W->ticket = (unsigned long long*)(base + o_tick);
+        W->stats = (unsigned long long*)(base + o_stats);
+        W->dir = (double*)(base + o_dir);
+    }
+    return top;
+}
+
+void launch_plan_sequence(const SceneRec& sc, const EdgeBatchView& B, const PlanWork& W, int lgG, int spt,
+                          int grid, hipStream_t st) {
+    launch_plan_build(sc, B, W, lgG, st);
+    launch_plan_run(sc, B, W, lgG, spt, grid, st);
+    // edges with more passes than the queue holds: one workgroup per edge
+    launch_direct(sc, B, W.long_list, 0, W.meta + 3, 1, 0, 4 * ::device_cus(), W.stats, st);
+}
+}  // namespace mrcc_internal
+
 struct mrcc_edge_plan {
     SceneRec* sc;
     EdgeBatchView B;
@@ -500,11 +576,13 @@ struct mrcc_edge_plan {
 static void plan_launch_sequence(mrcc_edge_plan* p, hipStream_t st) {
     const bool ev = p->ev[0] != nullptr;
     if (ev) (void)hipEventRecord(p->ev[0], st);
-    if (p->tune.persistent) {
+    if (p->tune.persistent && !ev) {
+        mrcc_internal::launch_plan_sequence(*p->sc, p->B, p->W, p->lgG, p->tune.slots_per_ticket, p->grid, st);
+    } else if (p->tune.persistent) {
         launch_plan_build(*p->sc, p->B, p->W, p->lgG, st);
-        if (ev) (void)hipEventRecord(p->ev[1], st);
+        (void)hipEventRecord(p->ev[1], st);
         launch_plan_run(*p->sc, p->B, p->W, p->lgG, p->tune.slots_per_ticket, p->grid, st);
-        if (ev) (void)hipEventRecord(p->ev[2], st);
+        (void)hipEventRecord(p->ev[2], st);
         // edges with more passes than the queue holds: one workgroup per edge
         launch_direct(*p->sc, p->B, p->W.long_list, 0, p->W.meta + 3, 1, 0, 4 * device_cus(),
                       p->W.stats, st);
@@ -531,7 +609,7 @@ int mrcc_init(int device) {
     HIP_OK(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(MRCC_ERR_INVALID, "device %d of %d", device, n);
     HIP_OK(hipSetDevice(device));
-    return MRCC_OK;
+    return mrcc_noalloc_setup();  // per-device table of the no-allocation entry point
 }
 
 int mrcc_debug_inject_failure(int code, int count) {
@@ -825,54 +903,27 @@ int mrcc_edge_plan_create(const mrcc_scene* scene, const mrcc_edge_batch* batch,
     int rc = check_batch(sc, batch);
     if (rc != MRCC_OK) return rc;
     mrcc_tune t;
-    memset(&t, 0, sizeof(t));
-    if (tune) t = *tune;
-    const bool arms = sc->kind == KIND_ARMS;
-    if (t.slots_per_pass <= 0) t.slots_per_pass = arms ? 4 : 8;
-    if (t.slots_per_ticket <= 0) t.slots_per_ticket = arms ? 16 : 4;
-    if (t.waves_per_cu <= 0) t.waves_per_cu = arms ? 16 : 16;
-    if (!tune) {
-        t.sorted = 1;
-        t.persistent = 1;
-        t.use_graph = 1;
-    }
-    if ((t.slots_per_pass & (t.slots_per_pass - 1)) || t.slots_per_pass > 64)
-        return fail(MRCC_ERR_INVALID, "slots_per_pass must be a power of two <= 64");
-    if (arms && t.slots_per_ticket > 64)  // one lane resolves one slot of a claimed chunk
-        return fail(MRCC_ERR_INVALID, "arm scenes: slots_per_ticket <= 64");
+    rc = mrcc_internal::tune_defaults(sc, tune, &t);
+    if (rc != MRCC_OK) return rc;
     mrcc_edge_plan* p = new mrcc_edge_plan();
     memset(p, 0, sizeof(*p));
     p->sc = sc;
     p->B = view_of(batch);
     p->tune = t;
     p->lgG = __builtin_ctz((unsigned)t.slots_per_pass);
-    const int cus = device_cus();
-    p->grid = arms ? t.waves_per_cu * cus : std::max(1, t.waves_per_cu * cus / 4);
+    p->grid = mrcc_internal::plan_grid(sc, t);
     // workspace
-    const long long n = std::max<long long>(batch->n_edges, 1);
-    const int D = sc->D;
-    Packer P;  // sizes only
-    auto sz = [&](size_t bytes) { size_t off = (P.buf.size() + 255) & ~size_t(255); P.buf.resize(off + bytes); return off; };
-    size_t o_rec = sz(16 * n), o_order = sz(4 * n), o_hist = sz(4 * (MRCC_MAX_PASSES + 1)), o_cur = sz(4 * (MRCC_MAX_PASSES + 1)),
-           o_ps = sz(8 * (MRCC_MAX_PASSES + 1)), o_meta = sz(8 * 8), o_long = sz(4 * n), o_tick = sz(8 * 32 * 8),
-           o_stats = sz(8 * 8), o_dir = sz(8 * (size_t)n * D);
-    if (hipMalloc(&p->work_mem, P.buf.size()) != hipSuccess) {
+    const size_t work_bytes = mrcc_internal::plan_layout(batch->n_edges, sc->D, nullptr, nullptr);
+    if (hipMalloc(&p->work_mem, work_bytes) != hipSuccess) {
         delete p;
-        return fail(MRCC_ERR_NOMEM, "workspace of %zu bytes", P.buf.size());
+        return fail(MRCC_ERR_NOMEM, "workspace of %zu bytes", work_bytes);
     }
     unsigned char* w = (unsigned char*)p->work_mem;
-    p->W.erec = (int4*)(w + o_rec);
-    p->W.order = (int*)(w + o_order);
-    p->W.hist = (int*)(w + o_hist);
-    p->W.cursor = (int*)(w + o_cur);
-    p->W.pass_start = (long long*)(w + o_ps);
-    p->W.meta = (long long*)(w + o_meta);
-    p->W.long_list = (int*)(w + o_long);
-    p->W.ticket = (unsigned long long*)(w + o_tick);
-    p->W.stats = t.collect_stats ? (unsigned long long*)(w + o_stats) : nullptr;
-    p->W.dir = (double*)(w + o_dir);
-    if (t.collect_stats) HIP_OK(hipMemset(w + o_stats, 0, 64));
-    HIP_OK(hipMemset(w + o_meta, 0, 64));
+    mrcc_internal::plan_layout(batch->n_edges, sc->D, w, &p->W);
+    unsigned char* stats_mem = (unsigned char*)p->W.stats;
+    if (!t.collect_stats) p->W.stats = nullptr;
+    if (t.collect_stats) HIP_OK(hipMemset(stats_mem, 0, 64));
+    HIP_OK(hipMemset(p->W.meta, 0, 64));
     if (t.time_kernels && !t.use_graph)
         for (int i = 0; i < 4; ++i) HIP_OK(hipEventCreate(&p->ev[i]));
     if (t.use_graph && batch->n_edges > 0) {

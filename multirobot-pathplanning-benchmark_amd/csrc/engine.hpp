@@ -1,6 +1,7 @@
 // engine.hpp — host-side records of the collision engine and the kernel launchers.
 #pragma once
 
+#include "../../include/mrcc.h"
 #include "evaluators.hpp"
 
 // A scene as held by the library: the kind, its device view and the device allocation behind it.
@@ -70,3 +71,24 @@ struct CostArgs {
     int sl[2 * MRCC_COST_MAX_SLICES];
 };
 void launch_config_cost(const CostArgs& A, int max_grid, hipStream_t st);
+
+// api.hip internals shared with the no-allocation entry point (noalloc.hip)
+namespace mrcc_internal {
+int fail(int code, const char* fmt, ...);  // sets the thread's last-error message, returns code
+int injected();                            // an injected failure (mrcc_debug_inject_failure), or 0
+int device_cus();
+const SceneRec* scene_rec(const mrcc_scene* s);
+int check_batch(const SceneRec* sc, const mrcc_edge_batch* b);
+EdgeBatchView batch_view(const mrcc_edge_batch* b);
+// execution shape: defaults filled in and checked (mrcc_edge_plan_create's rules)
+int tune_defaults(const SceneRec* sc, const mrcc_tune* tune, mrcc_tune* t);
+int plan_grid(const SceneRec* sc, const mrcc_tune& t);
+// Bytes of a plan's workspace for n_edges edges of D coordinates; with W, lays it out from base.
+size_t plan_layout(long long n_edges, int D, unsigned char* base, PlanWork* W);
+// queue build, persistent kernel and long-edge fallback on stream st
+void launch_plan_sequence(const SceneRec& sc, const EdgeBatchView& B, const PlanWork& W, int lgG, int spt,
+                          int grid, hipStream_t st);
+}  // namespace mrcc_internal
+
+// noalloc.hip: looks up the current device's scratch array (called by mrcc_init)
+int mrcc_noalloc_setup();
