@@ -112,7 +112,7 @@ def roofline_all_free(args, backend, sc, desc, rng, dev, res, fl_state, pair_inf
 
     return {"achieved": achieved, "k_ms": k_ms, "f_ms": f_ms, "f_split": f_split, "exec_flops": exec_flops,
             "f_states": f_states, "free_rate": free_rate, "n_edges": len(fq1), "bf_achieved": bf_achieved,
-            "bf_ms": bf_ms, "fst": fst}
+            "bf_ms": bf_ms, "fst": fst, "fq1": fq1, "fq2": fq2}
 
 
 def main():
@@ -258,6 +258,18 @@ def main():
     per_call_match = bool(np.array_equal(out3.cpu().numpy().astype(bool), gpu_free[:m]))
     log(f"plain per-call: {m} launches in {pc_time * 1e3:.1f} ms -> {per_call_rate:.4g} edge-states/s; "
         f"single launch {single_rate:.4g}")
+    # the same per-call form as the planners reach it: MrccEnvironment.is_edge_collision_free from
+    # Python on host configurations (staging copy in, one launch, copy out, per call)
+    env = backend.MrccEnvironment(desc, collision_resolution=res)
+    mp = min(1000, m)
+    env.is_edge_collision_free(q1[0], q2[0], None)
+    t0 = time.perf_counter()
+    py_free = [env.is_edge_collision_free(q1[i], q2[i], None) for i in range(mp)]
+    py_time = time.perf_counter() - t0
+    py_rate = int(np.sum(N[:mp] - 2)) / py_time
+    py_match = bool(np.array_equal(np.array(py_free), gpu_free[:mp]))
+    log(f"per-call through MrccEnvironment.is_edge_collision_free: {py_time / mp * 1e6:.1f} us/call -> "
+        f"{py_rate:.4g} edge-states/s")
 
     # ---- configuration checks ----------------------------------------------------------------
     qc = torch.from_numpy(Qc).to(dev)
@@ -275,6 +287,18 @@ def main():
     # ---- all-free variant: roofline of the persistent kernel ----------------------------------
     rf = roofline_all_free(args, backend, sc, desc, rng, dev, res, fl_state, pair_info)
     achieved, k_ms, f_ms = rf["achieved"], rf["k_ms"], rf["f_ms"]
+    free_check = None
+    if not args.skip_cpu:
+        # the all-free set is certified by the oracle, not by the GPU that filtered it
+        from oracle import oracle as O
+
+        kf = min(8192, len(rf["fq1"]))
+        t0 = time.perf_counter()
+        fref, fn = O.Scene(desc).check_edges(rf["fq1"][:kf], rf["fq2"][:kf], res, threads=len(os.sched_getaffinity(0)),
+                                              count=True)
+        free_check = {"edges": kf, "states_evaluated_by_oracle": int(fn), "not_free_by_oracle": int(np.sum(~fref)),
+                      "seconds": time.perf_counter() - t0}
+        log(f"all-free set vs oracle: {free_check}")
 
     # ---- CPU baseline: the oracle on the host cores -------------------------------------------
     cpu = None
@@ -294,7 +318,7 @@ def main():
         reps = 0
         t0 = time.perf_counter()
         while True:  # repeat the sample until about cpu-seconds of work
-            ref = o.check_edges(q1[:k], q2[:k], res, threads=threads)
+            ref, n_phys = o.check_edges(q1[:k], q2[:k], res, threads=threads, count=True)
             reps += 1
             cpu_t = time.perf_counter() - t0
             if cpu_t >= args.cpu_seconds or reps >= 100:
@@ -305,6 +329,7 @@ def main():
                          f"FP64 oracle, OpenMP over edges, {cpu_t:.1f} s"}
         mism = int(np.sum(ref != gpu_free[:k]))
         checked += ks
+        cpu["compared"] = {"edges": k, "logical_edge_states": ks, "states_evaluated_by_oracle": int(n_phys)}
         # the same oracle on one thread, over a 2 s sample of the same edges
         k1 = max(1, min(k, int(k * 2.0 / (cpu_t / reps) / threads)))
         t0 = time.perf_counter()
@@ -355,10 +380,16 @@ def main():
                                         "run_split_ms": rf["f_split"],
                                         "note": "as shipped; flops are the tests it executes"}},
         "cpu_baseline": cpu,
-        "plain_per_call": {"value": per_call_rate, "unit": "edge-states/s", "edges": m, "matches": per_call_match},
+        "plain_per_call": {"value": per_call_rate, "unit": "edge-states/s", "edges": m, "matches": per_call_match,
+                           "us_per_edge": pc_time / m * 1e6,
+                           "note": "one kernel launch per edge on device rows (mrcc_check_edges_per_call)",
+                           "python_env": {"value": py_rate, "edges": mp, "us_per_call": py_time / mp * 1e6,
+                                          "matches": py_match,
+                                          "note": "MrccEnvironment.is_edge_collision_free per edge from Python, "
+                                                  "host rows (the unchanged planners' call)"}},
         "single_launch": {"value": single_rate, "unit": "edge-states/s", "matches": single_match},
         "all_free": {"value": rf["free_rate"], "unit": "edge-states/s", "edges": rf["n_edges"],
-                     "edge_states": rf["f_states"]},
+                     "edge_states": rf["f_states"], "oracle_check": free_check},
         "configs": {"value": cfg_rate, "unit": "configs/s", "n": len(Qc), "free_frac": float(cfg_free.mean())},
         "stats": stats,
         "run_split_ms": head_split,
@@ -411,9 +442,117 @@ def timed_split(plan, steps):
     return acc
 
 
+def lane_flops_per_state(desc):
+    """Algorithmic FP32 flops of one state of the lane-per-state scenes (FMA = 2, min/max/abs/cmp =
+    1; interpolation 2 per coordinate): disks 6 per robot pair and 10 per disk-rectangle test (C1:
+    8 + 6 + 40 * 10 = 414); planar boxes 50 per separating-axis test over robot pairs and robot x
+    obstacle pairs (C2: 24 + (6 + 400) * 50 = 20,324; the 4 binary64 sincos are not counted)."""
+    D = desc.dim
+    n = desc.n_robots
+    pairs = n * (n - 1) // 2
+    if desc.kind == "disks":
+        return 2 * D + 6 * pairs + 10 * n * len(desc.rect_center) + 6 * n * len(desc.sphere_radius)
+    return 2 * D + 50 * (pairs + n * len(desc.obstacle_pose))
+
+
+def lane_free_edges(rng, desc, sc, n, res, dev, short=0.2):
+    """n short edges of a lane-per-state scene whose every state is free (filtered on the GPU):
+    uniform starts, steps of at most `short` in the max metric."""
+    import torch
+
+    q1s, q2s, have = [], [], 0
+    while have < n:
+        m = 4 * n
+        a = scenes_uniform(rng, desc, m)
+        d = rng.uniform(-1, 1, size=(m, desc.dim))
+        d *= (rng.uniform(0.02, short, m) / np.max(np.abs(d), axis=1))[:, None]
+        b = np.clip(a + d, desc.limits[0], desc.limits[1])
+        A, B = torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev)
+        oa, ob, oe = (torch.empty(m, dtype=torch.uint8, device=dev) for _ in range(3))
+        sc.check_configs_device(A, oa)
+        sc.check_configs_device(B, ob)
+        sc.check_edges_device(sc.edge_batch(A, B, oe, res))
+        torch.cuda.synchronize()
+        ok = (oa.cpu().numpy() & ob.cpu().numpy() & oe.cpu().numpy()).astype(bool)
+        q1s.append(a[ok])
+        q2s.append(b[ok])
+        have += int(ok.sum())
+    return np.concatenate(q1s)[:n], np.concatenate(q2s)[:n]
+
+
+def scenes_uniform(rng, desc, m):
+    return rng.uniform(desc.limits[0], desc.limits[1], size=(m, desc.dim))
+
+
+def per_call_rate(sc, q1, q2, res, N, dev, m=4096):
+    """The plain per-call form: one launch per edge (mrcc_check_edges_per_call) over the first m
+    edges; logical edge-states per second and the verdicts."""
+    import torch
+
+    m = min(m, len(q1))
+    a, b = torch.from_numpy(q1[:m]).to(dev), torch.from_numpy(q2[:m]).to(dev)
+    o = torch.empty(m, dtype=torch.uint8, device=dev)
+    batch = sc.edge_batch(a, b, o, res)
+    sc.check_edges_per_call_device(batch)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sc.check_edges_per_call_device(batch)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return int(np.sum(N[:m] - 2)) / dt, o.cpu().numpy().astype(bool), m, dt / m * 1e6
+
+
+def lane_roofline(args, backend, sc, desc, rng, dev, res, threads, oracle_scene):
+    """Roofline of the persistent lane-per-state kernel on an all-free edge set (every state
+    evaluated, no early exit; the set is checked against the oracle), from HIP events around the
+    kernel; and, for the disk scene, of the configuration kernel (HBM: 8 D + 1 bytes per
+    configuration) on 2^22 configurations."""
+    import torch
+
+    fl = lane_flops_per_state(desc)
+    fq1, fq2 = lane_free_edges(rng, desc, sc, 65536, res, dev)
+    f_states, fN = logical_states(fq1, fq2, res)
+    fa, fb = torch.from_numpy(fq1).to(dev), torch.from_numpy(fq2).to(dev)
+    fo = torch.empty(len(fq1), dtype=torch.uint8, device=dev)
+    split = timed_split(backend.EdgePlan(sc, sc.edge_batch(fa, fb, fo, res),
+                                         backend.make_tune(use_graph=0, time_kernels=1)), args.steps)
+    ach = f_states * fl / (split["kernel_ms"] * 1e-3) / 1e12
+    ent = {"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+           "frac": ach / PEAK_FP32_TFLOPS, "flops_per_state": fl, "states_per_launch": f_states,
+           "edges": len(fq1), "ms_per_launch": split["kernel_ms"],
+           "kernel": "k_lane_persistent, all-free edges (every state evaluated)"}
+    if oracle_scene is not None:
+        k = min(len(fq1), 16384)
+        ref, n_ev = oracle_scene.check_edges(fq1[:k], fq2[:k], res, threads=threads, count=True)
+        ent["oracle_checked_edges"] = k
+        ent["oracle_states_evaluated"] = int(n_ev)
+        ent["oracle_mismatches"] = int(np.sum(~ref))  # the set is all free
+    if desc.kind == "disks":
+        n = 1 << 22
+        Q = torch.from_numpy(scenes_uniform(rng, desc, n)).to(dev)
+        o = torch.empty(n, dtype=torch.uint8, device=dev)
+        sc.check_configs_device(Q, o)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(args.steps):
+            sc.check_configs_device(Q, o)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        nbytes = n * (8 * desc.dim + 1)
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        ent["configs"] = {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                          "frac": gbs / PEAK_HBM_GBS, "configs": n, "bytes_per_launch": nbytes, "ms": ms,
+                          "valu_tflops": n * (fl - 2 * desc.dim) / (ms * 1e-3) / 1e12,
+                          "kernel": "k_lane_configs<DiskView, 2, 2>"}
+    return ent
+
+
 def secondary_workloads(args, backend, scenes, dev):
-    """C1 (the reference's abstract environment) and C2 (planar boxes): batched GPU rate, CPU oracle
-    on the host cores, and for C1 the numpy restatement of abstract_env.py called per edge."""
+    """C1 (the reference's abstract environment) and C2 (planar boxes): batched GPU rate, plain
+    per-call rate, all-free roofline, CPU oracle on the host cores, and for C1 the numpy
+    restatement of abstract_env.py called per edge."""
     import torch
     from oracle import oracle as O
     from oracle.abstract_np import AbstractNp
@@ -443,15 +582,22 @@ def secondary_workloads(args, backend, scenes, dev):
         torch.cuda.synchronize()
         gpu_rate = n_states / ((time.perf_counter() - t0) / reps)
         got = o_.cpu().numpy().astype(bool)
-        ent = {"edges": len(q1), "edge_states": n_states, "gpu_batched": gpu_rate, "unit": "edge-states/s"}
+        pc_rate, pc_got, pc_m, pc_us = per_call_rate(sc, q1, q2, res, N, dev)
+        ent = {"edges": len(q1), "edge_states": n_states, "gpu_batched": gpu_rate, "unit": "edge-states/s",
+               "gpu_per_call": {"value": pc_rate, "edges": pc_m, "us_per_edge": pc_us,
+                                "matches": bool(np.array_equal(pc_got, got[:pc_m])),
+                                "note": "one launch per edge (mrcc_check_edges_per_call)"}}
+        o = None if args.skip_cpu else O.Scene(desc)
+        ent["roofline"] = lane_roofline(args, backend, sc, desc, np.random.default_rng(11), dev, res, threads, o)
         if not args.skip_cpu:
-            o = O.Scene(desc)
             t0 = time.perf_counter()
-            ref = o.check_edges(q1, q2, res, threads=threads)  # every edge of the workload
+            ref, n_ev = o.check_edges(q1, q2, res, threads=threads, count=True)  # every edge of the workload
             dt = time.perf_counter() - t0
             ent["cpu_oracle"] = {"value": n_states / dt, "cores": threads, "edges": len(q1)}
             ent["mismatches"] = int(np.sum(ref != got))
             ent["checks_compared"] = n_states
+            ent["states_evaluated_compared"] = int(n_ev)
+            ent["edges_compared"] = len(q1)
             if gen == "pairs":
                 mirror = AbstractNp(desc)
                 t0 = time.perf_counter()
@@ -493,6 +639,28 @@ def secondary_workloads(args, backend, scenes, dev):
     ent = {"instances": n_inst, "edges": len(i1), "edge_states": n_states, "gpu_batched": gpu_rate,
            "unit": "edge-states/s", "free_frac": float(got.mean()),
            "autotuned": {"choice": best, "ms": tune_ms[best], "candidates": len(tune_ms)}}
+    # executed-flop roofline of the persistent kernel on this mixed batch (early exits and culling:
+    # the flops are the tests the kernel counts, never the brute-force count): kernel time from
+    # HIP events of a plan with the same shape, test counts from a plan with statistics
+    tt = backend.make_tune(tune.slots_per_pass, tune.slots_per_ticket, tune.waves_per_cu, use_graph=0,
+                           time_kernels=1)
+    split = timed_split(backend.EdgePlan(sc, batch, tt), 5)
+    cst = backend.EdgePlan(sc, batch, backend.make_tune(tune.slots_per_pass, tune.slots_per_ticket,
+                                                        tune.waves_per_cu, use_graph=0, collect_stats=1))
+    cst.run()
+    fst = cst.stats()
+    del cst
+    fl_full, pinfo = arm_flops_per_state(desc)
+    S_, L_, D_ = pinfo["spheres"], desc.n_arms * desc.dof, desc.dim
+    per_state_fixed = 2 * D_ + 72 * D_ + 18 * S_ + 18 * L_  # all 8 arms: an upper bound per state
+    ex = (fst["states"] * per_state_fixed + 15 * fst["link_box_tests"] + 15 * fst["sphere_box_tests"]
+          + 11 * fst["link_pair_tests"] + 11 * fst["sphere_pair_tests"])
+    ach = ex / (split["kernel_ms"] * 1e-3) / 1e12
+    ent["roofline_executed"] = {"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                                "frac": ach / PEAK_FP32_TFLOPS, "ms_per_launch": split["kernel_ms"],
+                                "run_split_ms": split, "stats": fst,
+                                "note": "executed tests counted by the kernel; kinematics of all 8 arms per "
+                                        "state (an upper bound for modes with fewer arms)"}
     # plain per-instance form: one single-launch check per instance (first 256 instances)
     k_inst = min(256, n_inst)
     k_e = int(np.searchsorted(inst, k_inst))
@@ -709,8 +877,49 @@ def clearance_workload(args, backend, scenes, dev, threads):
     torch.cuda.synchronize()
     fb = (time.perf_counter() - t0) / reps
     g = (q.grad / reps).detach().cpu().numpy().astype(np.float64)
+    # the two kernels alone: HIP events around the library calls on the current stream
+    lib = importlib.import_module(f"{PKG}._lib")
+    import ctypes
+    qf = q.detach().reshape(-1, D).contiguous()
+    n = qf.shape[0]
+    cost = torch.empty(n, dtype=torch.float32, device=dev)
+    gq = torch.empty_like(qf)
+    gc = torch.ones(n, dtype=torch.float32, device=dev)
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    fwd = lambda: lib.check(lib.lib().mrcc_clearance_forward(sc.handle, ctypes.c_void_p(qf.data_ptr()), n,
+                                                             ctypes.c_float(0.02), ctypes.c_void_p(cost.data_ptr()), st))
+    bwd = lambda: lib.check(lib.lib().mrcc_clearance_backward(sc.handle, ctypes.c_void_p(qf.data_ptr()), n,
+                                                              ctypes.c_float(0.02), ctypes.c_void_p(gc.data_ptr()),
+                                                              ctypes.c_void_p(gq.data_ptr()), st))
+    kms = {}
+    for name, f in (("forward", fwd), ("backward", bwd)):
+        f()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            f()
+        e1.record()
+        torch.cuda.synchronize()
+        kms[name] = e0.elapsed_time(e1) / reps
+    fl_fwd, pinfo = arm_flops_per_state(desc)
+    # backward: the forward's pair set again (recomputed) + per sphere and joint of its chain the
+    # cross product z x (c - o) and its dot with the force (about 18 flops; a dof/2 chain on average)
+    fl_bwd = fl_fwd + 18 * pinfo["spheres"] * desc.dof // 2
+    ach_f = n * fl_fwd / (kms["forward"] * 1e-3) / 1e12
+    ach_b = n * fl_bwd / (kms["backward"] * 1e-3) / 1e12
     ent = {"paths": P, "waypoints": P * W, "dof": D, "ms_forward_backward": fb * 1e3,
-           "waypoints_per_s": P * W / fb, "mean_cost": float(c.detach().mean())}
+           "waypoints_per_s": P * W / fb, "mean_cost": float(c.detach().mean()),
+           "kernel_ms": kms,
+           "roofline": {"bound": "valu", "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                        "forward": {"achieved": ach_f, "frac": ach_f / PEAK_FP32_TFLOPS, "flops_per_waypoint": fl_fwd},
+                        "backward": {"achieved": ach_b, "frac": ach_b / PEAK_FP32_TFLOPS,
+                                     "flops_per_waypoint": fl_bwd},
+                        "kernel": "k_clearance (one waypoint per wavefront, full pair set, square root only "
+                                  "for pairs within reach)"},
+           "note": "ms_forward_backward is wall time of the autograd pair (forward, sum, backward, "
+                   "gradient accumulation) from Python; kernel_ms are HIP events around the two library "
+                   "kernels alone"}
     if not args.skip_cpu:
         o = O.Scene(desc)
         pick = rng.choice(P, 64, replace=False)

@@ -810,35 +810,60 @@ __global__ void __launch_bounds__(256, 3) k_arm_configs(ArmK V, const double* qv
     }
 }
 
-// Plain per-call form: one launch checks edge 0 of B with one workgroup.  Its waves split the
-// edge's states (wave w: slots w, w + nw, ...) and share the verdict through LDS: a wave that finds a
+// Plain per-call form: one launch checks edge 0 of B with one workgroup of nw waves; wave w takes
+// the edge's slots w, w + nw, ...  The waves share the verdict through LDS: a wave that finds a
 // collision raises the flag, the others stop before their next state, and thread 0 writes the
-// verdict once every wave is done, so the output needs no initialisation.
+// verdict once every wave is done, so the output needs no initialisation.  The rows are loaded
+// before the scene is staged, so the two round trips overlap.
 template <bool GRASP, bool CHAIN>
-__global__ void __launch_bounds__(1024) k_arm_edge(ArmK V, EdgeBatchView B) {
+__global__ void __launch_bounds__(512) k_arm_edge(ArmK V, EdgeBatchView B) {
     extern __shared__ double lds_arm[];
     __shared__ int s_hit;
+    const int lane = wave_lane();
+    const int D = V.D;
+    // coordinate `lane` of both rows (lanes >= D: 0)
+    const double* a = row1(B, 0);
+    const double* b = row2(B, 0);
+    const double av = lane < D ? a[lane * B.ds] : 0.0;
+    const double bv = lane < D ? b[lane * B.ds] : 0.0;
     if (threadIdx.x == 0) s_hit = 0;
     arm_stage_scene(lds_arm, V);  // ends with a barrier
+    int N;
+    if (B.N || B.N0 > 0) {
+        N = B.N ? B.N[0] : B.N0;
+    } else {
+        // edge_N_dev's rule (m = |a0 - b0|, then m = v > m ? v : m over the coordinates): lane 0
+        // starts from coordinate 0 and the reduction only replaces m by a larger number, so a NaN
+        // in coordinate 0 sticks and a NaN elsewhere is skipped, as in the sequential loop
+        double m = lane < D ? __builtin_fabs(av - bv) : 0.0;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const double v = __shfl_xor(m, o, 64);
+            m = v > m ? v : m;
+        }
+        N = edge_N_from_max(__shfl(m, 0), B.res);
+    }
     const ArmLds Wl = arm_lds(lds_arm, V);
     const ArmRegs R = arm_regs(V, Wl);
     const int nw = blockDim.x >> 6, wv = threadIdx.x >> 6;
     const int mode = batch_mode(B, 0);
     const bool bad_mode = mode < 0 || mode >= V.nm;
-    const int N = edge_N_dev(B, 0, V.D);
     const EdgeWin w = edge_window(N, batch_N_start(B, 0), batch_N_max(B, 0), B.incl);
     NoCount rechecks;
     ArmCountT<false> cnt;
     if (!w.err && !bad_mode) {
-        const int Dact = arm_mact(V, Wl, mode) * V.dof;
         volatile int* hit = &s_hit;
         for (int local = wv; local < w.M; local += nw) {
             if (*hit) break;  // wave-uniform: one LDS word
             const int i = edge_state_index(w, local, B.incl);
             if (i < 0) continue;
-            const double qv = arm_load_state(B, 0, N, i, Dact);
+            double qv;
+            {
+#pragma clang fp contract(off)
+                qv = av + ((bv - av) / (double)(N - 1)) * (double)i;  // abstract_env.py:341, :348
+            }
             if (arm_state_collides<GRASP, CHAIN, false>(V, Wl, R, qv, mode, B.mask, rechecks, cnt)) {
-                if (wave_lane() == 0) *hit = 1;
+                if (lane == 0) *hit = 1;
                 break;
             }
         }

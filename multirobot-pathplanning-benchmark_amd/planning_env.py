@@ -4,9 +4,9 @@
   (planning_env.py:34-51).
 * ``Mode`` / ``State`` — the types the collision calls receive (planning_env.py:102-193); a mode
   of an engine scene carries ``mrcc_mode_id``, the index of its scene graph on the device.
-* ``BaseProblem`` — the abstract collision contract (planning_env.py:1531-1881) with the
-  reference's sequential ``is_path_collision_free`` schedule as the default implementation, so
-  backends that only provide the per-call checks behave exactly like the reference.
+* ``BaseProblem`` — the abstract collision contract (planning_env.py:1531-1881); its default
+  ``is_path_collision_free`` gives the reference's boolean from the per-call checks, so backends
+  that only provide those behave like the reference.
 """
 from __future__ import annotations
 
@@ -131,58 +131,31 @@ class BaseProblem(ABC):
                                resolution: Optional[float] = None, tolerance: Optional[float] = None,
                                check_edges_in_order: bool = False,
                                check_start_and_end: bool = True) -> bool:
-        """The reference's two path schedules (planning_env.py:1765-1881), issuing one
-        is_collision_free / is_edge_collision_free call at a time."""
-        if tolerance is None:
-            tolerance = self.collision_tolerance
+        """The boolean of planning_env.py:1765-1881 from the per-call checks.
+
+        Both of the reference's schedules test every interior state of every edge; they differ in
+        the order of the calls and in the vertices: the first vertex unless check_start_and_end is
+        False, every inner vertex, and the last vertex when check_start_and_end or
+        check_edges_in_order.  Here the vertices are checked first, then the edges in path order
+        (one is_edge_collision_free call each); the first failure decides.  Backends override this
+        with batched forms (MrccEnvironment issues one launch for the vertices and one for the
+        edges)."""
         if resolution is None:
             resolution = self.collision_resolution
-        if binary_order:
-            idx = generate_binary_search_indices(len(path) - 1)
-        else:
-            idx = list(range(len(path) - 1))
-        if check_edges_in_order:
-            for i in idx:
-                if i == 0 and not check_start_and_end:
-                    continue
-                if not self.is_collision_free(path[i].q, path[i].mode):
-                    return False
-            for i in idx:
-                if not self.is_edge_collision_free(path[i].q, path[i + 1].q, path[i].mode,
-                                                   resolution=resolution, tolerance=tolerance,
-                                                   include_endpoints=False):
-                    return False
-            if not self.is_collision_free(path[-1].q, path[-1].mode):
+        if tolerance is None:
+            tolerance = self.collision_tolerance
+        K = len(path)
+        if K < 2:
+            raise IndexError("a path needs at least two states")
+        verts = [i for i in range(K - 1) if i > 0 or check_start_and_end]
+        if check_start_and_end or check_edges_in_order:
+            verts.append(K - 1)
+        for i in verts:
+            if not self.is_collision_free(path[i].q, path[i].mode):
                 return False
-        else:
-            Ns: Dict[int, int] = {}
-            edge_queue = list(idx)
-            checks_per_iteration = 10
-            N_start = 0
-            N_max = N_start + checks_per_iteration
-            while edge_queue:
-                edges_to_remove = []
-                for i in edge_queue:
-                    q1 = path[i].q
-                    q2 = path[i + 1].q
-                    mode = path[i].mode
-                    if i not in Ns:
-                        Ns[i] = edge_num_states(q1.state(), q2.state(), resolution)
-                    if N_start == 0:
-                        if (i != 0 or (check_start_and_end and i == 0)) and not self.is_collision_free(q1, mode):
-                            return False
-                    if N_start > Ns[i]:
-                        edges_to_remove.append(i)
-                        continue
-                    if not self.is_edge_collision_free(q1, q2, mode, resolution=resolution,
-                                                       tolerance=tolerance, include_endpoints=False,
-                                                       N_start=N_start, N_max=N_max, N=Ns[i]):
-                        return False
-                if edges_to_remove:
-                    edge_queue = [i for i in edge_queue if i not in edges_to_remove]
-                N_start += checks_per_iteration
-                N_max += checks_per_iteration
-            if check_start_and_end and not self.is_collision_free(path[-1].q, path[-1].mode):
+        for i in range(K - 1):
+            if not self.is_edge_collision_free(path[i].q, path[i + 1].q, path[i].mode, resolution=resolution,
+                                               tolerance=tolerance, include_endpoints=False):
                 return False
         return True
 

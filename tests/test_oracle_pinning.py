@@ -8,8 +8,9 @@
 * DH forward kinematics: against an independent numpy product of 4x4 homogeneous matrices.
 * 2D / 3D separating-axis tests: against polygon intersection / LP feasibility.
 * Sphere-box: against the textbook per-axis distance form.
-* Path schedules: the reference's call counts (tests/test.py:93-122) through BaseProblem with a
-  mocked is_collision_free.
+* Path schedules: the reference's call counts (tests/test.py:93-122) through the oracle's
+  restatement of the schedules (oracle/path_schedule.py) and the package's default, with a mocked
+  is_collision_free; the default's boolean against the schedules on random paths.
 * Clearance gradient: against central finite differences of the FP64 cost.
 """
 import math
@@ -19,7 +20,9 @@ import numpy as np
 import pytest
 
 from helpers import O, pkg, scenes
+from oracle import path_schedule as PS
 from oracle.abstract_np import AbstractNp
+from oracle_env import OracleEnv
 
 
 # ---------------------------------------------------------------------------------------------
@@ -263,60 +266,38 @@ def test_sphere_obb_vs_projection():
 
 
 # ---------------------------------------------------------------------------------------------
-class OracleEnv(pkg.BaseProblem):
-    """BaseProblem over the oracle, to run the reference's sequential path schedule."""
-
-    def __init__(self, desc):
-        self.o = O.Scene(desc)
-        self.limits = desc.limits
-        self.start_pos = pkg.NpConfiguration.from_list([[-0.8, 0], [0.8, 0]])
-        self.start_mode = pkg.Mode([0, 0], self.start_pos)
-        self.collision_resolution = 0.01
-        self.collision_tolerance = 0.01
-        pkg.BaseProblem.__init__(self)
-
-    def get_scenegraph_info_for_mode(self, mode, is_start_mode=False):
-        return {}
-
-    def set_to_mode(self, mode):
-        pass
-
-    def is_collision_free(self, q, mode):
-        return self.o.config_free(q.state())
-
-    def is_edge_collision_free(self, q1, q2, m, resolution=None, tolerance=None, include_endpoints=False,
-                               N_start=0, N_max=None, N=None):
-        # per-state loop of abstract_env.py:337-354 through self.is_collision_free
-        if resolution is None:
-            resolution = self.collision_resolution
-        if N is None:
-            N = pkg.edge_num_states(q1.state(), q2.state(), resolution)
-        if N_start > N:
-            assert False
-        N_max = N if N_max is None else min(N, N_max)
-        idx = pkg.generate_binary_search_indices(N)
-        d = (q2.state() - q1.state()) / (N - 1)
-        for i in idx[N_start:N_max]:
-            if not include_endpoints and (i == 0 or i == N - 1):
-                continue
-            if not self.is_collision_free(q1.from_flat(q1.state() + d * i), m):
-                return False
-        return True
-
-
 def test_path_collision_checking_call_counts():
-    """tests/test.py:93-122 with the registry key of abstract_env.py:381."""
+    """tests/test.py:93-122 with the registry key of abstract_env.py:381: the oracle's restatement
+    of the reference's schedules (oracle/path_schedule.py) and the package's default
+    is_path_collision_free make the same number of is_collision_free calls."""
     env = OracleEnv(scenes.abstract_test_scene())
     q1 = env.start_pos.from_flat(np.array([-1, 0, 1, 1.0]))
     q2 = env.start_pos.from_flat(np.array([-1, 1, 1, 0.0]))
     q3 = env.start_pos.from_flat(np.array([-1, 2, 1, -1.0]))
     s = [pkg.State(q, env.start_mode) for q in (q1, q2, q3)]
-    assert env.is_path_collision_free(s, resolution=0.5)
-    for kw, calls in (({"resolution": 0.5, "check_edges_in_order": True}, 5), ({"resolution": 0.5}, 5),
-                      ({"resolution": 0.1, "check_edges_in_order": True}, 21), ({"resolution": 0.1}, 21)):
-        with mock.patch.object(env, "is_collision_free", return_value=True) as m:
-            env.is_path_collision_free(s, **kw)
-            assert m.call_count == calls, kw
+    for check in (lambda **kw: PS.sequential_path_collision_free(env, s, **kw),
+                  lambda **kw: env.is_path_collision_free(s, **kw)):
+        assert check(resolution=0.5)
+        for kw, calls in (({"resolution": 0.5, "check_edges_in_order": True}, 5), ({"resolution": 0.5}, 5),
+                          ({"resolution": 0.1, "check_edges_in_order": True}, 21), ({"resolution": 0.1}, 21)):
+            with mock.patch.object(env, "is_collision_free", return_value=True) as m:
+                check(**kw)
+                assert m.call_count == calls, kw
+
+
+def test_default_path_check_matches_schedules():
+    """The package's default is_path_collision_free gives the reference schedules' boolean on random
+    paths of the abstract test scene (CPU, both through the oracle)."""
+    env = OracleEnv(scenes.center_rect_scene(2))
+    rng = np.random.default_rng(31)
+    for trial in range(150):
+        K = int(rng.integers(2, 6))
+        path = [pkg.State(env.start_pos.from_flat(rng.uniform(-2, 2, 4)), env.start_mode) for _ in range(K)]
+        for in_order in (False, True):
+            for se in (False, True):
+                assert env.is_path_collision_free(path, check_edges_in_order=in_order, check_start_and_end=se) == \
+                    PS.sequential_path_collision_free(env, path, check_edges_in_order=in_order,
+                                                      check_start_and_end=se), (trial, in_order, se)
 
 
 def test_windows_partition_the_edge():
