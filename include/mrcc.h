@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define MRCC_ABI_VERSION 1
+#define MRCC_ABI_VERSION 2
 
 enum {
     MRCC_OK = 0,
@@ -102,13 +102,18 @@ int mrcc_scene_create_arm_chains(int n_arms, int dof, const double* base, const 
                                  int n_boxes, const double* boxes, mrcc_scene** out);
 
 /* Modes of an arm scene (mode -> scene graph, planning_env.py:1711-1722; attachment semantics
- * rai_base_env.py:801-805, pinocchio_env.py:266-372).  Mode m: arms [0, n_active[m]) exist; arm a
- * holds a box iff has_grasp[m*n_arms+a]; the box pose is T_ee(q) * T_rel with
- * grasp_rel[(m*n_arms+a)*12 ..] (row-major R then t) and half extents grasp_half[(m*n_arms+a)*3..].
+ * rai_base_env.py:704-839, pinocchio_env.py:266-372).  Mode m: arms [0, n_active[m]) exist.  Each
+ * mode has n_arms + n_rest box slots (at most 32); slot s of mode m is present iff
+ * has_grasp[m*slots+s], with transform grasp_rel[(m*slots+s)*12 ..] (row-major R then t) and half
+ * extents grasp_half[(m*slots+s)*3 ..]:
+ *   slot a < n_arms  a box held by arm a, pose T_ee(q) * T_rel (T_rel frozen at mode entry);
+ *   slot n_arms + j  a box resting at the world pose T_rel (a movable object put down).
  * A held box does not collide with the spheres of its holding link (the last link of its arm); it
  * is tested against every other sphere, its own arm's included, the obstacle boxes and the other
- * held boxes.  Mode 0 exists by default (all arms, no grasps). */
-int mrcc_scene_set_modes(mrcc_scene* scene, int n_modes, const int32_t* n_active,
+ * boxes of the mode.  A resting box is tested against every sphere and the held boxes (static
+ * against static is no collision of a configuration).  Mode 0 exists by default (all arms, no
+ * boxes). */
+int mrcc_scene_set_modes(mrcc_scene* scene, int n_modes, const int32_t* n_active, int n_rest,
                          const uint8_t* has_grasp, const double* grasp_rel,
                          const double* grasp_half);
 /* Arm scenes: 1 (default) = link bounding spheres cull sphere tests whose link pair (or link and
@@ -241,12 +246,13 @@ int mrcc_sincos(const double* x, int64_t n, double* s, double* c, void* stream);
 /* ---- differentiable clearance cost (arm scenes, FP32) ------------------------------------ */
 /* cost[r] = sum over the checked pair set of max(0, margin + r_i + r_j - d_ij)^2 (sphere-sphere)
  * and max(0, margin + r_i - d(sphere, box))^2 (sphere-box) for config row r (q: device float,
- * rows contiguous, D = n_arms*dof). */
-int mrcc_clearance_forward(const mrcc_scene* scene, const float* q, int64_t n, float margin,
+ * rows contiguous of dim coordinates; dim must equal the scene's D = n_arms*dof, else
+ * MRCC_ERR_INVALID). */
+int mrcc_clearance_forward(const mrcc_scene* scene, const float* q, int64_t n, int32_t dim, float margin,
                            float* cost, void* stream);
 /* grad_q[r][k] = grad_cost[r] * d cost[r] / d q[r][k]; FK is recomputed, nothing is stored by
  * the forward. */
-int mrcc_clearance_backward(const mrcc_scene* scene, const float* q, int64_t n, float margin,
+int mrcc_clearance_backward(const mrcc_scene* scene, const float* q, int64_t n, int32_t dim, float margin,
                             const float* grad_cost, float* grad_q, void* stream);
 
 /* ---- configuration-pair costs and distances (device) ------------------------------------- */

@@ -64,7 +64,7 @@ SIGNATURES = {
                                           _c.POINTER(_vp)]),
     "mrcc_scene_create_arm_chains": (_c.c_int, [_c.c_int, _c.c_int, _dp, _dp, _ip, _dp, _c.c_int, _dp,
                                                 _c.POINTER(_vp)]),
-    "mrcc_scene_set_modes": (_c.c_int, [_vp, _c.c_int, _ip, _u8p, _dp, _dp]),
+    "mrcc_scene_set_modes": (_c.c_int, [_vp, _c.c_int, _ip, _c.c_int, _u8p, _dp, _dp]),
     "mrcc_scene_set_broadphase": (_c.c_int, [_vp, _c.c_int]),
     "mrcc_scene_destroy": (_c.c_int, [_vp]),
     "mrcc_scene_info": (_c.c_int, [_vp, _ip]),
@@ -88,8 +88,8 @@ SIGNATURES = {
     "mrcc_edge_num_states": (_i64, [_i32, _i32, _i32, _i32]),
     "mrcc_arm_fk": (_c.c_int, [_vp, _vp, _i64, _i64, _i64, _vp, _vp]),
     "mrcc_sincos": (_c.c_int, [_vp, _i64, _vp, _vp, _vp]),
-    "mrcc_clearance_forward": (_c.c_int, [_vp, _vp, _i64, _c.c_float, _vp, _vp]),
-    "mrcc_clearance_backward": (_c.c_int, [_vp, _vp, _i64, _c.c_float, _vp, _vp, _vp]),
+    "mrcc_clearance_forward": (_c.c_int, [_vp, _vp, _i64, _i32, _c.c_float, _vp, _vp]),
+    "mrcc_clearance_backward": (_c.c_int, [_vp, _vp, _i64, _i32, _c.c_float, _vp, _vp, _vp]),
     "mrcc_config_cost": (_c.c_int, [_vp, _i64, _vp, _i64, _i64, _i32, _ip, _i32, _i32, _i32, _c.c_double,
                                     _vp, _vp]),
 }

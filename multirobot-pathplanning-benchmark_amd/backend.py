@@ -89,8 +89,13 @@ class Scene:
         g = np.ascontiguousarray(modes["has_grasp"], dtype=np.uint8)
         rel = np.ascontiguousarray(modes["grasp_rel"], dtype=np.float64)
         half = np.ascontiguousarray(modes["grasp_half"], dtype=np.float64)
-        check(lib().mrcc_scene_set_modes(self.handle, len(na), iptr(na), g.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
-                                         dptr(rel), dptr(half)))
+        n_rest = int(modes.get("n_rest", 0))
+        ns = self.desc.n_arms + n_rest
+        if g.shape != (len(na), ns) or rel.shape != (len(na), ns, 12) or half.shape != (len(na), ns, 3):
+            raise ValueError(f"modes: has_grasp [m, {ns}], grasp_rel [m, {ns}, 12], grasp_half [m, {ns}, 3] "
+                             f"(n_arms + n_rest box slots per mode)")
+        check(lib().mrcc_scene_set_modes(self.handle, len(na), iptr(na), n_rest,
+                                         g.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), dptr(rel), dptr(half)))
         self.n_modes = len(na)
 
     def __del__(self):
@@ -223,8 +228,13 @@ class MrccEnvironment(BaseProblem):
 
     def __init__(self, desc: SceneDesc, robots: Optional[List[str]] = None, start_pos=None,
                  collision_resolution: float = 0.01, collision_tolerance: float = 0.01,
-                 device: int = 0):
+                 device: int = 0, manipulation=None):
+        """``manipulation``: a ``modes.ManipulationSpec`` (movable boxes and tasks) for problems whose
+        modes change the scene; the device modes are then made from each mode's scene graph."""
         self.desc = desc
+        if manipulation is not None:
+            if desc.kind != "arms" or desc.modes is not None:
+                raise ValueError("manipulation problems need an arm scene without preset modes")
         self.scene = Scene(desc, device)
         self.limits = desc.limits
         n = desc.n_robots
@@ -240,17 +250,53 @@ class MrccEnvironment(BaseProblem):
         self.collision_tolerance = collision_tolerance
         self._slices = sl
         self._current_mode = self.start_mode
+        self._graphs = None
+        if manipulation is not None:
+            from .modes import SceneGraphs
+
+            self._graphs = SceneGraphs(manipulation, desc, self.robots, start_pos.state())
+            self._sg_ids: Dict[tuple, int] = {}
+            self._dev_modes: List[tuple] = []
+            self._mode_of(self.start_mode)
         BaseProblem.__init__(self)
 
     # ---- modes ----------------------------------------------------------------------------
+    def _mode_of(self, mode) -> int:
+        """Device mode of a manipulation problem's mode: its scene graph (replayed from the start
+        mode, modes.SceneGraphs) mapped to a device mode, made on first use."""
+        cached = getattr(mode, "_mrcc_device_mode", None)
+        if cached is not None and cached[0] is self:
+            return cached[1]
+        sg = self._graphs.scene_graph(mode)
+        key = self._graphs.key(sg)
+        mid = self._sg_ids.get(key)
+        if mid is None:
+            mid = len(self._dev_modes)
+            self._dev_modes.append(self._graphs.device_slots(sg))
+            n = len(self._dev_modes)
+            self.scene.set_modes({"n_active": np.full(n, self.desc.n_arms, dtype=np.int32),
+                                  "has_grasp": np.stack([m[0] for m in self._dev_modes]),
+                                  "grasp_rel": np.stack([m[1] for m in self._dev_modes]),
+                                  "grasp_half": np.stack([m[2] for m in self._dev_modes]),
+                                  "n_rest": len(self._graphs.spec.objects)})
+            self._sg_ids[key] = mid
+        try:
+            mode._mrcc_device_mode = (self, mid)
+        except AttributeError:
+            pass
+        return mid
+
     def mode_id(self, mode) -> int:
-        """Device mode id of a mode: an int as is; a Mode by its ``mrcc_mode_id``.  A Mode without
-        one is mode 0 only in a scene with a single mode; otherwise ValueError (a wrong scene graph
-        would give wrong answers silently)."""
+        """Device mode id of a mode: an int as is; in a manipulation problem, the device mode of
+        the Mode's scene graph; otherwise a Mode by its ``mrcc_mode_id``.  A Mode without one is
+        mode 0 only in a scene with a single mode; otherwise ValueError (a wrong scene graph would
+        give wrong answers silently)."""
         if mode is None:
             return 0
         if isinstance(mode, (int, np.integer)):
             m = int(mode)
+        elif self._graphs is not None:
+            return self._mode_of(mode)
         else:
             m = getattr(mode, "mrcc_mode_id", None)
             if m is None:
@@ -264,10 +310,24 @@ class MrccEnvironment(BaseProblem):
         return m
 
     def set_to_mode(self, mode: Mode) -> None:
+        """rai_base_env.py:704-839: the scene of `mode` (its device mode is made if new)."""
+        if self._graphs is not None:
+            self._mode_of(mode)
         self._current_mode = mode
 
     def get_scenegraph_info_for_mode(self, mode: Mode, is_start_mode: bool = False) -> Dict:
-        return {"mrcc_mode_id": self.mode_id(mode)}
+        """The mode's scene graph as rai reports it (rai_base_env.py:678-702): movable object ->
+        (parent frame, pose relative to it rounded to 3 decimals, as bytes); {} without movable
+        objects."""
+        if self._graphs is None:
+            return {}
+        self._mode_of(mode)
+        out = {}
+        for name, (holder, T) in self._graphs.scene_graph(mode).items():
+            rel = np.round(T, 3)
+            rel = np.where(abs(rel) < 1e-6, 0.0, rel)
+            out[name] = (self._graphs.spec.root if holder is None else self.robots[holder], rel.tobytes())
+        return out
 
     def robot_mask(self, r) -> int:
         if isinstance(r, str):

@@ -43,7 +43,7 @@ class _ClearanceFn:
                     n = q.shape[0]
                     cost = torch.empty(n, dtype=torch.float32, device=q.device)
                     st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-                    check(lib().mrcc_clearance_forward(scene.handle, ctypes.c_void_p(q.data_ptr()), n,
+                    check(lib().mrcc_clearance_forward(scene.handle, ctypes.c_void_p(q.data_ptr()), n, q.shape[1],
                                                        ctypes.c_float(margin), ctypes.c_void_p(cost.data_ptr()),
                                                        st))
                     ctx.save_for_backward(q)
@@ -58,7 +58,8 @@ class _ClearanceFn:
                     grad_q = torch.empty_like(q)
                     st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
                     check(lib().mrcc_clearance_backward(ctx.scene.handle, ctypes.c_void_p(q.data_ptr()), q.shape[0],
-                                                        ctypes.c_float(ctx.margin), ctypes.c_void_p(g.data_ptr()),
+                                                        q.shape[1], ctypes.c_float(ctx.margin),
+                                                        ctypes.c_void_p(g.data_ptr()),
                                                         ctypes.c_void_p(grad_q.data_ptr()), st))
                     return grad_q, None, None
 
@@ -71,6 +72,8 @@ def clearance_cost(scene, q, margin: float = 0.02):
     torch = _torch()
     if q.dtype != torch.float32 or not q.is_cuda:
         raise TypeError("clearance_cost needs a float32 HIP tensor")
+    if q.shape[-1] != scene.D:
+        raise ValueError(f"clearance_cost: rows of {q.shape[-1]} coordinates, the scene has D = {scene.D}")
     shape = q.shape[:-1]
     flat = q.reshape(-1, q.shape[-1])
     return _ClearanceFn.get().apply(flat, scene, float(margin)).reshape(shape)

@@ -109,6 +109,7 @@ struct ArmHost {
     std::vector<double> sph;       // [S*4]
     std::vector<double> boxes;     // [nb*6] centre half
     int nm;
+    int nslot;                     // mode-box slots per mode: na held + resting
     std::vector<int> m_active;
     std::vector<unsigned char> m_grasp;
     std::vector<double> m_rel, m_half;
@@ -235,7 +236,7 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     // held boxes: reach of the end-effector + grasp offset + box extent
     double grasp_reach = 0.0;
     for (size_t i = 0; i < H.m_half.size(); ++i) hmax = std::max(hmax, fabs(H.m_half[i]));
-    for (size_t i = 0; i < H.m_rel.size(); i += 12)
+    for (size_t i = 0; i < H.m_rel.size(); i += 12)  // held offsets, resting positions
         grasp_reach = std::max(grasp_reach, fabs(H.m_rel[i + 9]) + fabs(H.m_rel[i + 10]) + fabs(H.m_rel[i + 11]));
     B += grasp_reach + 2.0 * hmax;
     // broad phase: per link a bounding sphere (centre of the local sphere centres' box, radius
@@ -334,7 +335,7 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     if (old) (void)hipFree(old);
     void* d = sc->dev_mem;
     ArmView& V = sc->arm;
-    V.na = na; V.dof = dof; V.D = na * dof; V.L = L; V.S = S; V.nb = nb; V.nm = H.nm;
+    V.na = na; V.dof = dof; V.D = na * dof; V.L = L; V.S = S; V.nb = nb; V.nm = H.nm; V.nslot = H.nslot;
     V.chain = H.is_chain;
     V.broadphase = bp;
     // binary32 margins: squared distances near contact are off by <= ~2^-21 sqrt(3) R B; box SAT
@@ -392,6 +393,7 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
         ArmK& K = sc->armk;
         memset(&K, 0, sizeof(K));
         K.na = na; K.dof = dof; K.D = V.D; K.L = L; K.S = S; K.nb = nb; K.nm = nm; K.NP = NP; K.nclu = nclu;
+        K.nslot = H.nslot;
         K.chain = V.chain;
         K.broadphase = bp;
         K.grasp = V.grasp;
@@ -411,7 +413,7 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     sc->arm_nw = 0;
     int best = 0;
     for (int nw = 4; nw >= 1; nw >>= 1) {
-        const size_t b = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, nw);
+        const size_t b = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, nw, H.nslot);
         if (b > 65536) continue;
         const int resident = (int)(163840 / b) * nw;
         if (resident > best) {
@@ -423,13 +425,13 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     // the per-call kernel: one workgroup of as many waves (up to 8) as 64 KiB of LDS holds
     sc->arm_nw_edge = 1;
     for (int nw = 8; nw >= 1; --nw)
-        if (arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, nw) <= 65536) {
+        if (arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, nw, H.nslot) <= 65536) {
             sc->arm_nw_edge = nw;
             break;
         }
-    sc->arm_lds_edge = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, sc->arm_nw_edge);
-    sc->arm_lds = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, sc->arm_nw);
-    sc->arm_lds1 = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, 1);
+    sc->arm_lds_edge = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, sc->arm_nw_edge, H.nslot);
+    sc->arm_lds = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, sc->arm_nw, H.nslot);
+    sc->arm_lds1 = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, 1, H.nslot);
     sc->arm_na = na;
     sc->arm_dof = dof;
     sc->arm_S = S;
@@ -796,6 +798,7 @@ static int create_arm_scene(int n_arms, int dof, const double* base, const doubl
     H.sph.assign(sphere_local, sphere_local + 4 * S);
     H.boxes.assign(boxes, boxes + 6 * n_boxes);
     H.nm = 1;
+    H.nslot = n_arms;
     H.m_active.assign(1, n_arms);
     H.m_grasp.assign(n_arms, 0);
     H.m_rel.assign(12 * n_arms, 0.0);
@@ -826,24 +829,33 @@ int mrcc_scene_create_arm_chains(int n_arms, int dof, const double* base, const 
                             boxes, out);
 }
 
-int mrcc_scene_set_modes(mrcc_scene* scene, int n_modes, const int32_t* n_active,
-                         const uint8_t* has_grasp, const double* grasp_rel,
-                         const double* grasp_half) {
+int mrcc_scene_set_modes(mrcc_scene* scene, int n_modes, const int32_t* n_active, int n_rest,
+                         const uint8_t* has_grasp, const double* grasp_rel, const double* grasp_half) {
     CHECK_INJECT();
     if (!scene) return fail(MRCC_ERR_INVALID, "scene is null");
     SceneBox* box = (SceneBox*)scene;
     if (box->rec.kind != KIND_ARMS) return fail(MRCC_ERR_UNSUPPORTED, "modes need an arm scene");
     if (n_modes < 1) return fail(MRCC_ERR_INVALID, "n_modes < 1");
     ArmHost& H = box->arm_host;
-    const size_t nma = (size_t)n_modes * H.na;
+    if (n_rest < 0 || H.na + n_rest > MRCC_MAX_SLOTS)
+        return fail(MRCC_ERR_INVALID, "n_rest %d: at most %d boxes per mode (held and resting)", n_rest, MRCC_MAX_SLOTS);
+    if (!n_active || !has_grasp || !grasp_rel || !grasp_half) return fail(MRCC_ERR_INVALID, "null argument");
+    const int ns = H.na + n_rest;
+    const size_t nms = (size_t)n_modes * ns;
     for (int m = 0; m < n_modes; ++m)
         if (n_active[m] < 1 || n_active[m] > H.na)
             return fail(MRCC_ERR_INVALID, "mode %d: n_active %d not in 1..%d", m, n_active[m], H.na);
+    for (size_t i = 0; i < 12 * nms; ++i)
+        if (!std::isfinite(grasp_rel[i])) return fail(MRCC_ERR_INVALID, "box transform %zu is not finite", i / 12);
+    for (size_t i = 0; i < 3 * nms; ++i)
+        if (!(grasp_half[i] >= 0.0) || !std::isfinite(grasp_half[i]))
+            return fail(MRCC_ERR_INVALID, "box half extent %zu is negative or not finite", i);
     H.nm = n_modes;
+    H.nslot = ns;
     H.m_active.assign(n_active, n_active + n_modes);
-    H.m_grasp.assign(has_grasp, has_grasp + nma);
-    H.m_rel.assign(grasp_rel, grasp_rel + 12 * nma);
-    H.m_half.assign(grasp_half, grasp_half + 3 * nma);
+    H.m_grasp.assign(has_grasp, has_grasp + nms);
+    H.m_rel.assign(grasp_rel, grasp_rel + 12 * nms);
+    H.m_half.assign(grasp_half, grasp_half + 3 * nms);
     HIP_OK(hipDeviceSynchronize());  // kernels may still read the old arrays
     return build_arm_device(&box->rec, H);
 }
@@ -1327,25 +1339,27 @@ int mrcc_sincos(const double* x, int64_t n, double* s, double* c, void* stream) 
     return MRCC_OK;
 }
 
-int mrcc_clearance_forward(const mrcc_scene* scene, const float* q, int64_t n, float margin,
+int mrcc_clearance_forward(const mrcc_scene* scene, const float* q, int64_t n, int32_t dim, float margin,
                            float* cost, void* stream) {
     CHECK_INJECT();
     if (!scene || (!q && n) || (!cost && n)) return fail(MRCC_ERR_INVALID, "null argument");
     const SceneRec* sc = rec_of(scene);
     if (sc->kind != KIND_ARMS) return fail(MRCC_ERR_UNSUPPORTED, "clearance needs an arm scene");
+    if (dim != sc->D) return fail(MRCC_ERR_INVALID, "rows of %d coordinates, the scene has D = %d", dim, sc->D);
     if (n <= 0) return MRCC_OK;
     launch_clearance(*sc, q, n, margin, nullptr, cost, nullptr, (hipStream_t)stream);
     HIP_OK(hipGetLastError());
     return MRCC_OK;
 }
 
-int mrcc_clearance_backward(const mrcc_scene* scene, const float* q, int64_t n, float margin,
+int mrcc_clearance_backward(const mrcc_scene* scene, const float* q, int64_t n, int32_t dim, float margin,
                             const float* grad_cost, float* grad_q, void* stream) {
     CHECK_INJECT();
     if (!scene || (!q && n) || (!grad_cost && n) || (!grad_q && n))
         return fail(MRCC_ERR_INVALID, "null argument");
     const SceneRec* sc = rec_of(scene);
     if (sc->kind != KIND_ARMS) return fail(MRCC_ERR_UNSUPPORTED, "clearance needs an arm scene");
+    if (dim != sc->D) return fail(MRCC_ERR_INVALID, "rows of %d coordinates, the scene has D = %d", dim, sc->D);
     if (n <= 0) return MRCC_OK;
     launch_clearance(*sc, q, n, margin, grad_cost, nullptr, grad_q, (hipStream_t)stream);
     HIP_OK(hipGetLastError());

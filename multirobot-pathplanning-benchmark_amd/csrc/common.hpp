@@ -396,6 +396,7 @@ struct PlanarView {
 
 struct ArmView {
     int na, dof, D, L, S, nb, nm;
+    int nslot;    // mode-box slots per mode (na held + resting)
     float Msph;   // binary32 margin of sphere tests (squared-distance units)
     float Mobb;   // binary32 margin of box SAT axis values
     const double* base;      // [na*12]
@@ -440,6 +441,7 @@ struct ArmView {
 // which a workgroup copies with 16-byte loads.
 struct ArmK {
     int na, dof, D, L, S, nb, nm, NP, nclu;
+    int nslot;       // mode-box slots: na held boxes (slot a: arm a's end-effector) + resting boxes
     int chain;       // 1: chain joints
     int broadphase;  // 1: link bounding spheres cull sphere tests
     int grasp;       // 1: some mode holds a box
@@ -458,9 +460,9 @@ struct ArmK {
     __device__ const int* sph_lo() const { return (const int*)(g + o_sphlo); }          // [S]
     __device__ const int2* lpairs2() const { return (const int2*)(g + o_pairs); }       // [NP]
     __device__ const int* m_active() const { return (const int*)(g + o_mact); }         // [nm]
-    __device__ const unsigned char* m_grasp() const { return g + o_mgrasp; }            // [nm*na]
-    __device__ const double* m_rel() const { return (const double*)(g + o_mrel); }      // [nm*na*12]
-    __device__ const double* m_half() const { return (const double*)(g + o_mhalf); }    // [nm*na*3]
+    __device__ const unsigned char* m_grasp() const { return g + o_mgrasp; }            // [nm*nslot]
+    __device__ const double* m_rel() const { return (const double*)(g + o_mrel); }      // [nm*nslot*12]
+    __device__ const double* m_half() const { return (const double*)(g + o_mhalf); }    // [nm*nslot*3]
 };
 
 // batch of edges as seen by kernels.  The scalars N0, Ns0, Nm0, md0 stand for the arrays N, N_start,

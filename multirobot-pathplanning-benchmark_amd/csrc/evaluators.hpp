@@ -352,24 +352,26 @@ __host__ __device__ inline ArmStaticLayout arm_static_layout(int L, int na, int 
 }
 
 #define ARM_PEND_MAX 64  // band pairs awaiting their binary64 recheck, per wave
+#define MRCC_MAX_SLOTS 32  // mode boxes per mode (held + resting): one bit each in a mask
 
-// Wave region: X [12 L] doubles, held boxes G64 [12 na] doubles (grasp scenes only), the
-// band-pair queue, one area that holds the joints' kinematics coefficients [8 D] doubles during
-// the kinematics and the sphere table [S] float4 afterwards, link spheres [L] float4, three
-// candidate lists [64] int, held boxes G32 [16 na] float (grasp scenes only).  Every part is a
-// multiple of 16 bytes.
+// Wave region: X [12 L] doubles, mode boxes G64 [12 nslot] doubles (scenes with boxes in some
+// mode only), the band-pair queue, one area that holds the joints' kinematics coefficients [8 D]
+// doubles during the kinematics and the sphere table [S] float4 afterwards, link spheres [L]
+// float4, three candidate lists [64] int, mode boxes G32 [16 nslot] float (same scenes).  Every
+// part is a multiple of 16 bytes.  Mode-box slots: slot a < na is the box held by arm a, slot
+// na + j the j-th resting box.
 __host__ __device__ inline int arm_sph_bytes(int D, int S) { return 16 * S > 64 * D ? 16 * S : 64 * D; }
 
-__host__ __device__ inline int arm_wave_bytes(int D, int L, int na, int S, int grasp) {
-    return 96 * L + (grasp ? 96 * na : 0) + 8 * ARM_PEND_MAX + arm_sph_bytes(D, S) + 16 * L + 3 * 4 * 64 +
-           (grasp ? 64 * na : 0);
+__host__ __device__ inline int arm_wave_bytes(int D, int L, int nslot, int S, int grasp) {
+    return 96 * L + (grasp ? 96 * nslot : 0) + 8 * ARM_PEND_MAX + arm_sph_bytes(D, S) + 16 * L + 3 * 4 * 64 +
+           (grasp ? 64 * nslot : 0);
 }
 
 // dynamic LDS of an arm kernel launched with `nw` waves per workgroup
 __host__ __device__ inline size_t arm_lds_bytes(int D, int L, int na, int nb, int S, int NP, int nm,
-                                                int nclu, int grasp, int chain, int nw) {
+                                                int nclu, int grasp, int chain, int nw, int nslot) {
     return (size_t)arm_static_layout(L, na, nb, S, NP, nm, nclu, chain).bytes +
-           (size_t)nw * arm_wave_bytes(D, L, na, S, grasp);
+           (size_t)nw * arm_wave_bytes(D, L, nslot, S, grasp);
 }
 
 struct ArmLds {
@@ -440,11 +442,11 @@ __device__ inline ArmLds arm_lds(void* base, const ArmK& V) {
     W.pairs = (const int2*)(b + o.o_pairs);
     W.mact = (const int*)(b + o.o_mact);
     W.sets = (const int*)(b + o.o_sets);
-    unsigned char* w = b + o.bytes + (threadIdx.x >> 6) * arm_wave_bytes(V.D, V.L, V.na, V.S, V.grasp);
+    unsigned char* w = b + o.bytes + (threadIdx.x >> 6) * arm_wave_bytes(V.D, V.L, V.nslot, V.S, V.grasp);
     W.X = (double*)w;
     w += 96 * V.L;
     W.G64 = (double*)w;
-    if (V.grasp) w += 96 * V.na;
+    if (V.grasp) w += 96 * V.nslot;
     W.pend = (int2*)w;
     w += 8 * ARM_PEND_MAX;
     W.sph = (float4*)w;
@@ -561,8 +563,8 @@ __device__ inline bool arm_flush(const ArmK& V, const ArmLds& W, int& np, RC& re
     return __any(hit);
 }
 
-// Rechecks queued held-box pairs in binary64: kind 2 held box i - obstacle box j, 3 sphere i -
-// held box j, 4 held boxes i, j.  Empties the queue; returns the wave-uniform "some pair collides".
+// Rechecks queued mode-box pairs in binary64: kind 2 held box i - obstacle box j, 3 sphere i -
+// mode box j, 4 mode boxes i, j.  Empties the queue; returns the wave-uniform "some pair collides".
 template <class RC>
 __device__ inline bool arm_flush_held(const ArmK& V, const ArmLds& W, int mode, int& np, RC& rechecks) {
     wave_sync();
@@ -576,14 +578,14 @@ __device__ inline bool arm_flush_held(const ArmK& V, const ArmLds& W, int mode, 
             A[9] = bc[0];
             A[10] = bc[1];
             A[11] = bc[2];
-            hit |= obb3_overlap_f64(A, bc + 3, W.G64 + 12 * i, V.m_half() + 3 * (mode * V.na + i));
+            hit |= obb3_overlap_f64(A, bc + 3, W.G64 + 12 * i, V.m_half() + 3 * (mode * V.nslot + i));
         } else if (e.x == 3) {
             double c[3];
             xform_point_f64(W.X + 12 * (W.sphLink[i] & 255), W.sphL[3 * i], W.sphL[3 * i + 1], W.sphL[3 * i + 2], c);
-            hit |= sph_obb_f64(c, V.sph64()[4 * i + 3], W.G64 + 12 * j, V.m_half() + 3 * (mode * V.na + j));
+            hit |= sph_obb_f64(c, V.sph64()[4 * i + 3], W.G64 + 12 * j, V.m_half() + 3 * (mode * V.nslot + j));
         } else {
-            hit |= obb3_overlap_f64(W.G64 + 12 * i, V.m_half() + 3 * (mode * V.na + i), W.G64 + 12 * j,
-                                    V.m_half() + 3 * (mode * V.na + j));
+            hit |= obb3_overlap_f64(W.G64 + 12 * i, V.m_half() + 3 * (mode * V.nslot + i), W.G64 + 12 * j,
+                                    V.m_half() + 3 * (mode * V.nslot + j));
         }
     }
     if (wave_lane() == 0) rechecks += np;
@@ -715,8 +717,8 @@ __device__ inline bool arm_state_collides(const ArmK& V, const ArmLds& W, const 
                     R2 = quad_bcast_f64(y, 2);
                     t = quad_bcast_f64(y, 3);
                 }
-                if (GRASP && c == 0 && V.m_grasp()[mode * V.na + a]) {
-                    const double* Rel = V.m_rel() + 12 * (mode * V.na + a);
+                if (GRASP && c == 0 && V.m_grasp()[mode * V.nslot + a]) {
+                    const double* Rel = V.m_rel() + 12 * (mode * V.nslot + a);
                     double g[4];
                     compose_row_f64(R0, R1, R2, t, Rel, g);
                     for (int cc = 0; cc < 3; ++cc) {
@@ -725,9 +727,20 @@ __device__ inline bool arm_state_collides(const ArmK& V, const ArmLds& W, const 
                     }
                     W.G64[12 * a + 9 + rr] = g[3];
                     W.G32[16 * a + 9 + rr] = (float)g[3];
-                    W.G32[16 * a + 12 + rr] = (float)V.m_half()[3 * (mode * V.na + a) + rr];
+                    W.G32[16 * a + 12 + rr] = (float)V.m_half()[3 * (mode * V.nslot + a) + rr];
                 }
             }
+        }
+    }
+    if (GRASP && V.nslot > V.na) {
+        // resting boxes of the mode: their world pose is the slot's transform
+        for (int e = lane; e < 12 * (V.nslot - V.na); e += 64) {
+            const int sl = V.na + e / 12, k = e % 12;
+            if (!V.m_grasp()[mode * V.nslot + sl]) continue;
+            const double v = V.m_rel()[12 * (mode * V.nslot + sl) + k];
+            W.G64[12 * sl + k] = v;
+            W.G32[16 * sl + k] = (float)v;
+            if (k < 3) W.G32[16 * sl + 12 + k] = (float)V.m_half()[3 * (mode * V.nslot + sl) + k];
         }
     }
     wave_sync();
@@ -1129,14 +1142,18 @@ __device__ inline bool arm_state_collides(const ArmK& V, const ArmLds& W, const 
         }
     }
     if (np && arm_flush(V, W, np, rechecks)) return true;
-    // 6. held boxes
+    // 6. mode boxes: slot a < na holds the box held by arm a, slot na + j the j-th resting box
     if constexpr (!GRASP) return false;
-    unsigned gmask = 0;
-    for (int a = 0; a < na; ++a)
-        if (V.m_grasp()[mode * V.na + a] && ((mask >> a) & 1u)) gmask |= 1u << a;
+    unsigned gmask = 0;  // held boxes of active arms in the mask, resting boxes
+    for (int a = 0; a < V.nslot; ++a) {
+        const bool held = a < V.na;
+        if (held && (a >= na || !((mask >> a) & 1u))) continue;
+        if (V.m_grasp()[mode * V.nslot + a]) gmask |= 1u << a;
+    }
     if (gmask) {
         const float Mb = V.Mobb;
-        // 6a. held box vs obstacle boxes
+        // 6a. held box vs obstacle boxes (resting boxes do not move: they are not tested against
+        //     the static obstacles)
         for (int a = 0; a < na; ++a) {
             if (!((gmask >> a) & 1u)) continue;
             const float* G = W.G32 + 16 * a;
@@ -1156,17 +1173,17 @@ __device__ inline bool arm_state_collides(const ArmK& V, const ArmLds& W, const 
             }
         }
         if (__any(hit)) return true;
-        // 6b. held box vs the spheres of every arm in the mask, except those of the holding link
-        //     (the last link of arm a): the attachment deactivates only the (object, holding
-        //     link) pair (pinocchio_env.py:266-272; rai attach + setContact(-1),
+        // 6b. mode box vs the spheres of every arm in the mask; a held box skips the spheres of
+        //     its holding link (the last link of arm a): the attachment deactivates only the
+        //     (object, holding link) pair (pinocchio_env.py:266-272; rai attach + setContact(-1),
         //     rai_base_env.py:801-805)
-        for (int a = 0; a < na; ++a) {
+        for (int a = 0; a < V.nslot; ++a) {
             if (!((gmask >> a) & 1u)) continue;
             const float* G = W.G32 + 16 * a;
-            const int hold = a * dof + dof - 1;
+            const int hold = a < V.na ? a * dof + dof - 1 : 255;  // 255: no link
             for (int base = 0; base < S_act; base += 64) {
                 const int s = base + lane;
-                const int sl = s < S_act ? W.sphLink[s] : hold | (a << 8);
+                const int sl = s < S_act ? W.sphLink[s] : 255;
                 const int as = sl >> 8;
                 const bool act = s < S_act && (sl & 255) != hold && ((mask >> as) & 1u);
                 bool band = false;
@@ -1191,11 +1208,14 @@ __device__ inline bool arm_state_collides(const ArmK& V, const ArmLds& W, const 
             }
         }
         if (__any(hit)) return true;
-        // 6c. held box vs held box (at most 8 x 7 / 2 pairs: one lane each)
-        {
-            const int a = lane >> 3, b = lane & 7;
+        // 6c. mode box pairs (a, b), a < b, a held: held vs held and held vs resting (resting
+        //     boxes do not move relative to each other); one lane per pair
+        const int ns = V.nslot;
+        for (int p0 = 0; p0 < ns * ns; p0 += 64) {
+            const int p = p0 + lane;
+            const int a = p / ns, b = p - a * ns;
             bool band = false;
-            if (a < na && b < na && a < b && ((gmask >> a) & 1u) && ((gmask >> b) & 1u)) {
+            if (p < ns * ns && a < b && a < V.na && ((gmask >> a) & 1u) && ((gmask >> b) & 1u)) {
                 const float* Ga = W.G32 + 16 * a;
                 const float* Gb = W.G32 + 16 * b;
                 const int r = obb3_overlap_f32(Ga, Ga + 12, Gb, Gb + 12, Mb);
@@ -1203,6 +1223,7 @@ __device__ inline bool arm_state_collides(const ArmK& V, const ArmLds& W, const 
                 else if (r == 2) band = true;
             }
             arm_pend(W, band, 4, a, b, np);
+            if (np > ARM_PEND_MAX - 64 && arm_flush_held(V, W, mode, np, rechecks)) return true;
         }
         if (__any(hit)) return true;
         if (np && arm_flush_held(V, W, mode, np, rechecks)) return true;

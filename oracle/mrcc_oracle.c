@@ -44,11 +44,13 @@ struct or_scene {
     double* box;            /* [nb,6] lo xyz hi xyz */
     double* boxch;          /* [nb,6] centre xyz half xyz */
     int nm;
+    int nslot;              /* mode-box slots: na held (slot a: arm a's end-effector) + resting */
     int32_t* m_active;      /* [nm] */
-    uint8_t* m_grasp;       /* [nm*na] */
-    double* m_rel;          /* [nm*na*12] */
-    double* m_half;         /* [nm*na*3] */
+    uint8_t* m_grasp;       /* [nm*nslot] slot present in the mode */
+    double* m_rel;          /* [nm*nslot*12] held: pose in the end-effector frame; resting: world */
+    double* m_half;         /* [nm*nslot*3] */
 };
+#define MAX_SLOTS 32
 
 /* ------------------------------------------------------------------------------------------ */
 /* sincos: Cody-Waite reduction by pi/2 (three-part constant, 33+33+53 bits) and Taylor series of
@@ -369,7 +371,7 @@ or_scene* or_scene_arms(int n_arms, int dof, const double* base, const double* d
     int32_t na = n_arms;
     uint8_t g[MAX_ARMS] = {0};
     double rel[MAX_ARMS * 12] = {0}, half[MAX_ARMS * 3] = {0};
-    or_scene_set_modes(s, 1, &na, g, rel, half);
+    or_scene_set_modes(s, 1, &na, 0, g, rel, half);
     return s;
 }
 
@@ -388,13 +390,14 @@ or_scene* or_scene_arm_chains(int n_arms, int dof, const double* base, const dou
     return s;
 }
 
-int or_scene_set_modes(or_scene* s, int n_modes, const int32_t* n_active,
+int or_scene_set_modes(or_scene* s, int n_modes, const int32_t* n_active, int n_rest,
                        const uint8_t* has_grasp, const double* grasp_rel,
                        const double* grasp_half) {
-    if (s->kind != K_ARMS || n_modes < 1) return -1;
+    if (s->kind != K_ARMS || n_modes < 1 || n_rest < 0 || s->na + n_rest > MAX_SLOTS) return -1;
     free(s->m_active); free(s->m_grasp); free(s->m_rel); free(s->m_half);
     s->nm = n_modes;
-    size_t nma = (size_t)n_modes * s->na;
+    s->nslot = s->na + n_rest;
+    size_t nma = (size_t)n_modes * s->nslot;
     s->m_active = (int32_t*)malloc(sizeof(int32_t) * n_modes);
     s->m_grasp = (uint8_t*)malloc(nma);
     s->m_rel = (double*)malloc(sizeof(double) * 12 * nma);
@@ -538,16 +541,24 @@ static const double IDENT12[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
 static int arm_free(const or_scene* s, const double* q, int mode, uint32_t mask) {
     if (mode < 0 || mode >= s->nm) return -1;
     if (!all_finite(q, s->D)) return 0;
-    const int na = s->m_active[mode], dof = s->dof, L = s->na * dof;
+    const int na = s->m_active[mode], dof = s->dof, L = s->na * dof, ns = s->nslot;
     double* X = (double*)malloc(sizeof(double) * 12 * (size_t)L);
     double* C = (double*)malloc(sizeof(double) * 3 * (size_t)(s->S + 1));
-    double G[MAX_ARMS][12];
+    double G[MAX_SLOTS][12];
+    int present[MAX_SLOTS];
     int free_ = 1;
     or_arm_fk(s, q, X);
     for (int t = 0; t < s->S; ++t) xform_point(X + 12 * s->sph_link[t], s->sph + 4 * t, C + 3 * t);
-    for (int a = 0; a < na; ++a)
-        if (s->m_grasp[mode * s->na + a])
-            compose(X + 12 * (a * dof + dof - 1), s->m_rel + 12 * (mode * s->na + a), G[a]);
+    /* mode boxes: slot a < na is held by arm a (pose T_ee(q) T_rel), slot na + j rests at a world
+     * pose.  A held box counts when its arm is active and in the mask; a resting box always. */
+    for (int a = 0; a < ns; ++a) {
+        const int held = a < s->na;
+        present[a] = s->m_grasp[mode * ns + a] && (!held || (a < na && ((mask >> a) & 1u)));
+        if (!present[a]) continue;
+        const double* rel = s->m_rel + 12 * (mode * ns + a);
+        if (held) compose(X + 12 * (a * dof + dof - 1), rel, G[a]);
+        else memcpy(G[a], rel, sizeof(double) * 12);
+    }
     for (int a = 0; a < na && free_; ++a) {
         if (!((mask >> a) & 1u)) continue;
         const int s0 = s->link_first[a * dof], s1 = s->link_first[(a + 1) * dof];
@@ -562,26 +573,8 @@ static int arm_free(const or_scene* s, const double* q, int mode, uint32_t mask)
             for (int u = u0; u < s1; ++u)
                 if (sph_sph(C + 3 * t, s->sph[4 * t + 3], C + 3 * u, s->sph[4 * u + 3])) { free_ = 0; break; }
         }
-        /* held box vs boxes */
-        if (free_ && s->m_grasp[mode * s->na + a]) {
-            const double* h = s->m_half + 3 * (mode * s->na + a);
-            for (int b = 0; b < s->nb; ++b) {
-                double Bx[12];
-                memcpy(Bx, IDENT12, sizeof(Bx));
-                for (int k = 0; k < 3; ++k) Bx[9 + k] = s->boxch[6 * b + k];
-                if (obb3_overlap(Bx, s->boxch + 6 * b + 3, G[a], h)) { free_ = 0; break; }
-            }
-        }
-        /* held box vs the spheres of its own arm, except those of the holding (last) link: the
-         * attachment deactivates only the (object, holding link) pair (pinocchio_env.py:266-272;
-         * rai attach + setContact(-1), rai_base_env.py:801-805) */
-        if (free_ && s->m_grasp[mode * s->na + a]) {
-            const double* h = s->m_half + 3 * (mode * s->na + a);
-            const int s_hold = s->link_first[a * dof + dof - 1];
-            for (int t = s0; t < s_hold; ++t)
-                if (sph_obb(C + 3 * t, s->sph[4 * t + 3], G[a], h)) { free_ = 0; break; }
-        }
     }
+    /* spheres of two different arms */
     for (int a = 0; a < na && free_; ++a) {
         if (!((mask >> a) & 1u)) continue;
         for (int b = a + 1; b < na && free_; ++b) {
@@ -591,16 +584,42 @@ static int arm_free(const or_scene* s, const double* q, int mode, uint32_t mask)
             for (int t = a0; t < a1 && free_; ++t)
                 for (int u = b0; u < b1; ++u)
                     if (sph_sph(C + 3 * t, s->sph[4 * t + 3], C + 3 * u, s->sph[4 * u + 3])) { free_ = 0; break; }
-            const int ga = s->m_grasp[mode * s->na + a], gb = s->m_grasp[mode * s->na + b];
-            const double* ha = s->m_half + 3 * (mode * s->na + a);
-            const double* hb = s->m_half + 3 * (mode * s->na + b);
-            if (free_ && ga)
-                for (int u = b0; u < b1; ++u)
-                    if (sph_obb(C + 3 * u, s->sph[4 * u + 3], G[a], ha)) { free_ = 0; break; }
-            if (free_ && gb)
-                for (int t = a0; t < a1; ++t)
-                    if (sph_obb(C + 3 * t, s->sph[4 * t + 3], G[b], hb)) { free_ = 0; break; }
-            if (free_ && ga && gb && obb3_overlap(G[a], ha, G[b], hb)) free_ = 0;
+        }
+    }
+    /* held boxes vs obstacle boxes (resting boxes do not move) */
+    for (int a = 0; a < s->na && a < ns && free_; ++a) {
+        if (!present[a]) continue;
+        const double* h = s->m_half + 3 * (mode * ns + a);
+        for (int b = 0; b < s->nb; ++b) {
+            double Bx[12];
+            memcpy(Bx, IDENT12, sizeof(Bx));
+            for (int k = 0; k < 3; ++k) Bx[9 + k] = s->boxch[6 * b + k];
+            if (obb3_overlap(Bx, s->boxch + 6 * b + 3, G[a], h)) { free_ = 0; break; }
+        }
+    }
+    /* mode boxes vs the spheres of the active arms in the mask, except, for a held box, those of
+     * its holding link: the attachment deactivates only the (object, holding link) pair
+     * (pinocchio_env.py:266-272; rai attach + setContact(-1), rai_base_env.py:801-805) */
+    for (int a = 0; a < ns && free_; ++a) {
+        if (!present[a]) continue;
+        const double* h = s->m_half + 3 * (mode * ns + a);
+        for (int b = 0; b < na && free_; ++b) {
+            if (!((mask >> b) & 1u)) continue;
+            const int t0 = s->link_first[b * dof];
+            const int t1 = (a == b) ? s->link_first[b * dof + dof - 1] : s->link_first[(b + 1) * dof];
+            for (int t = t0; t < t1; ++t)
+                if (sph_obb(C + 3 * t, s->sph[4 * t + 3], G[a], h)) { free_ = 0; break; }
+        }
+    }
+    /* pairs of mode boxes with at least one held box (resting boxes do not move) */
+    for (int a = 0; a < s->na && a < ns && free_; ++a) {
+        if (!present[a]) continue;
+        for (int b = a + 1; b < ns; ++b) {
+            if (!present[b]) continue;
+            if (obb3_overlap(G[a], s->m_half + 3 * (mode * ns + a), G[b], s->m_half + 3 * (mode * ns + b))) {
+                free_ = 0;
+                break;
+            }
         }
     }
     free(X);

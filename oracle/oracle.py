@@ -49,7 +49,7 @@ def lib():
         L.or_scene_arms.restype = ctypes.c_void_p
         L.or_scene_arm_chains.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _ip, _dp, ctypes.c_int, _dp]
         L.or_scene_arm_chains.restype = ctypes.c_void_p
-        L.or_scene_set_modes.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, _u8p, _dp, _dp]
+        L.or_scene_set_modes.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, ctypes.c_int, _u8p, _dp, _dp]
         L.or_scene_free.argtypes = [ctypes.c_void_p]
         L.or_scene_dim.argtypes = [ctypes.c_void_p]
         L.or_config_free.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_uint32]
@@ -170,7 +170,8 @@ class Scene:
                 rel, prel = _d(m["grasp_rel"])
                 half, phalf = _d(m["grasp_half"])
                 self._keep += [na, g, rel, half]
-                rc = L.or_scene_set_modes(self.h, len(na), pna, g.ctypes.data_as(_u8p), prel, phalf)
+                rc = L.or_scene_set_modes(self.h, len(na), pna, int(m.get("n_rest", 0)), g.ctypes.data_as(_u8p),
+                                          prel, phalf)
                 assert rc == 0
         else:
             raise ValueError(k)

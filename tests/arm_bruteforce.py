@@ -13,9 +13,11 @@ mode, the held boxes) is checked by a third computation:
   every sphere against every obstacle box;
 * arms [0, n_active[mode]) exist in a mode;
 * a held box of arm a (pose T_ee(q) T_rel, half extents h) meets the obstacle boxes, the other held
-  boxes and every sphere except those of its holding link, the last link of arm a: the attachment
-  deactivates only the (object, new parent) pair (pinocchio_env.py:266-272; rai attach +
-  setContact(-1), rai_base_env.py:801-805);
+  boxes, the resting boxes and every sphere except those of its holding link, the last link of arm
+  a: the attachment deactivates only the (object, new parent) pair (pinocchio_env.py:266-272; rai
+  attach + setContact(-1), rai_base_env.py:801-805);
+* a resting box of the mode (a movable object put down at a world pose) meets every sphere and the
+  held boxes; static against static (obstacles, other resting boxes) is not a state's collision;
 * robot mask: a pair counts iff every robot in it is in the mask (rai_base_env.py:560-586,
   collisions with robots outside the set are ignored);
 * non-finite coordinates collide;
@@ -128,7 +130,7 @@ def sph_box_slack(c, r, T, h):
 
 
 CATS = ("sphere_box", "self", "inter_arm", "held_box", "held_sphere_other", "held_sphere_own",
-        "held_held")
+        "held_held", "rest_sphere", "rest_held")
 
 
 def pair_slacks(desc, q, mode=0, eps=1e-9, mutant=None):
@@ -146,6 +148,7 @@ def pair_slacks(desc, q, mode=0, eps=1e-9, mutant=None):
     else:
         n_act, has = na, np.zeros(na, dtype=bool)
         rel = half = None
+    n_slots = len(has)
     F = link_frames(desc, q)
     link, arm = sphere_table(desc)
     loc = desc.sphere_local
@@ -176,6 +179,15 @@ def pair_slacks(desc, q, mode=0, eps=1e-9, mutant=None):
     for a in range(n_act):
         if has[a]:
             G[a] = F[a * dof + dof - 1] @ from12(rel[a])
+    for j in range(na, n_slots):  # resting boxes: world poses
+        if has[j]:
+            hj = np.asarray(half[j], dtype=np.float64)
+            Gj = from12(rel[j])
+            for i in idx:
+                out.append(("rest_sphere", frozenset([arm[i]]), sph_box_slack(C[i], r[i], Gj, hj)))
+            for a, Ga in G.items():
+                out.append(("rest_held", frozenset([a]),
+                            box_box_slack(Ga, np.asarray(half[a], dtype=np.float64), Gj, hj, eps)))
     for a, Ga in G.items():
         ha = np.asarray(half[a], dtype=np.float64)
         for b in range(len(boxes)):
@@ -233,7 +245,7 @@ def _rand_rot(rng):
 
 
 def tiny_arm_scene(rng, scenes, n_arms=3, dof=4, n_boxes=3, n_modes=6, chain=False, spread=0.35,
-                   held_half=(0.02, 0.09), rel_offset=0.15, sphere_r=(0.02, 0.07)):
+                   held_half=(0.02, 0.09), rel_offset=0.15, sphere_r=(0.02, 0.07), n_rest=0):
     """A small arm scene with modes: arm bases on a circle of radius `spread` (close enough for
     inter-arm contacts), 1..3 spheres per link, a few boxes, and modes with fewer active arms and
     held boxes of various offsets."""
@@ -275,15 +287,18 @@ def tiny_arm_scene(rng, scenes, n_arms=3, dof=4, n_boxes=3, n_modes=6, chain=Fal
                             joint_fixed=jf if chain else None)
     n_active = rng.integers(1, n_arms + 1, size=n_modes).astype(np.int32)
     n_active[0] = n_arms
-    has = (rng.uniform(size=(n_modes, n_arms)) < 0.8).astype(np.uint8)
+    ns = n_arms + n_rest
+    has = (rng.uniform(size=(n_modes, ns)) < 0.8).astype(np.uint8)
     has[0] = 0
     for m in range(n_modes):
-        has[m, n_active[m]:] = 0
-    rel = np.zeros((n_modes, n_arms, 12))
-    half = rng.uniform(held_half[0], held_half[1], size=(n_modes, n_arms, 3))
+        has[m, n_active[m]:n_arms] = 0
+    rel = np.zeros((n_modes, ns, 12))
+    half = rng.uniform(held_half[0], held_half[1], size=(n_modes, ns, 3))
     for m in range(n_modes):
-        for a in range(n_arms):
+        for a in range(ns):
             rel[m, a, :9] = _rand_rot(rng).reshape(-1)
             rel[m, a, 9:] = rng.uniform(-rel_offset, rel_offset, 3)
-    desc.modes = {"n_active": n_active, "has_grasp": has, "grasp_rel": rel, "grasp_half": half}
+            if a >= n_arms:  # resting boxes among the arms
+                rel[m, a, 9:] = [*rng.uniform(-0.4, 0.4, 2), rng.uniform(0.0, 0.4)]
+    desc.modes = {"n_active": n_active, "has_grasp": has, "grasp_rel": rel, "grasp_half": half, "n_rest": n_rest}
     return desc

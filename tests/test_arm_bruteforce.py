@@ -20,7 +20,8 @@ def _cases(seed, kind, n_cfg=260):
     """(desc, list of (q, mode, mask, expected collides, deciding category)).  kind: "dh" or
     "chain" (3 arms x 4 links, 3 boxes), "dh-close" / "chain-close" (3 arms x 3 links close
     together, no boxes, larger held boxes: inter-arm and held-box contacts decide more states),
-    "dh-reach" (2 arms, held boxes far out on their end-effectors: held boxes meet each other)."""
+    "dh-reach" (2 arms, held boxes far out on their end-effectors: held boxes meet each other),
+    "dh-rest" / "chain-rest" (3 arms x 3 links and 3 resting boxes among them)."""
     rng = np.random.default_rng(seed)
     chain = kind.startswith("chain")
     if kind.endswith("close"):
@@ -28,6 +29,8 @@ def _cases(seed, kind, n_cfg=260):
     elif kind.endswith("reach"):
         kw = dict(n_arms=2, dof=3, n_boxes=0, spread=0.25, held_half=(0.06, 0.18), rel_offset=0.35,
                   sphere_r=(0.01, 0.03))
+    elif kind.endswith("rest"):
+        kw = dict(n_arms=3, dof=3, n_boxes=0, spread=0.25, held_half=(0.04, 0.12), n_rest=3)
     else:
         kw = dict(n_arms=3, dof=4, n_boxes=3)
     desc = BF.tiny_arm_scene(rng, scenes, chain=chain, **kw)
@@ -56,7 +59,7 @@ def _cases(seed, kind, n_cfg=260):
 
 
 SEEDS = [(11, "dh"), (12, "dh"), (13, "chain"), (14, "chain"), (15, "dh-close"), (16, "chain-close"),
-         (17, "dh-reach"), (18, "dh-reach")]
+         (17, "dh-reach"), (18, "dh-reach"), (19, "dh-rest"), (20, "chain-rest")]
 
 
 @pytest.fixture(scope="module", params=SEEDS, ids=[f"seed{s}-{c}" for s, c in SEEDS])

@@ -96,11 +96,11 @@ def test_ur10_clearance_matches_oracle(ur10):
     L = pkg.lib()
     qd = torch.from_numpy(Q).cuda()
     cost = torch.empty(64, dtype=torch.float32, device="cuda")
-    pkg._lib.check(L.mrcc_clearance_forward(sc.handle, ctypes.c_void_p(qd.data_ptr()), 64, ctypes.c_float(0.02),
+    pkg._lib.check(L.mrcc_clearance_forward(sc.handle, ctypes.c_void_p(qd.data_ptr()), 64, desc.dim, ctypes.c_float(0.02),
                                             ctypes.c_void_p(cost.data_ptr()), None))
     g_out = torch.ones(64, dtype=torch.float32, device="cuda")
     grad = torch.empty_like(qd)
-    pkg._lib.check(L.mrcc_clearance_backward(sc.handle, ctypes.c_void_p(qd.data_ptr()), 64, ctypes.c_float(0.02),
+    pkg._lib.check(L.mrcc_clearance_backward(sc.handle, ctypes.c_void_p(qd.data_ptr()), 64, desc.dim, ctypes.c_float(0.02),
                                              ctypes.c_void_p(g_out.data_ptr()), ctypes.c_void_p(grad.data_ptr()),
                                              None))
     torch.cuda.synchronize()

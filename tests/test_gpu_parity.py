@@ -505,11 +505,11 @@ def test_clearance_cost_and_grad(B):
     qd = torch.from_numpy(Q.astype(np.float32)).cuda()
     cost = torch.empty(64, dtype=torch.float32, device="cuda")
     L = pkg.lib()
-    pkg._lib.check(L.mrcc_clearance_forward(sc.handle, ctypes.c_void_p(qd.data_ptr()), 64, ctypes.c_float(0.02),
+    pkg._lib.check(L.mrcc_clearance_forward(sc.handle, ctypes.c_void_p(qd.data_ptr()), 64, desc.dim, ctypes.c_float(0.02),
                                             ctypes.c_void_p(cost.data_ptr()), None))
     g_out = torch.ones(64, dtype=torch.float32, device="cuda")
     grad = torch.empty_like(qd)
-    pkg._lib.check(L.mrcc_clearance_backward(sc.handle, ctypes.c_void_p(qd.data_ptr()), 64, ctypes.c_float(0.02),
+    pkg._lib.check(L.mrcc_clearance_backward(sc.handle, ctypes.c_void_p(qd.data_ptr()), 64, desc.dim, ctypes.c_float(0.02),
                                              ctypes.c_void_p(g_out.data_ptr()), ctypes.c_void_p(grad.data_ptr()),
                                              None))
     torch.cuda.synchronize()
