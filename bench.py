@@ -886,9 +886,9 @@ def clearance_workload(args, backend, scenes, dev, threads):
     gq = torch.empty_like(qf)
     gc = torch.ones(n, dtype=torch.float32, device=dev)
     st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    fwd = lambda: lib.check(lib.lib().mrcc_clearance_forward(sc.handle, ctypes.c_void_p(qf.data_ptr()), n,
+    fwd = lambda: lib.check(lib.lib().mrcc_clearance_forward(sc.handle, ctypes.c_void_p(qf.data_ptr()), n, D,
                                                              ctypes.c_float(0.02), ctypes.c_void_p(cost.data_ptr()), st))
-    bwd = lambda: lib.check(lib.lib().mrcc_clearance_backward(sc.handle, ctypes.c_void_p(qf.data_ptr()), n,
+    bwd = lambda: lib.check(lib.lib().mrcc_clearance_backward(sc.handle, ctypes.c_void_p(qf.data_ptr()), n, D,
                                                               ctypes.c_float(0.02), ctypes.c_void_p(gc.data_ptr()),
                                                               ctypes.c_void_p(gq.data_ptr()), st))
     kms = {}
@@ -920,6 +920,32 @@ def clearance_workload(args, backend, scenes, dev, threads):
            "note": "ms_forward_backward is wall time of the autograd pair (forward, sum, backward, "
                    "gradient accumulation) from Python; kernel_ms are HIP events around the two library "
                    "kernels alone"}
+    # repair: 80 Adam steps on clearance + smoothness, every result re-validated by the exact checker
+    env = backend.MrccEnvironment(desc, collision_resolution=0.05)
+    ra = np.clip(rng.normal(0, 0.4, (1024, D)), -2.9, 2.9)
+    rb = np.clip(rng.normal(0, 0.4, (1024, D)), -2.9, 2.9)
+    keep = env.batch_is_collision_free(ra) & env.batch_is_collision_free(rb)
+    ra, rb = ra[keep], rb[keep]
+    tr = np.linspace(0, 1, 32)[None, :, None]
+    rp = ra[:, None] + tr * (rb - ra)[:, None]
+    rp[:, 1:-1] += rng.normal(0, 0.1, rp[:, 1:-1].shape)
+    rp = np.clip(rp, -2.9, 2.9)
+    before = clr.validate_paths(env, rp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rout, rok = clr.repair_paths(env, rp, steps=80)
+    rms = (time.perf_counter() - t0) * 1e3
+    bad = ~before
+    ent["repair"] = {"paths": int(len(rp)), "waypoints": 32, "steps": 80, "in_collision": int(bad.sum()),
+                     "repaired": int((rok & bad).sum()),
+                     "acceptance_rate": float(rok[bad].mean()) if bad.any() else None, "ms": rms}
+    if not args.skip_cpu:
+        o = O.Scene(desc)
+        sel = np.nonzero(rok & bad)[0][:16]
+        ent["repair"]["oracle_checked_paths"] = int(len(sel))
+        ent["repair"]["oracle_invalid"] = int(sum(
+            not (all(o.config_free(v) for v in rout[p]) and
+                 all(o.edge_free(rout[p][i], rout[p][i + 1], 0.05) for i in range(31))) for p in sel))
     if not args.skip_cpu:
         o = O.Scene(desc)
         pick = rng.choice(P, 64, replace=False)
