@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define MRCC_ABI_VERSION 2
+#define MRCC_ABI_VERSION 3
 
 enum {
     MRCC_OK = 0,
@@ -116,6 +116,14 @@ int mrcc_scene_create_arm_chains(int n_arms, int dof, const double* base, const 
 int mrcc_scene_set_modes(mrcc_scene* scene, int n_modes, const int32_t* n_active, int n_rest,
                          const uint8_t* has_grasp, const double* grasp_rel,
                          const double* grasp_half);
+/* Joint types of an arm scene: joint_type[l] (l = arm * dof + k) 0 = revolute, 1 = prismatic.
+ * A prismatic joint is a chain joint T <- T Tz(q) C (the reference's slide joints,
+ * mujoco_env.py worlds with MJCF slide joints; URDF prismatic joints of pinocchio_env.py), with
+ * C as in mrcc_scene_create_arm_chains; DH scenes reject it (MRCC_ERR_UNSUPPORTED).  travel[l]
+ * bounds |q| of a prismatic joint (normally max(|lower|, |upper|) of its limits): the binary32
+ * margins hold for coordinates within it, and a configuration with |q_l| > travel[l] counts as a
+ * collision, as one with a non-finite coordinate does.  Default: every joint revolute. */
+int mrcc_scene_set_joint_types(mrcc_scene* scene, const uint8_t* joint_type, const double* travel);
 /* Arm scenes: 1 (default) = link bounding spheres cull sphere tests whose link pair (or link and
  * box) is certainly separated; 0 = every sphere pair of the pair set is tested (brute force, used
  * to measure the narrow phase against the FP32 roofline).  Results are identical. */

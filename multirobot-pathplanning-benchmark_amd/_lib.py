@@ -66,6 +66,7 @@ SIGNATURES = {
                                                 _c.POINTER(_vp)]),
     "mrcc_scene_set_modes": (_c.c_int, [_vp, _c.c_int, _ip, _c.c_int, _u8p, _dp, _dp]),
     "mrcc_scene_set_broadphase": (_c.c_int, [_vp, _c.c_int]),
+    "mrcc_scene_set_joint_types": (_c.c_int, [_vp, _u8p, _dp]),
     "mrcc_scene_destroy": (_c.c_int, [_vp]),
     "mrcc_scene_info": (_c.c_int, [_vp, _ip]),
     "mrcc_check_configs": (_c.c_int, [_vp, _vp, _i64, _i64, _i64, _vp, _u32, _vp, _vp]),

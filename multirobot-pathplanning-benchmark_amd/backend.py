@@ -72,6 +72,13 @@ class Scene:
             raise ValueError(f"unknown scene kind {desc.kind}")
         del keep
         self.handle = h
+        if desc.kind == "arms" and getattr(desc, "joint_type", None) is not None:
+            jt = np.ascontiguousarray(desc.joint_type, dtype=np.uint8)
+            tr = np.ascontiguousarray(desc.joint_travel(), dtype=np.float64)
+            if jt.shape != (desc.n_arms * desc.dof,):
+                raise ValueError(f"joint_type: [{desc.n_arms * desc.dof}] (one per joint)")
+            check(L.mrcc_scene_set_joint_types(self.handle, jt.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                               dptr(tr)))
         if desc.kind == "arms" and desc.modes is not None:
             self.set_modes(desc.modes)
         self.broadphase = True

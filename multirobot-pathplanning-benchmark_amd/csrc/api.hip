@@ -105,6 +105,8 @@ struct ArmHost {
     int is_chain;                  // 1: chain joints (C per joint), 0: DH joints
     std::vector<double> base, dh;  // dh: [L*3] d a alpha (DH joints)
     std::vector<double> chain;     // [L*12] C = [R | t] rows of each chain joint
+    std::vector<double> travel;    // [L] |q| bound of each prismatic chain joint (0: revolute)
+    unsigned long long prism = 0;  // bit l: chain joint l is prismatic
     std::vector<int> counts;
     std::vector<double> sph;       // [S*4]
     std::vector<double> boxes;     // [nb*6] centre half
@@ -140,6 +142,7 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
                 const double* C = H.chain.data() + 12 * l;
                 for (int i = 0; i < 12; ++i) dh4[12 * l + i] = C[i];
                 reach += sqrt(C[3] * C[3] + C[7] * C[7] + C[11] * C[11]);
+                if ((H.prism >> l) & 1ull) reach += H.travel[l];  // translation |q| <= travel
                 continue;
             }
             mrcc::SinCos t = mrcc::sincos_f64(H.dh[3 * l + 2]);
@@ -328,6 +331,9 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
            o_b64 = P.add(box64), o_bch64 = P.add(bx), o_bch32 = P.add(boxch32), o_clu = P.add(clu32),
            o_ma = P.add(H.m_active), o_mg = P.add(H.m_grasp), o_mr = P.add(H.m_rel),
            o_mh = P.add(H.m_half);
+    std::vector<double> travel(H.travel);
+    travel.resize(L, 0.0);
+    const size_t o_tr = P.add(travel);
     const int bp = sc->dev_mem ? sc->arm.broadphase : 1;  // keep the setting across mode updates
     void* old = sc->dev_mem;
     int rc = upload(sc, P);
@@ -337,6 +343,7 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     ArmView& V = sc->arm;
     V.na = na; V.dof = dof; V.D = na * dof; V.L = L; V.S = S; V.nb = nb; V.nm = H.nm; V.nslot = H.nslot;
     V.chain = H.is_chain;
+    V.prism = H.prism;
     V.broadphase = bp;
     // binary32 margins: squared distances near contact are off by <= ~2^-21 sqrt(3) R B; box SAT
     // axis values by <= ~2^-21 (B + 3 H)
@@ -357,6 +364,7 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     V.m_grasp = at<unsigned char>(d, o_mg);
     V.m_rel = at<double>(d, o_mr);
     V.m_half = at<double>(d, o_mh);
+    V.travel = at<double>(d, o_tr);
     V.Mcull = (float)(ldexp(1.0, -18) * (1.0 + B) * (1.0 + 2.0 * Rl_max));
     for (int k = 0; k < 3; ++k) {  // union of the obstacle boxes, rounded outward
         double lo = INFINITY, hi = -INFINITY;
@@ -395,6 +403,7 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
         K.na = na; K.dof = dof; K.D = V.D; K.L = L; K.S = S; K.nb = nb; K.nm = nm; K.NP = NP; K.nclu = nclu;
         K.nslot = H.nslot;
         K.chain = V.chain;
+        K.prism = V.prism;
         K.broadphase = bp;
         K.grasp = V.grasp;
         K.img_bytes = lay.bytes;
@@ -407,6 +416,7 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
         K.o_img = (int)o_img; K.o_box32 = (int)o_b32; K.o_box64 = (int)o_b64; K.o_boxch32 = (int)o_bch32;
         K.o_boxch64 = (int)o_bch64; K.o_sph64 = (int)o_s64; K.o_sphlo = (int)o_slo; K.o_pairs = (int)o_lp;
         K.o_mact = (int)o_ma; K.o_mgrasp = (int)o_mg; K.o_mrel = (int)o_mr; K.o_mhalf = (int)o_mh;
+        K.o_travel = (int)o_tr;
     }
     // waves per workgroup: the most resident waves per CU (160 KiB of LDS) within 64 KiB per
     // workgroup; the scene is staged once per workgroup
@@ -856,6 +866,30 @@ int mrcc_scene_set_modes(mrcc_scene* scene, int n_modes, const int32_t* n_active
     H.m_grasp.assign(has_grasp, has_grasp + nms);
     H.m_rel.assign(grasp_rel, grasp_rel + 12 * nms);
     H.m_half.assign(grasp_half, grasp_half + 3 * nms);
+    HIP_OK(hipDeviceSynchronize());  // kernels may still read the old arrays
+    return build_arm_device(&box->rec, H);
+}
+
+int mrcc_scene_set_joint_types(mrcc_scene* scene, const uint8_t* joint_type, const double* travel) {
+    CHECK_INJECT();
+    if (!scene || !joint_type || !travel) return fail(MRCC_ERR_INVALID, "null argument");
+    SceneBox* box = (SceneBox*)scene;
+    if (box->rec.kind != KIND_ARMS) return fail(MRCC_ERR_UNSUPPORTED, "joint types apply to arm scenes");
+    ArmHost& H = box->arm_host;
+    const int L = H.na * H.dof;
+    unsigned long long prism = 0;
+    for (int l = 0; l < L; ++l) {
+        if (joint_type[l] > 1) return fail(MRCC_ERR_INVALID, "joint %d: type %d is neither 0 (revolute) nor 1 (prismatic)", l, joint_type[l]);
+        if (joint_type[l] == 0) continue;
+        if (!H.is_chain) return fail(MRCC_ERR_UNSUPPORTED, "prismatic joints need a chain scene (mrcc_scene_create_arm_chains)");
+        if (!(travel[l] >= 0.0) || !std::isfinite(travel[l]))
+            return fail(MRCC_ERR_INVALID, "joint %d: travel must be finite and >= 0", l);
+        prism |= 1ull << l;
+    }
+    H.prism = prism;
+    H.travel.assign(L, 0.0);
+    for (int l = 0; l < L; ++l)
+        if ((prism >> l) & 1ull) H.travel[l] = travel[l];
     HIP_OK(hipDeviceSynchronize());  // kernels may still read the old arrays
     return build_arm_device(&box->rec, H);
 }

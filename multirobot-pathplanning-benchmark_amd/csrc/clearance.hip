@@ -5,7 +5,8 @@
 // with the pair set of the boolean check (links >= 2 apart in one arm, all links of different
 // arms).  One waypoint per wavefront.  The backward recomputes the forward kinematics, gathers the
 // force on every sphere centre (d cost / d c) in LDS and maps it to the joints with
-// d c / d q_k = z_{k-1} x (c - o_{k-1}) for spheres on links >= k of the same arm.
+// d c / d q_k = z_{k-1} x (c - o_{k-1}) (revolute joints) or z_{k-1} (prismatic chain joints) for
+// spheres on links >= k of the same arm.
 #include "engine.hpp"
 
 namespace {
@@ -37,8 +38,9 @@ __device__ inline void clr_fk(const ArmView& V, const ClrLds& W, const float* q)
     if (lane < V.D) {
         float s, c;
         sincosf(q[lane], &s, &c);
-        W.cs[lane] = c;
-        W.sn[lane] = s;
+        const bool pr = (V.prism >> lane) & 1ull;  // prismatic: Tz(q) C = Rz(0) C + q e_z
+        W.cs[lane] = pr ? 1.f : c;
+        W.sn[lane] = pr ? 0.f : s;
     }
     __syncthreads();
     if (lane < V.na) {
@@ -48,7 +50,7 @@ __device__ inline void clr_fk(const ArmView& V, const ClrLds& W, const float* q)
             const int l = lane * V.dof + k;
             const float cs = W.cs[l], sn = W.sn[l];
             float A[3][4];
-            if (V.chain) {  // A = Rz(q) C
+            if (V.chain) {  // A = Rz(q) C (revolute), Tz(q) C (prismatic)
                 const double* C = V.dh + 12 * l;
                 for (int c = 0; c < 4; ++c) {
                     const float c0 = (float)C[c], c1 = (float)C[4 + c];
@@ -56,6 +58,7 @@ __device__ inline void clr_fk(const ArmView& V, const ClrLds& W, const float* q)
                     A[1][c] = sn * c0 + cs * c1;
                     A[2][c] = (float)C[8 + c];
                 }
+                if ((V.prism >> l) & 1ull) A[2][3] += q[l];
             } else {
                 const double* p = V.dh + 4 * l;
                 const float d = (float)p[0], a = (float)p[1], ca = (float)p[2], sa = (float)p[3];
@@ -196,7 +199,12 @@ __global__ void __launch_bounds__(64) k_clearance(ArmView V, const float* q, lon
                 }
                 float g = 0.f;
                 const int s0 = V.link_first[a * V.dof + k], s1 = V.link_first[(a + 1) * V.dof];
+                const bool pr = (V.prism >> lane) & 1ull;
                 for (int s = s0; s < s1; ++s) {
+                    if (pr) {  // d c / d q = z: g = z . f
+                        g += z0 * W.F[3 * s] + z1 * W.F[3 * s + 1] + z2 * W.F[3 * s + 2];
+                        continue;
+                    }
                     const float4 cs = W.sph[s];
                     const float w0 = cs.x - o0, w1 = cs.y - o1, w2 = cs.z - o2;
                     const float f0 = W.F[3 * s], f1 = W.F[3 * s + 1], f2 = W.F[3 * s + 2];

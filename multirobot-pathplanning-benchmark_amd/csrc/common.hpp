@@ -130,7 +130,16 @@ __host__ __device__ inline ChainCoef chain_coef_f64(double cs, double sn, double
     return k;
 }
 
-__host__ __device__ inline void chain_step_f64(double* X, const double* C, SinCos j) {
+// Row 2, column 3 of a chain joint's step: C23 for a revolute joint (A = Rz(q) C), C23 + q for a
+// prismatic joint (A = Tz(q) C, whose rows 0 and 1 are those of Rz(0) C).
+__host__ __device__ inline double chain_a23_f64(const double* C, bool prismatic, double q) {
+    return prismatic ? C[11] + q : C[11];
+}
+
+// X <- X * A for a chain joint with joint value q: A = Rz(q) C (revolute, j = sincos(q)) or
+// A = Tz(q) C (prismatic, j = {cos 0, sin 0} = {1, 0}).
+__host__ __device__ inline void chain_step_f64(double* X, const double* C, SinCos j, bool prismatic = false,
+                                               double q = 0.0) {
 #pragma clang fp contract(off)
     double A[3][4];
 #pragma unroll
@@ -138,7 +147,7 @@ __host__ __device__ inline void chain_step_f64(double* X, const double* C, SinCo
         const ChainCoef k = chain_coef_f64(j.c, j.s, C[c], C[4 + c]);
         A[0][c] = k.a0;
         A[1][c] = k.a1;
-        A[2][c] = C[8 + c];
+        A[2][c] = c == 3 ? chain_a23_f64(C, prismatic, q) : C[8 + c];
     }
     double Y[12];
 #pragma unroll
@@ -402,6 +411,8 @@ struct ArmView {
     const double* base;      // [na*12]
     int chain;               // 1: joints are chain joints (dh holds C, 12 per joint)
     const double* dh;        // [L*4] d a cos(alpha) sin(alpha), or [L*12] C of each chain joint
+    unsigned long long prism;  // bit l: chain joint l is prismatic (Tz(q) C)
+    const double* travel;    // [L] |q| bound of each prismatic joint
     const int* link_first;   // [L+1]
     const int* sph_link;     // [S] global link index
     const int* sph_lo;       // [S] first allowed partner sphere (same arm, links >= link+2)
@@ -446,11 +457,12 @@ struct ArmK {
     int broadphase;  // 1: link bounding spheres cull sphere tests
     int grasp;       // 1: some mode holds a box
     int img_bytes;   // size of the static LDS image (multiple of 16)
+    unsigned long long prism;  // bit l: chain joint l is prismatic (Tz(q) C)
     float Msph, Mobb, Mcull, Marm;
     float bb_lo[3], bb_hi[3];
     const unsigned char* g;
     int o_img, o_box32, o_box64, o_boxch32, o_boxch64, o_sph64, o_sphlo, o_pairs, o_mact, o_mgrasp,
-        o_mrel, o_mhalf;
+        o_mrel, o_mhalf, o_travel;
     __device__ const float4* img() const { return (const float4*)(g + o_img); }
     __device__ const float* box32() const { return (const float*)(g + o_box32); }       // [nb*8]
     __device__ const double* box64() const { return (const double*)(g + o_box64); }     // [nb*6] lo hi
@@ -463,6 +475,7 @@ struct ArmK {
     __device__ const unsigned char* m_grasp() const { return g + o_mgrasp; }            // [nm*nslot]
     __device__ const double* m_rel() const { return (const double*)(g + o_mrel); }      // [nm*nslot*12]
     __device__ const double* m_half() const { return (const double*)(g + o_mhalf); }    // [nm*nslot*3]
+    __device__ const double* travel() const { return (const double*)(g + o_travel); }   // [L]
 };
 
 // batch of edges as seen by kernels.  The scalars N0, Ns0, Nm0, md0 stand for the arrays N, N_start,

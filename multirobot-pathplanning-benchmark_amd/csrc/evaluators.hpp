@@ -356,11 +356,14 @@ __host__ __device__ inline ArmStaticLayout arm_static_layout(int L, int na, int 
 
 // Wave region: X [12 L] doubles, mode boxes G64 [12 nslot] doubles (scenes with boxes in some
 // mode only), the band-pair queue, one area that holds the joints' kinematics coefficients [8 D]
-// doubles during the kinematics and the sphere table [S] float4 afterwards, link spheres [L]
+// doubles and the chain joints' A23 [D] doubles during the kinematics and the sphere table [S] float4 afterwards, link spheres [L]
 // float4, three candidate lists [64] int, mode boxes G32 [16 nslot] float (same scenes).  Every
 // part is a multiple of 16 bytes.  Mode-box slots: slot a < na is the box held by arm a, slot
 // na + j the j-th resting box.
-__host__ __device__ inline int arm_sph_bytes(int D, int S) { return 16 * S > 64 * D ? 16 * S : 64 * D; }
+__host__ __device__ inline int arm_sph_bytes(int D, int S) {
+    const int fk = (72 * D + 15) & ~15;  // 8 coefficients + A23 per joint
+    return 16 * S > fk ? 16 * S : fk;
+}
 
 __host__ __device__ inline int arm_wave_bytes(int D, int L, int nslot, int S, int grasp) {
     return 96 * L + (grasp ? 96 * nslot : 0) + 8 * ARM_PEND_MAX + arm_sph_bytes(D, S) + 16 * L + 3 * 4 * 64 +
@@ -376,8 +379,8 @@ __host__ __device__ inline size_t arm_lds_bytes(int D, int L, int na, int nb, in
 
 struct ArmLds {
     // per wave
-    double* fkc;    // [D*8] per joint: cos, sin, and the products of the DH step (kinematics
-                    // phase; shares memory with sph)
+    double* fkc;    // [D*8] per joint: cos, sin, and the products of the DH step, then [D] A23 of
+                    // the chain joints (kinematics phase; shares memory with sph)
     double* X;
     double* G64;
     float4* sph;
@@ -653,11 +656,14 @@ __device__ inline bool arm_state_collides(const ArmK& V, const ArmLds& W, const 
     //    step A = Rz(q) C, column by column: [A00, A10, A01, A11, A02, A12, A03, A13].  DH joints:
     //    dh_row_f64's products; chain joints: chain_coef_f64 (A0c = cs C0c - sn C1c,
     //    A1c = sn C0c + cs C1c), which gives the DH products exactly for C of a DH joint.
+    //    A prismatic chain joint (bit l of V.prism) has A = Tz(q) C: cs = 1, sn = 0 and
+    //    A23 = C23 + q; |q| beyond its travel counts as a collision, as a non-finite coordinate.
     bool bad = false;
     if (lane < Dact) {
         bad = !__builtin_isfinite(qv);
+        const bool pr = CHAIN && ((V.prism >> lane) & 1ull);
         const SinCos t = sincos_f64(qv);
-        const double cs = t.c, sn = t.s;
+        const double cs = pr ? 1.0 : t.c, sn = pr ? 0.0 : t.s;
         double2* f = (double2*)(W.fkc + 8 * lane);
         if (CHAIN) {
             const double* C = W.dh + 12 * lane;
@@ -666,6 +672,8 @@ __device__ inline bool arm_state_collides(const ArmK& V, const ArmLds& W, const 
                 const ChainCoef k = chain_coef_f64(cs, sn, C[c], C[4 + c]);
                 f[c] = make_double2(k.a0, k.a1);
             }
+            W.fkc[8 * V.D + lane] = chain_a23_f64(C, pr, qv);
+            if (pr) bad = bad || fabs(qv) > V.travel()[lane];
         } else {
             const double* p = W.dh + 4 * lane;  // d a cos(alpha) sin(alpha)
             const double a = p[1], ca = p[2], sa = p[3];
@@ -696,13 +704,16 @@ __device__ inline bool arm_state_collides(const ArmK& V, const ArmLds& W, const 
                 const int l0 = a * dof;
                 // joint k's coefficients are loaded one step ahead, off the chain's critical path
                 double2 kk = ((const double2*)(W.fkc + 8 * l0))[c];
-                double k2 = chain ? W.dh[12 * l0 + 8 + c] : (c == 0 ? 0.0 : W.dh[4 * l0 + k2i]);
+                // chain joints: A2c = C2c for c < 3, A23 from the kinematics coefficients
+                const double* a2 = c == 3 ? W.fkc + 8 * V.D : W.dh + 8 + c;
+                const int a2s = c == 3 ? 1 : 12;
+                double k2 = chain ? a2[a2s * l0] : (c == 0 ? 0.0 : W.dh[4 * l0 + k2i]);
                 for (int k = 0; k < dof; ++k) {
                     const int l = l0 + k;
                     const double k0 = kk.x, k1 = kk.y, kz = k2;
                     if (k + 1 < dof) {
                         kk = ((const double2*)(W.fkc + 8 * (l + 1)))[c];
-                        k2 = chain ? W.dh[12 * (l + 1) + 8 + c] : (c == 0 ? 0.0 : W.dh[4 * (l + 1) + k2i]);
+                        k2 = chain ? a2[a2s * (l + 1)] : (c == 0 ? 0.0 : W.dh[4 * (l + 1) + k2i]);
                     }
                     double y;
                     {

@@ -921,10 +921,14 @@ __global__ void k_arm_fk(ArmView V, const double* qv, long long n, long long rs,
     for (int k = 0; k < 12; ++k) X[k] = V.base[12 * a + k];
     for (int k = 0; k < V.dof; ++k) {
         const int l = a * V.dof + k;
-        SinCos j = sincos_f64(qv[r * rs + (long long)l * ds]);
+        const double q = qv[r * rs + (long long)l * ds];
         if (V.chain) {
-            chain_step_f64(X, V.dh + 12 * l, j);
+            const bool pr = (V.prism >> l) & 1ull;
+            SinCos j = sincos_f64(q);
+            if (pr) j.s = 0.0, j.c = 1.0;
+            chain_step_f64(X, V.dh + 12 * l, j, pr, q);
         } else {
+            const SinCos j = sincos_f64(q);
             const double* p = V.dh + 4 * l;
             dh_step_f64(X, p[0], p[1], p[2], p[3], j);
         }

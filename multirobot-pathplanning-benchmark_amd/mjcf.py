@@ -6,15 +6,21 @@ The mujoco environments of the reference load MJCF worlds (mujoco_env.py; e.g.
 ``collision_tolerance`` (mujoco_env.py:404-407).  ``load_mjcf_arms`` turns such a world into an
 arm scene of the engine (``SceneDesc`` with ``joint_fixed``):
 
-* every attached (or inline) robot is a serial chain of hinge joints.  MJCF gives body k's frame
-  as B_k = B_{k-1} P_k J_k(q), with P_k the body's pos / quat and J_k the rotation about the joint
-  axis through the joint's pos.  Writing J_k = pre_k Rz(q) post_k (pre_k carries z onto the axis),
-  the frames F_k = B_k pre_{k+1} (F_last = B_last) follow F_k = F_{k-1} Rz(q_k) C_k with
-  C_k = post_k pre_{k+1} (C_last = post_last); the arm's base is F_0 = B_0 pre_1;
-* collision geoms (capsules and spheres; visual geoms, contype = conaffinity = 0, are skipped)
-  become spheres in F_k: a capsule of radius r and half-length h is covered by n spheres spaced
-  s = 2h / (n - 1) along its axis, of radius sqrt(r^2 + (s/2)^2), so that their union contains
-  the capsule; a link's capsules share at most 8 spheres;
+* every attached (or inline) robot is a serial chain of hinge and slide joints, one or more per
+  body (``mujoco.swap``'s ``mj_two_dim.xml``: slide x, slide y, hinge yaw on one body).  MJCF gives
+  a body's frame as B = B_parent P J_1(q_1) ... J_m(q_m), with P the body's pos / quat, J the
+  rotation about a hinge's axis through its pos, or the translation along a slide's axis.  Each
+  joint is one link.  Writing J_k = pre_k Z(q) post_k (pre_k carries z onto the axis, Z = Rz for a
+  hinge and Tz for a slide, pre_k also carrying P for the first joint of a body), the frames
+  F_k = B_k pre_{k+1} (F_last = B_last) follow F_k = F_{k-1} Z(q_k) C_k with C_k = post_k pre_{k+1}
+  (C_last = post_last); the arm's base is F_0 = B_0 pre_1; slide joints are the engine's prismatic
+  joints (``SceneDesc.joint_type``), and need a range (their travel);
+* collision geoms (capsules, cylinders and spheres; visual geoms, contype = conaffinity = 0, are
+  skipped) become spheres in F_k of the body's last joint: a capsule of radius r and half-length h
+  is covered by n spheres spaced s = 2h / (n - 1) along its axis, of radius sqrt(r^2 + (s/2)^2),
+  so that their union contains the capsule; a cylinder by n spheres at the centres of n slabs of
+  height s = 2h / n, radius sqrt(r^2 + (s/2)^2); a link's capsules and cylinders share at most 8
+  spheres;
 * static box geoms and the boxes of free bodies (at their initial pose) become obstacle boxes
   (axis-aligned when their rotation is a multiple of 90 degrees about the axes; otherwise their
   axis-aligned bounding box); a plane becomes a thick box below it;
@@ -176,6 +182,8 @@ def _link_geoms(body: ET.Element, defs: _Defaults, cls: str, deg: bool) -> List[
 def _allot(lengths: List[float], budget: int) -> List[int]:
     """Spheres per capsule: at least one each, the rest by length, at most `budget` in total."""
     n = [1] * len(lengths)
+    if not lengths:
+        return n
     for _ in range(budget - len(lengths)):
         i = int(np.argmax([l / (k + 1) for l, k in zip(lengths, n)]))
         if lengths[i] <= 0:
@@ -185,7 +193,8 @@ def _allot(lengths: List[float], budget: int) -> List[int]:
 
 
 def _chain(root_body: ET.Element, defs: _Defaults, cls: str):
-    """The hinge chain below `root_body`: list of (body element, class, joint attrs)."""
+    """The serial chain below `root_body`: list of (body element, class, [joint attrs]) with one or
+    more hinge / slide joints per body, in the order MuJoCo applies them."""
     chain = []
     body, c = root_body, cls
     while True:
@@ -201,55 +210,87 @@ def _chain(root_body: ET.Element, defs: _Defaults, cls: str):
             return chain
         body, c = nxt
         joints = body.findall("joint")
-        if len(joints) != 1:
-            raise ValueError(f"body {body.get('name')}: exactly one joint per body is supported")
-        ja = defs.attrs(joints[0], joints[0].get("class", c))
-        if ja.get("type", "hinge") != "hinge":
-            raise ValueError(f"body {body.get('name')}: only hinge joints are supported")
-        chain.append((body, c, ja))
+        if not joints:
+            raise ValueError(f"body {body.get('name')}: a chain body without a joint is not supported")
+        jas = [defs.attrs(j, j.get("class", c)) for j in joints]
+        for ja in jas:
+            if ja.get("type", "hinge") not in ("hinge", "slide"):
+                raise ValueError(f"body {body.get('name')}: only hinge and slide joints are supported")
+        chain.append((body, c, jas))
+
+
+def _cylinder_spheres(T_geom: np.ndarray, r: float, h: float, n: int) -> List[Tuple[np.ndarray, float]]:
+    """n spheres covering a cylinder of radius r and half-length h (axis z of T_geom): the axis cut
+    into n slabs of height s = 2h / n, one sphere of radius sqrt(r^2 + (s/2)^2) at each slab's
+    centre, which contains the slab."""
+    n = max(1, n)
+    s = 2 * h / n
+    rr = math.sqrt(r * r + (s / 2) ** 2)
+    return [((T_geom @ np.array([0, 0, -h + s / 2 + i * s, 1.0]))[:3], rr) for i in range(n)]
 
 
 def _robot(root_body: ET.Element, defs: _Defaults, cls: str, B0: np.ndarray, degrees: bool, budget: int):
-    """Base transform, joint transforms C_k, spheres per link (in F_k), limits of one chain."""
+    """Base transform, joint transforms C_k, spheres per link (in F_k), limits, link names and joint
+    types (0 hinge, 1 slide) of one chain.  A body with several joints gives one link per joint; its
+    geoms belong to the link of its last joint."""
     chain = _chain(root_body, defs, cls)
     if not chain:
-        raise ValueError("no hinge joints below the robot's base body")
-    pre, post, P = [], [], []
-    for body, c, ja in chain:
-        P.append(_body_pose(defs.attrs(body, c), degrees))
+        raise ValueError("no joints below the robot's base body")
+    joints = []  # (body, class, joint attrs, body pose before the joint, last joint of its body)
+    for body, c, jas in chain:
+        P = _body_pose(defs.attrs(body, c), degrees)
+        for i, ja in enumerate(jas):
+            joints.append((body, c, ja, P if i == 0 else np.eye(4), i + 1 == len(jas)))
+    pre, post, types = [], [], []
+    for body, c, ja, P, _ in joints:
         axis = _floats(ja.get("axis"), 3, [0, 0, 1])
-        jp = _floats(ja.get("pos"), 3, [0, 0, 0])
         U = _z_to(axis)
-        pre.append(P[-1] @ _T(None, jp) @ _T(U))
-        post.append(_T(U.T) @ _T(None, -jp))
-    dof = len(chain)
+        if ja.get("type", "hinge") == "slide":  # T(axis q) = U Tz(q) U^T
+            pre.append(P @ _T(U))
+            post.append(_T(U.T))
+            types.append(1)
+        else:  # T(p) Rot(axis, q) T(-p) = T(p) U Rz(q) U^T T(-p)
+            jp = _floats(ja.get("pos"), 3, [0, 0, 0])
+            pre.append(P @ _T(None, jp) @ _T(U))
+            post.append(_T(U.T) @ _T(None, -jp))
+            types.append(0)
+    dof = len(joints)
     C = [post[k] @ pre[k + 1] if k + 1 < dof else post[k] for k in range(dof)]
     base = B0 @ pre[0]
     spheres, limits = [], []
-    for k, (body, c, ja) in enumerate(chain):
-        to_F = _inv(pre[k + 1]) if k + 1 < dof else np.eye(4)  # body frame -> F_k
-        geoms = _link_geoms(body, defs, c, degrees)
-        caps = [(T, s) for t, T, s in geoms if t == "capsule"]
-        sph = [(T, s) for t, T, s in geoms if t == "sphere"]
-        other = [t for t, _, _ in geoms if t not in ("capsule", "sphere")]
-        if other:
-            raise ValueError(f"body {body.get('name')}: collision geom types {other} are not supported")
-        n_caps = _allot([2 * s[1] / max(s[0], 1e-9) for _, s in caps], budget - len(sph))
+    for k, (body, c, ja, _, last) in enumerate(joints):
         link = []
-        for (T, s), n in zip(caps, n_caps):
-            link += _capsule_spheres(to_F @ T, float(s[0]), float(s[1]), n)
-        for T, s in sph:
-            link.append(((to_F @ T)[:3, 3], float(s[0])))
-        if len(link) > budget:
-            raise ValueError(f"body {body.get('name')}: more than {budget} spheres")
+        if last:
+            to_F = _inv(pre[k + 1]) if k + 1 < dof else np.eye(4)  # body frame -> F_k
+            geoms = _link_geoms(body, defs, c, degrees)
+            caps = [(T, s) for t, T, s in geoms if t == "capsule"]
+            cyls = [(T, s) for t, T, s in geoms if t == "cylinder"]
+            sph = [(T, s) for t, T, s in geoms if t == "sphere"]
+            other = [t for t, _, _ in geoms if t not in ("capsule", "cylinder", "sphere")]
+            if other:
+                raise ValueError(f"body {body.get('name')}: collision geom types {other} are not supported")
+            n_alloc = _allot([2 * s[1] / max(s[0], 1e-9) for _, s in caps + cyls], budget - len(sph))
+            for (T, s), n in zip(caps, n_alloc[:len(caps)]):
+                link += _capsule_spheres(to_F @ T, float(s[0]), float(s[1]), n)
+            for (T, s), n in zip(cyls, n_alloc[len(caps):]):
+                link += _cylinder_spheres(to_F @ T, float(s[0]), float(s[1]), n)
+            for T, s in sph:
+                link.append(((to_F @ T)[:3, 3], float(s[0])))
+            if len(link) > budget:
+                raise ValueError(f"body {body.get('name')}: more than {budget} spheres")
         spheres.append(link)
         rng = ja.get("range")
-        if rng is None:
+        if ja.get("type", "hinge") == "slide":
+            if rng is None:
+                raise ValueError(f"body {body.get('name')}: a slide joint needs a range (its travel)")
+            limits.append(_floats(rng, 2, [0, 0]))
+        elif rng is None:
             limits.append(np.array([-math.pi, math.pi]))
         else:
             lim = _floats(rng, 2, [-math.pi, math.pi])
             limits.append(np.radians(lim) if degrees else lim)
-    return base, np.stack(C), spheres, np.stack(limits), [b.get("name") for b, _, _ in chain]
+    names = [ja.get("name", body.get("name")) for body, _, ja, _, _ in joints]
+    return base, np.stack(C), spheres, np.stack(limits), names, types
 
 
 def _rows12(T: np.ndarray) -> np.ndarray:
@@ -336,18 +377,22 @@ def load_mjcf_arms(path: str, tolerance: float = 0.0, spheres_per_link: int = MA
             for c, rad in link:
                 sph.append([c[0], c[1], c[2], rad + tolerance / 2])
     limits = np.concatenate([r[3] for r in robots]).T.copy()
+    jt = np.asarray([t for r in robots for t in r[5]], dtype=np.uint8)
     desc = SceneDesc(kind="arms", limits=limits, n_arms=n_arms, dof=dof, base=base,
                      dh=np.zeros((n_arms, dof, 3)), link_sphere_count=np.asarray(counts, dtype=np.int32),
                      sphere_local=np.asarray(sph, dtype=np.float64).reshape(-1, 4),
-                     boxes=np.asarray(boxes, dtype=np.float64).reshape(-1, 6), joint_fixed=joint_fixed)
+                     boxes=np.asarray(boxes, dtype=np.float64).reshape(-1, 6), joint_fixed=joint_fixed,
+                     joint_type=jt if jt.any() else None)
     desc.link_names = [n for r in robots for n in r[4]]
     return desc
 
 
 def body_frames(path: str, q: np.ndarray) -> List[np.ndarray]:
-    """World frames B_k of every robot body of an MJCF world for the joint vector q (robots in
-    file order, joints in chain order), composed directly from the MJCF attributes:
-    B_k = B_{k-1} P_k T(p) Rot(axis, q_k) T(-p).  An independent check of ``load_mjcf_arms``."""
+    """World frames of every robot joint of an MJCF world for the joint vector q (robots in file
+    order, joints in chain order), composed directly from the MJCF attributes: a body's frame is its
+    parent's times its pose P, then times T(p) Rot(axis, q) T(-p) for each hinge and T(axis q) for
+    each slide joint, in order; the frame after each joint is listed.  An independent check of
+    ``load_mjcf_arms``."""
     world = ET.parse(path).getroot()
     here = os.path.dirname(os.path.abspath(path))
     wdeg = _degrees(world)
@@ -357,13 +402,18 @@ def body_frames(path: str, q: np.ndarray) -> List[np.ndarray]:
     qi = [0]
 
     def chain_frames(rb, defs, cls, B, deg):
-        for body, c, ja in _chain(rb, defs, cls):
-            P = _body_pose(defs.attrs(body, c), deg)
-            jp = _floats(ja.get("pos"), 3, [0, 0, 0])
-            J = _T(None, jp) @ _T(_axis_angle(_floats(ja.get("axis"), 3, [0, 0, 1]), q[qi[0]])) @ _T(None, -jp)
-            qi[0] += 1
-            B = B @ P @ J
-            frames.append(B)
+        for body, c, jas in _chain(rb, defs, cls):
+            B = B @ _body_pose(defs.attrs(body, c), deg)
+            for ja in jas:
+                axis = _floats(ja.get("axis"), 3, [0, 0, 1])
+                if ja.get("type", "hinge") == "slide":
+                    J = _T(None, axis / np.linalg.norm(axis) * q[qi[0]])
+                else:
+                    jp = _floats(ja.get("pos"), 3, [0, 0, 0])
+                    J = _T(None, jp) @ _T(_axis_angle(axis, q[qi[0]])) @ _T(None, -jp)
+                qi[0] += 1
+                B = B @ J
+                frames.append(B)
 
     def walk(el, B, cls):
         for b in el.findall("body"):

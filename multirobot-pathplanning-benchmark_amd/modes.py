@@ -80,11 +80,17 @@ def to12(T: np.ndarray) -> np.ndarray:
 
 def end_effector(desc, arm: int, q_arm: np.ndarray) -> np.ndarray:
     """World frame of the last link of `arm` for its joint values q_arm (binary64 products of 4 x 4
-    transforms: DH joints Rz(q) Tz(d) Tx(a) Rx(alpha), chain joints Rz(q) C)."""
+    transforms: DH joints Rz(q) Tz(d) Tx(a) Rx(alpha), chain joints Rz(q) C or, prismatic, Tz(q) C)."""
     T = _from12(desc.base[arm])
+    jt = getattr(desc, "joint_type", None)
     for k in range(desc.dof):
         if getattr(desc, "joint_fixed", None) is not None:
-            T = T @ _rz(q_arm[k]) @ _from_rows12(desc.joint_fixed[arm, k])
+            if jt is not None and jt[arm * desc.dof + k] == 1:  # prismatic: Tz(q) C
+                J = np.eye(4)
+                J[2, 3] = q_arm[k]
+            else:
+                J = _rz(q_arm[k])
+            T = T @ J @ _from_rows12(desc.joint_fixed[arm, k])
         else:
             d, a, al = desc.dh[arm, k]
             ca, sa = math.cos(al), math.sin(al)

@@ -49,10 +49,19 @@ class SceneDesc:
     # arms of any joint geometry: [n_arms, dof, 12] fixed transform C after Rz(q) of each joint
     # (rows R00 R01 R02 t0, R10 R11 R12 t1, R20 R21 R22 t2); when set, dh is unused
     joint_fixed: Optional[np.ndarray] = None
+    # chain arms: [n_arms * dof] joint types, 0 revolute (Rz(q) C), 1 prismatic (Tz(q) C); None: all
+    # revolute
+    joint_type: Optional[np.ndarray] = None
 
     @property
     def dim(self) -> int:
         return int(self.limits.shape[1])
+
+    def joint_travel(self) -> np.ndarray:
+        """[n_arms * dof] travel of each prismatic joint, max(|lower|, |upper|) of its limits (0 for
+        revolute joints): a configuration beyond it counts as a collision."""
+        jt = np.zeros(self.dim, dtype=np.uint8) if self.joint_type is None else np.asarray(self.joint_type)
+        return np.where(jt == 1, np.maximum(np.abs(self.limits[0]), np.abs(self.limits[1])), 0.0)
 
     @property
     def n_robots(self) -> int:

@@ -38,6 +38,8 @@ struct or_scene {
     double dh[MAX_ARMS][MAX_DOF][5]; /* d a cos(alpha) sin(alpha) alpha */
     int is_chain;           /* 1: joints are T <- T Rz(q) C with C below */
     double chain[MAX_ARMS][MAX_DOF][12]; /* C rows: R00 R01 R02 t0, R10 .. t1, R20 .. t2 */
+    uint8_t prism[MAX_ARMS * MAX_DOF];    /* 1: prismatic chain joint, T <- T Tz(q) C */
+    double travel[MAX_ARMS * MAX_DOF];    /* prismatic joints: |q| <= travel, else a collision */
     int* sph_link;          /* [S] global link index arm*dof+link */
     double* sph;            /* [S,4] local xyz r */
     int link_first[MAX_ARMS * MAX_DOF + 1];
@@ -390,6 +392,20 @@ or_scene* or_scene_arm_chains(int n_arms, int dof, const double* base, const dou
     return s;
 }
 
+int or_scene_set_joint_types(or_scene* s, const uint8_t* types, const double* travel) {
+    if (s->kind != K_ARMS) return -1;
+    const int L = s->na * s->dof;
+    for (int l = 0; l < L; ++l) {
+        if (types[l] > 1 || (types[l] == 1 && (!s->is_chain || !(travel[l] >= 0.0) || !isfinite(travel[l]))))
+            return -1;
+    }
+    for (int l = 0; l < L; ++l) {
+        s->prism[l] = types[l];
+        s->travel[l] = types[l] ? travel[l] : 0.0;
+    }
+    return 0;
+}
+
 int or_scene_set_modes(or_scene* s, int n_modes, const int32_t* n_active, int n_rest,
                        const uint8_t* has_grasp, const double* grasp_rel,
                        const double* grasp_half) {
@@ -409,17 +425,19 @@ int or_scene_set_modes(or_scene* s, int n_modes, const int32_t* n_active, int n_
     return 0;
 }
 
-/* T = T * Rz(q) C for one chain joint: A = Rz(q) C, rows A0c = cs C0c - sn C1c,
- * A1c = sn C0c + cs C1c, A2c = C2c, then the entries of T * A as dh_step. */
-static void chain_step(double X[12], const double C[12], double q) {
-    double sn, cs;
-    or_sincos(q, &sn, &cs);
+/* T = T * Rz(q) C for one revolute chain joint: A = Rz(q) C, rows A0c = cs C0c - sn C1c,
+ * A1c = sn C0c + cs C1c, A2c = C2c, then the entries of T * A as dh_step.  A prismatic joint is
+ * T = T * Tz(q) C: the same rows with cs = 1, sn = 0, and A23 = C23 + q. */
+static void chain_step(double X[12], const double C[12], double q, int prismatic) {
+    double sn = 0.0, cs = 1.0;
+    if (!prismatic) or_sincos(q, &sn, &cs);
     double A[3][4];
     for (int c = 0; c < 4; ++c) {
         A[0][c] = cs * C[c] - sn * C[4 + c];
         A[1][c] = sn * C[c] + cs * C[4 + c];
         A[2][c] = C[8 + c];
     }
+    if (prismatic) A[2][3] = C[11] + q;
     double Y[12];
     for (int r = 0; r < 3; ++r) {
         const double* R = X + 3 * r;
@@ -455,7 +473,7 @@ int or_arm_fk(const or_scene* s, const double* q, double* out) {
         double X[12];
         memcpy(X, s->base[a], sizeof(X));
         for (int k = 0; k < s->dof; ++k) {
-            if (s->is_chain) chain_step(X, s->chain[a][k], q[a * s->dof + k]);
+            if (s->is_chain) chain_step(X, s->chain[a][k], q[a * s->dof + k], s->prism[a * s->dof + k]);
             else dh_step(X, s->dh[a][k], q[a * s->dof + k]);
             memcpy(out + 12 * (a * s->dof + k), X, sizeof(X));
         }
@@ -540,8 +558,12 @@ static const double IDENT12[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
 
 static int arm_free(const or_scene* s, const double* q, int mode, uint32_t mask) {
     if (mode < 0 || mode >= s->nm) return -1;
-    if (!all_finite(q, s->D)) return 0;
     const int na = s->m_active[mode], dof = s->dof, L = s->na * dof, ns = s->nslot;
+    /* the coordinates of the mode's arms: non-finite, or a prismatic joint beyond its travel, is a
+     * collision */
+    if (!all_finite(q, na * dof)) return 0;
+    for (int l = 0; l < na * dof; ++l)
+        if (s->prism[l] && fabs(q[l]) > s->travel[l]) return 0;
     double* X = (double*)malloc(sizeof(double) * 12 * (size_t)L);
     double* C = (double*)malloc(sizeof(double) * 3 * (size_t)(s->S + 1));
     double G[MAX_SLOTS][12];
@@ -769,13 +791,18 @@ static double clearance(const or_scene* s, const double* q, double m, double* gr
         }
     }
     if (grad) {
-        /* d c / d q_k = z_{k-1} x (c - o_{k-1}) for spheres on links >= k of the same arm */
+        /* d c / d q_k = z_{k-1} x (c - o_{k-1}) (revolute) or z_{k-1} (prismatic) for spheres on
+         * links >= k of the same arm */
         for (int a = 0; a < na; ++a)
             for (int k = 0; k < dof; ++k) {
                 const double* P = (k == 0) ? s->base[a] : X + 12 * (a * dof + k - 1);
                 double z[3] = {P[2], P[5], P[8]}, o[3] = {P[9], P[10], P[11]};
                 double g = 0.0;
                 for (int t = s->link_first[a * dof + k]; t < s->link_first[(a + 1) * dof]; ++t) {
+                    if (s->prism[a * dof + k]) {
+                        g += z[0] * F[3 * t] + z[1] * F[3 * t + 1] + z[2] * F[3 * t + 2];
+                        continue;
+                    }
                     double w[3] = {C[3 * t] - o[0], C[3 * t + 1] - o[1], C[3 * t + 2] - o[2]};
                     double j0 = z[1] * w[2] - z[2] * w[1];
                     double j1 = z[2] * w[0] - z[0] * w[2];

@@ -56,6 +56,9 @@ or_scene* or_scene_arms(int n_arms, int dof, const double* base, const double* d
 or_scene* or_scene_arm_chains(int n_arms, int dof, const double* base, const double* joint_fixed,
                               const int32_t* link_sphere_count, const double* sphere_local,
                               int n_boxes, const double* boxes);
+/* joint types of an arm scene, types [n_arms*dof]: 0 revolute, 1 prismatic (chain scenes only:
+ * T <- T Tz(q) C); a prismatic joint with |q| > travel[l] makes the configuration a collision */
+int or_scene_set_joint_types(or_scene* s, const uint8_t* types, const double* travel);
 /* modes: slots per mode = n_arms + n_rest; slot a < n_arms is the box held by arm a (pose
  * T_ee(q) rel), slot n_arms + j a box resting at world pose rel; has_grasp [n_modes*slots] */
 int or_scene_set_modes(or_scene* s, int n_modes, const int32_t* n_active, int n_rest,

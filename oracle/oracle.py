@@ -50,6 +50,7 @@ def lib():
         L.or_scene_arm_chains.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _ip, _dp, ctypes.c_int, _dp]
         L.or_scene_arm_chains.restype = ctypes.c_void_p
         L.or_scene_set_modes.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip, ctypes.c_int, _u8p, _dp, _dp]
+        L.or_scene_set_joint_types.argtypes = [ctypes.c_void_p, _u8p, _dp]
         L.or_scene_free.argtypes = [ctypes.c_void_p]
         L.or_scene_dim.argtypes = [ctypes.c_void_p]
         L.or_config_free.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, ctypes.c_uint32]
@@ -163,6 +164,12 @@ class Scene:
                 self.h = L.or_scene_arm_chains(desc.n_arms, desc.dof, pb, pjf, pcnt, psl, len(desc.boxes), pbx)
             else:
                 self.h = L.or_scene_arms(desc.n_arms, desc.dof, pb, pdh, pcnt, psl, len(desc.boxes), pbx)
+            if self.h and getattr(desc, "joint_type", None) is not None:
+                jt = np.ascontiguousarray(desc.joint_type, dtype=np.uint8)
+                tr, ptr = _d(desc.joint_travel())
+                self._keep += [jt, tr]
+                if L.or_scene_set_joint_types(self.h, jt.ctypes.data_as(_u8p), ptr) != 0:
+                    raise ValueError("oracle rejected the joint types")
             if desc.modes is not None:
                 m = desc.modes
                 na, pna = _i(m["n_active"])

@@ -5,7 +5,8 @@ kernels, so that a rule shared by those two (which pairs are tested, the mask, t
 mode, the held boxes) is checked by a third computation:
 
 * kinematics as products of 4x4 homogeneous matrices (np.cos / np.sin):
-  DH joints T_k = T_{k-1} Rz(q_k) Tz(d_k) Tx(a_k) Rx(alpha_k); chain joints T_k = T_{k-1} Rz(q_k) C_k;
+  DH joints T_k = T_{k-1} Rz(q_k) Tz(d_k) Tx(a_k) Rx(alpha_k); chain joints T_k = T_{k-1} Rz(q_k) C_k
+  (revolute) or T_{k-1} Tz(q_k) C_k (prismatic);
 * spheres fixed in their link frames; sphere vs obstacle box (axis aligned) and sphere vs held box
   (oriented) by the distance from the centre to its projection onto the box, closed (touching
   collides); sphere vs sphere closed;
@@ -20,7 +21,8 @@ mode, the held boxes) is checked by a third computation:
   held boxes; static against static (obstacles, other resting boxes) is not a state's collision;
 * robot mask: a pair counts iff every robot in it is in the mask (rai_base_env.py:560-586,
   collisions with robots outside the set are ignored);
-* non-finite coordinates collide;
+* non-finite coordinates of the mode's arms collide, and so does a prismatic joint beyond its
+  travel (chain joints T_k = T_{k-1} Tz(q_k) C_k);
 * box vs box by linear programming (scipy linprog): the largest s such that both boxes, shrunk by s
   on every face, still share a point.  s >= 0: overlap (closed); no separating-axis test.
 
@@ -81,7 +83,9 @@ def link_frames(desc, q):
         for k in range(dof):
             l = a * dof + k
             if getattr(desc, "joint_fixed", None) is not None:
-                T = T @ rot_z(q[l]) @ from_rows12(desc.joint_fixed[a, k])
+                jt = getattr(desc, "joint_type", None)
+                J = trans(0, 0, q[l]) if jt is not None and jt[l] == 1 else rot_z(q[l])  # prismatic / revolute
+                T = T @ J @ from_rows12(desc.joint_fixed[a, k])
             else:
                 d, aa, al = desc.dh[a, k]
                 T = T @ rot_z(q[l]) @ trans(0, 0, d) @ trans(aa, 0, 0) @ rot_x(al)
@@ -219,8 +223,21 @@ def decide(pairs, mask=0xFFFFFFFF, eps=1e-9):
     return None
 
 
+def out_of_range(desc, q, mode=0):
+    """A coordinate of the mode's arms is non-finite, or a prismatic joint is beyond its travel."""
+    n_act = desc.n_arms if desc.modes is None else int(desc.modes["n_active"][mode])
+    qa = np.asarray(q, dtype=np.float64)[:n_act * desc.dof]
+    if not np.all(np.isfinite(qa)):
+        return True
+    jt = getattr(desc, "joint_type", None)
+    if jt is None:
+        return False
+    tr = desc.joint_travel()[:n_act * desc.dof]
+    return bool(np.any((np.asarray(jt)[:n_act * desc.dof] == 1) & (np.abs(qa) > tr)))
+
+
 def collides(desc, q, mode=0, mask=0xFFFFFFFF, eps=1e-9):
-    if not np.all(np.isfinite(q)):
+    if out_of_range(desc, q, mode):
         return True
     return decide(pair_slacks(desc, q, mode, eps), mask, eps)
 
@@ -245,10 +262,11 @@ def _rand_rot(rng):
 
 
 def tiny_arm_scene(rng, scenes, n_arms=3, dof=4, n_boxes=3, n_modes=6, chain=False, spread=0.35,
-                   held_half=(0.02, 0.09), rel_offset=0.15, sphere_r=(0.02, 0.07), n_rest=0):
+                   held_half=(0.02, 0.09), rel_offset=0.15, sphere_r=(0.02, 0.07), n_rest=0, prism=0.0):
     """A small arm scene with modes: arm bases on a circle of radius `spread` (close enough for
     inter-arm contacts), 1..3 spheres per link, a few boxes, and modes with fewer active arms and
-    held boxes of various offsets."""
+    held boxes of various offsets.  Chain scenes: each joint is prismatic with probability `prism`
+    (limits +-0.3)."""
     base = np.zeros((n_arms, 12))
     for a in range(n_arms):
         ang = 2 * math.pi * a / n_arms
@@ -285,6 +303,9 @@ def tiny_arm_scene(rng, scenes, n_arms=3, dof=4, n_boxes=3, n_modes=6, chain=Fal
                             link_sphere_count=np.array(counts, dtype=np.int32),
                             sphere_local=np.array(sph, dtype=np.float64), boxes=boxes,
                             joint_fixed=jf if chain else None)
+    if chain and prism > 0:
+        desc.joint_type = (rng.uniform(size=D) < prism).astype(np.uint8)
+        desc.limits[:, desc.joint_type == 1] = [[-0.3], [0.3]]
     n_active = rng.integers(1, n_arms + 1, size=n_modes).astype(np.int32)
     n_active[0] = n_arms
     ns = n_arms + n_rest

@@ -21,7 +21,9 @@ def _cases(seed, kind, n_cfg=260):
     "chain" (3 arms x 4 links, 3 boxes), "dh-close" / "chain-close" (3 arms x 3 links close
     together, no boxes, larger held boxes: inter-arm and held-box contacts decide more states),
     "dh-reach" (2 arms, held boxes far out on their end-effectors: held boxes meet each other),
-    "dh-rest" / "chain-rest" (3 arms x 3 links and 3 resting boxes among them)."""
+    "dh-rest" / "chain-rest" (3 arms x 3 links and 3 resting boxes among them), "chain-prism" (3 arms
+    x 4 links, 40% of the joints prismatic; one state in ten has a prismatic coordinate beyond its
+    travel, which counts as a collision)."""
     rng = np.random.default_rng(seed)
     chain = kind.startswith("chain")
     if kind.endswith("close"):
@@ -31,21 +33,30 @@ def _cases(seed, kind, n_cfg=260):
                   sphere_r=(0.01, 0.03))
     elif kind.endswith("rest"):
         kw = dict(n_arms=3, dof=3, n_boxes=0, spread=0.25, held_half=(0.04, 0.12), n_rest=3)
+    elif kind.endswith("prism"):
+        kw = dict(n_arms=3, dof=4, n_boxes=3, prism=0.4)
     else:
         kw = dict(n_arms=3, dof=4, n_boxes=3)
     desc = BF.tiny_arm_scene(rng, scenes, chain=chain, **kw)
+    prism = kind.endswith("prism")
     nm = len(desc.modes["n_active"])
     masks = [0xFFFFFFFF, 0b1, 0b10, 0b101, 0b110]
     out = []
     reach = kind.endswith("reach")
     for t in range(n_cfg):
-        q = rng.uniform(-math.pi, math.pi, desc.dim) * rng.choice([0.3, 0.6, 1.0])
+        q = rng.uniform(desc.limits[0], desc.limits[1]) * rng.choice([0.3, 0.6, 1.0])
+        if prism and t % 10 == 9:  # a prismatic coordinate beyond its travel
+            k = int(rng.choice(np.nonzero(desc.joint_type == 1)[0]))
+            q[k] = rng.choice([-1, 1]) * rng.uniform(0.3001, 0.5)
         for _ in range(200 if reach else 0):  # end-effectors close to each other (held boxes meet)
             F = BF.link_frames(desc, q)
             if np.linalg.norm(F[desc.dof - 1][:3, 3] - F[2 * desc.dof - 1][:3, 3]) < 0.3:
                 break
             q = rng.uniform(-math.pi, math.pi, desc.dim)
         mode = int(rng.integers(0, nm))
+        if BF.out_of_range(desc, q, mode):
+            out.append((q, mode, masks[0], True, "travel"))
+            continue
         pairs = BF.pair_slacks(desc, q, mode)
         for mask in masks[:1 + (t % len(masks))]:
             exp = BF.decide(pairs, mask)
@@ -59,7 +70,8 @@ def _cases(seed, kind, n_cfg=260):
 
 
 SEEDS = [(11, "dh"), (12, "dh"), (13, "chain"), (14, "chain"), (15, "dh-close"), (16, "chain-close"),
-         (17, "dh-reach"), (18, "dh-reach"), (19, "dh-rest"), (20, "chain-rest")]
+         (17, "dh-reach"), (18, "dh-reach"), (19, "dh-rest"), (20, "chain-rest"), (21, "chain-prism"),
+         (22, "chain-prism")]
 
 
 @pytest.fixture(scope="module", params=SEEDS, ids=[f"seed{s}-{c}" for s, c in SEEDS])
@@ -117,7 +129,7 @@ def test_every_rule_decides_states():
         for *_, exp, cat in cases:
             if cat is not None:
                 counts[cat] = counts.get(cat, 0) + 1
-    for cat in BF.CATS + ("holder_only",):
+    for cat in BF.CATS + ("holder_only", "travel"):
         assert counts.get(cat, 0) >= MIN_DECISIVE, (cat, counts)
     # free states exist (a checker answering "collides" everywhere fails)
     assert sum(1 for s, c in SEEDS for *_, exp, _ in _cases(s, c)[1] if not exp) > 100
@@ -140,6 +152,27 @@ def test_oracle_mutants_fail():
                 if any(s < -1e-9 for s in counted) != (not o.config_free(q, m, mk)):
                     diff += 1
         assert diff > 0, which
+
+
+def test_prismatic_joints_decide_states():
+    """Reading the prismatic joints of the chain-prism scenes as revolute (T <- T Rz(q) C) changes
+    the oracle-checked verdict of some states: the joint type is exercised."""
+    import copy
+    diff = 0
+    for seed, kind in SEEDS:
+        if not kind.endswith("prism"):
+            continue
+        desc, cases = _cases(seed, kind)
+        rev = copy.copy(desc)
+        rev.joint_type = None
+        o = O.Scene(desc)
+        for q, m, mk, exp, cat in cases:
+            if cat == "travel":
+                continue
+            r = BF.decide(BF.pair_slacks(rev, q, m), mk)
+            assert o.config_free(q, m, mk) == (not exp)
+            diff += r is not None and r != exp
+    assert diff >= 20, diff
 
 
 # ---------------------------------------------------------------------------------------------

@@ -66,7 +66,8 @@ def test_registry_keys():
     importlib.import_module("multirobot-pathplanning-benchmark_amd.envs")
     names = pkg.list_envs()
     for k in ("abstract.test", "abstract.center_rect_2d", "abstract.center_rect_10d", "mrcc.quad_arms",
-              "mrcc.planar_disks", "mrcc.planar_boxes", "mrcc.mixed_arms"):
+              "mrcc.planar_disks", "mrcc.planar_boxes", "mrcc.mixed_arms", "mujoco.four_ur10", "mujoco.swap",
+              "pinocchio.middle_obs", "pinocchio.random_ur5"):
         assert k in names
     with pytest.raises(ValueError):
         pkg.get_env_by_name("abstract_test")  # the stale key of tests/test.py:51 is not registered

@@ -29,7 +29,7 @@ def test_exports_every_header_symbol():
 
 
 def test_abi_version():
-    assert pkg.lib().mrcc_abi_version() == 2
+    assert pkg.lib().mrcc_abi_version() == 3
 
 
 def test_host_bfs_indices_match_reference_order():
