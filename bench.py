@@ -611,6 +611,19 @@ def secondary_workloads(args, backend, scenes, dev):
                                          "note": "numpy restatement of abstract_env.py, one call per edge"}
         out[name] = ent
         log(f"{name}: {ent}")
+    # the reference's C++ per-call pattern (cpp/main.cpp:83-104) through the C ABI without Python:
+    # abstract.test, 10,000 configuration calls and 10,000 edge calls (a child process)
+    exe = os.path.join(ROOT, PKG, "cpp", "percall_bench")
+    if os.path.exists(exe):
+        import subprocess
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+        if r.returncode == 0:
+            out["cpp_per_call_abstract_test"] = {**json.loads(r.stdout.strip().splitlines()[-1]),
+                                                 "note": "cpp/percall_bench: mrcc_check_config_host / "
+                                                         "mrcc_check_edge_host per call, host buffers"}
+        else:
+            out["cpp_per_call_abstract_test"] = {"error": r.stderr[-400:]}
+        log(f"cpp per-call: {out['cpp_per_call_abstract_test']}")
     # C5: mixed arm counts and held boxes, one launch over all instances (paths of 1..64 edges)
     n_inst = args.mixed_instances
     rng = np.random.default_rng(5)

@@ -640,7 +640,11 @@ int mrcc_scene_create_disks(int n_robots, int robot_dim, const double* robot_rad
         return fail(MRCC_ERR_INVALID, "disks: need 1..%d robots of dimension 1..%d", MRCC_DISK_MAX_ROBOTS,
                     MRCC_DISK_MAX_RD);
     if (n_robots * robot_dim > MRCC_MAX_D) return fail(MRCC_ERR_INVALID, "D > %d", MRCC_MAX_D);
-    if (n_spheres < 0 || n_rects < 0) return fail(MRCC_ERR_INVALID, "negative obstacle count");
+    if (n_spheres < 0 || n_rects < 0 || n_spheres > (1 << 20) || n_rects > (1 << 20))
+        return fail(MRCC_ERR_INVALID, "obstacle counts must be in 0..%d", 1 << 20);
+    if (!robot_radius || (n_spheres > 0 && (!sphere_center || !sphere_radius)) ||
+        (n_rects > 0 && (!rect_center || !rect_size)))
+        return fail(MRCC_ERR_INVALID, "null argument");
     const int nr = n_robots, rd = robot_dim, ns = n_spheres, nq = n_rects;
     std::vector<float> r32(nr), rrT32(nr * nr), sc32(ns * rd + 1), srT32(nr * ns + 1),
         lo32(nq * rd + 1), hi32(nq * rd + 1), rp2_32(nr);
@@ -722,7 +726,9 @@ int mrcc_scene_create_planar_boxes(int n_robots, const double* robot_half_extent
     if (!out) return fail(MRCC_ERR_INVALID, "out is null");
     if (n_robots < 1 || n_robots > MRCC_PLANAR_MAX_ROBOTS)
         return fail(MRCC_ERR_INVALID, "planar: 1..%d robots", MRCC_PLANAR_MAX_ROBOTS);
-    if (n_obstacles < 0) return fail(MRCC_ERR_INVALID, "negative obstacle count");
+    if (n_obstacles < 0 || n_obstacles > (1 << 20)) return fail(MRCC_ERR_INVALID, "obstacle count must be in 0..%d", 1 << 20);
+    if (!robot_half_extents || (n_obstacles > 0 && (!obstacle_pose || !obstacle_half_extents)))
+        return fail(MRCC_ERR_INVALID, "null argument");
     const int nr = n_robots, no = n_obstacles;
     std::vector<double> rh64(2 * nr), ob64(8 * no + 1);
     std::vector<float> rh32(2 * nr), ob32(8 * no + 1);
