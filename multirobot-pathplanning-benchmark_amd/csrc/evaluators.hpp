@@ -531,6 +531,33 @@ __device__ inline float readlane_f(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
+__device__ inline double readlane_f64(double v, int l) {
+    const unsigned long long u = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | lo);
+}
+
+// v of another lane of the row of 16 by a DPP pattern (all lanes active)
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const unsigned long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// Largest of the 64 lanes' values, none of them NaN, in every lane: DPP steps within each row of
+// 16 lanes (quad swaps, half-row and row mirrors), then the four row maxima.  The order of a max
+// does not change its value, so any lane order gives the sequential loop's result.
+__device__ inline double wave_max_f64(double m) {
+    m = fmax(m, dpp_f64<0xb1>(m));   // quad_perm [1, 0, 3, 2]
+    m = fmax(m, dpp_f64<0x4e>(m));   // quad_perm [2, 3, 0, 1]
+    m = fmax(m, dpp_f64<0x141>(m));  // row_half_mirror
+    m = fmax(m, dpp_f64<0x140>(m));  // row_mirror
+    return fmax(fmax(readlane_f64(m, 0), readlane_f64(m, 16)), fmax(readlane_f64(m, 32), readlane_f64(m, 48)));
+}
+
 // Band pairs (binary32 test within the margin) are queued in LDS and rechecked in binary64 a
 // batch at a time, lane-parallel: the recheck code then runs where little else is live, instead
 // of a call inside the candidate loops (whose register cost would apply to the whole kernel).
@@ -643,7 +670,11 @@ __device__ inline float box_d2(float4 c, float4 lo, float4 hi) {
 // them (pays when most states collide; the caller decides from its recent states).
 // STATS = false compiles the test counters away.  A mode outside [0, nm) is reported as a
 // collision (callers reject such modes before they get here).
-template <bool GRASP, bool CHAIN, bool STATS>
+// LOWLAT = true selects forms that shorten one state's latency at the cost of registers (for a
+// kernel with one state per wave in flight: the per-call form): the kinematics load joint
+// coefficients three joints ahead, and the first round of sphere centres shares one block of
+// loads with the link spheres.
+template <bool GRASP, bool CHAIN, bool STATS, bool LOWLAT = false>
 __device__ inline bool arm_state_collides(const ArmK& V, const ArmLds& W, const ArmRegs& R, double qv,
                                           int mode, unsigned mask, RecheckT<STATS>& rechecks,
                                           ArmCountT<STATS>& cnt, bool lazy = false) {
@@ -702,19 +733,13 @@ __device__ inline bool arm_state_collides(const ArmK& V, const ArmLds& W, const 
                 double R0 = W.base[12 * a + 3 * rr], R1 = W.base[12 * a + 3 * rr + 1],
                        R2 = W.base[12 * a + 3 * rr + 2], t = W.base[12 * a + 9 + rr];
                 const int l0 = a * dof;
-                // joint k's coefficients are loaded one step ahead, off the chain's critical path
-                double2 kk = ((const double2*)(W.fkc + 8 * l0))[c];
-                // chain joints: A2c = C2c for c < 3, A23 from the kinematics coefficients
+                // chain joints: A2c = C2c for c < 3, A23 from the kinematics coefficients; the
+                // coefficients of later joints are loaded while a joint is stepped (an LDS round
+                // trip is longer than a step)
                 const double* a2 = c == 3 ? W.fkc + 8 * V.D : W.dh + 8 + c;
                 const int a2s = c == 3 ? 1 : 12;
-                double k2 = chain ? a2[a2s * l0] : (c == 0 ? 0.0 : W.dh[4 * l0 + k2i]);
-                for (int k = 0; k < dof; ++k) {
-                    const int l = l0 + k;
-                    const double k0 = kk.x, k1 = kk.y, kz = k2;
-                    if (k + 1 < dof) {
-                        kk = ((const double2*)(W.fkc + 8 * (l + 1)))[c];
-                        k2 = chain ? a2[a2s * (l + 1)] : (c == 0 ? 0.0 : W.dh[4 * (l + 1) + k2i]);
-                    }
+                // one joint step: entry c of row rr, then the row's four new entries from the quad
+                auto step = [&](int l, double k0, double k1, double kz) {
                     double y;
                     {
 #pragma clang fp contract(off)
@@ -727,6 +752,43 @@ __device__ inline bool arm_state_collides(const ArmK& V, const ArmLds& W, const 
                     R1 = quad_bcast_f64(y, 1);
                     R2 = quad_bcast_f64(y, 2);
                     t = quad_bcast_f64(y, 3);
+                };
+                auto coef = [&](int l) { return ((const double2*)(W.fkc + 8 * l))[c]; };
+                // loaded by every lane (no lane-masked load: k2i = 0 reads d) and replaced by 0
+                // for column 0 of a DH joint where it is used, so a load is waited for only there
+                auto coef2 = [&](int l) { return chain ? a2[a2s * l] : W.dh[4 * l + k2i]; };
+                const bool k2zero = !chain && c == 0;
+                if constexpr (LOWLAT) {
+                    constexpr int AH = 3;  // joints ahead
+                    double2 KK[MRCC_MAX_DOF];
+                    double K2[MRCC_MAX_DOF];
+#pragma unroll
+                    for (int k = 0; k < AH; ++k) {
+                        if (k >= dof) break;  // wave-uniform
+                        KK[k] = coef(l0 + k);
+                        K2[k] = coef2(l0 + k);
+                    }
+#pragma unroll
+                    for (int k = 0; k < MRCC_MAX_DOF; ++k) {
+                        if (k >= dof) break;  // wave-uniform
+                        if (k + AH < MRCC_MAX_DOF && k + AH < dof) {
+                            KK[k + AH] = coef(l0 + k + AH);
+                            K2[k + AH] = coef2(l0 + k + AH);
+                        }
+                        step(l0 + k, KK[k].x, KK[k].y, k2zero ? 0.0 : K2[k]);
+                    }
+                } else {
+                    // joint k + 1's coefficients are loaded while joint k is stepped
+                    double2 kk = coef(l0);
+                    double k2 = coef2(l0);
+                    for (int k = 0; k < dof; ++k) {
+                        const double k0 = kk.x, k1 = kk.y, kz = k2zero ? 0.0 : k2;
+                        if (k + 1 < dof) {
+                            kk = coef(l0 + k + 1);
+                            k2 = coef2(l0 + k + 1);
+                        }
+                        step(l0 + k, k0, k1, kz);
+                    }
                 }
                 if (GRASP && c == 0 && V.m_grasp()[mode * V.nslot + a]) {
                     const double* Rel = V.m_rel() + 12 * (mode * V.nslot + a);
@@ -763,14 +825,32 @@ __device__ inline bool arm_state_collides(const ArmK& V, const ArmLds& W, const 
     const int lastFC = __builtin_amdgcn_readlane(R.linkFC, Dact - 1);
     const int S_act = (lastFC & 0xffff) + (lastFC >> 16);
     const int S_first = lazy && V.broadphase && S_act > 64 ? 64 : S_act;
-    for (int s = lane; s < S_first; s += 64) {
+    float4 myLink = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (LOWLAT) {
+        // the first round of spheres and the link spheres in one block: all their loads are
+        // issued before any store
+        const bool hs = lane < S_first, hl = V.broadphase && lane < Dact;
+        const int s = hs ? lane : 0, ll = hl ? lane : 0;
+        const double* l = W.sphL + 3 * s;
+        const double* b = W.linkBC + 3 * ll;
+        const double l0 = l[0], l1 = l[1], l2 = l[2], b0 = b[0], b1 = b[1], b2 = b[2];
+        const float rs = W.sphR[s], rl = W.linkR[ll];
+        double c[3], e[3];
+        xform_point_f64(W.X + 12 * (W.sphLink[s] & 255), l0, l1, l2, c);
+        xform_point_f64(W.X + 12 * ll, b0, b1, b2, e);
+        if (hs) W.sph[s] = make_float4((float)c[0], (float)c[1], (float)c[2], rs);
+        if (hl) {
+            myLink = make_float4((float)e[0], (float)e[1], (float)e[2], rl);
+            W.linkC[ll] = myLink;
+        }
+    }
+    for (int s = (LOWLAT ? 64 : 0) + lane; s < S_first; s += 64) {
         double c[3];
         const double* l = W.sphL + 3 * s;
         xform_point_f64(W.X + 12 * (W.sphLink[s] & 255), l[0], l[1], l[2], c);
         W.sph[s] = make_float4((float)c[0], (float)c[1], (float)c[2], W.sphR[s]);
     }
-    float4 myLink = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (V.broadphase && lane < Dact) {
+    if (!LOWLAT && V.broadphase && lane < Dact) {
         const double* b = W.linkBC + 3 * lane;
         double c[3];
         xform_point_f64(W.X + 12 * lane, b[0], b[1], b[2], c);

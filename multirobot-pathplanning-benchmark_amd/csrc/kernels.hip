@@ -813,8 +813,10 @@ __global__ void __launch_bounds__(256, 3) k_arm_configs(ArmK V, const double* qv
 // Plain per-call form: one launch checks edge 0 of B with one workgroup of nw waves; wave w takes
 // the edge's slots w, w + nw, ...  The waves share the verdict through LDS: a wave that finds a
 // collision raises the flag, the others stop before their next state, and thread 0 writes the
-// verdict once every wave is done, so the output needs no initialisation.  The rows are loaded
-// before the scene is staged, so the two round trips overlap.
+// verdict once every wave is done, so the output needs no initialisation.  Spheres beyond the
+// first 64 are placed when a candidate pair needs them (most states of a per-call stream collide
+// within the first arm).  The rows are loaded before the scene is staged, so the two round trips
+// overlap.
 template <bool GRASP, bool CHAIN>
 __global__ void __launch_bounds__(512) k_arm_edge(ArmK V, EdgeBatchView B) {
     extern __shared__ double lds_arm[];
@@ -832,16 +834,13 @@ __global__ void __launch_bounds__(512) k_arm_edge(ArmK V, EdgeBatchView B) {
     if (B.N || B.N0 > 0) {
         N = B.N ? B.N[0] : B.N0;
     } else {
-        // edge_N_dev's rule (m = |a0 - b0|, then m = v > m ? v : m over the coordinates): lane 0
-        // starts from coordinate 0 and the reduction only replaces m by a larger number, so a NaN
-        // in coordinate 0 sticks and a NaN elsewhere is skipped, as in the sequential loop
-        double m = lane < D ? __builtin_fabs(av - bv) : 0.0;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const double v = __shfl_xor(m, o, 64);
-            m = v > m ? v : m;
-        }
-        N = edge_N_from_max(__shfl(m, 0), B.res);
+        // edge_N_dev's rule (m = |a0 - b0|, then m = v > m ? v : m over the coordinates): a NaN in
+        // coordinate 0 sticks and a NaN elsewhere is skipped, as in the sequential loop; the other
+        // values are >= 0, so the maximum of the non-NaN values (0 for lanes >= D) is the loop's
+        const double m = lane < D ? __builtin_fabs(av - bv) : 0.0;
+        const double m0 = readlane_f64(m, 0);
+        const double mx = wave_max_f64(__builtin_isnan(m) ? 0.0 : m);
+        N = edge_N_from_max(__builtin_isnan(m0) ? m0 : mx, B.res);
     }
     const ArmLds Wl = arm_lds(lds_arm, V);
     const ArmRegs R = arm_regs(V, Wl);
@@ -862,7 +861,7 @@ __global__ void __launch_bounds__(512) k_arm_edge(ArmK V, EdgeBatchView B) {
 #pragma clang fp contract(off)
                 qv = av + ((bv - av) / (double)(N - 1)) * (double)i;  // abstract_env.py:341, :348
             }
-            if (arm_state_collides<GRASP, CHAIN, false>(V, Wl, R, qv, mode, B.mask, rechecks, cnt)) {
+            if (arm_state_collides<GRASP, CHAIN, false, true>(V, Wl, R, qv, mode, B.mask, rechecks, cnt, true)) {
                 if (lane == 0) *hit = 1;
                 break;
             }

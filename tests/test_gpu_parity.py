@@ -312,6 +312,41 @@ def test_c3_edges(c3, B, bp):
     sc.set_broadphase(True)
 
 
+def test_c3_per_call_state_split(c3, B):
+    """The per-call kernel splits a state's narrow phase over up to 4 waves when the edge has fewer
+    states than waves, and waves stop once one finds a collision: short edges (1..10 states), free
+    and colliding, against the oracle; with held boxes (modes) and on chain joints."""
+    _say("C3 per call: state split over waves (1..10 states per edge), modes, chain joints vs oracle")
+    desc, sc, o, rng = c3
+    a = near_home(rng, desc, 2400, sigma=0.5)
+    d = rng.normal(0, 1, a.shape)
+    d /= np.max(np.abs(d), axis=1, keepdims=True)
+    L = 0.05 * rng.integers(1, 12, size=(len(a), 1)) + rng.uniform(0, 0.049, (len(a), 1))  # N - 2 = 0..10
+    b = np.clip(a + L * d, desc.limits[0], desc.limits[1])
+    ref = o.check_edges(a, b, 0.05)
+    assert 0.1 < ref.mean() < 0.9
+    assert np.array_equal(gpu_edges(sc, a, b, 0.05, per_call=True), ref)
+    # held boxes and fewer arms (the GRASP instantiation)
+    md = scenes.mixed_arms(rng=np.random.default_rng(23), n_arms=4, n_modes=16)
+    sm, om = B.Scene(md), O.Scene(md)
+    a2 = near_home(rng, md, 1200, sigma=0.5)
+    b2 = np.clip(a2 + rng.normal(0, 0.08, a2.shape), md.limits[0], md.limits[1])
+    modes = rng.integers(0, 16, len(a2)).astype(np.int32)
+    ref2 = om.check_edges(a2, b2, 0.05, modes=modes)
+    assert 0.05 < ref2.mean() < 0.95
+    assert np.array_equal(gpu_edges(sm, a2, b2, 0.05, modes=modes, per_call=True), ref2)
+    # chain joints (prismatic among them): the brute-force scene family
+    import arm_bruteforce as BF
+    cd = BF.tiny_arm_scene(np.random.default_rng(33), scenes, chain=True, prism=0.4, n_boxes=2, spread=0.5)
+    sc3, o3 = B.Scene(cd), O.Scene(cd)
+    a3 = rng.uniform(cd.limits[0], cd.limits[1], size=(1200, cd.dim)) * 0.5
+    b3 = np.clip(a3 + rng.normal(0, 0.1, a3.shape), cd.limits[0], cd.limits[1])
+    m3 = rng.integers(0, len(cd.modes["n_active"]), len(a3)).astype(np.int32)
+    ref3 = o3.check_edges(a3, b3, 0.05, modes=m3)
+    assert 0.05 < ref3.mean() < 0.95
+    assert np.array_equal(gpu_edges(sc3, a3, b3, 0.05, modes=m3, per_call=True), ref3)
+
+
 @pytest.mark.parametrize("spt,g,w", [(1, 1, 1), (16, 1, 12), (64, 4, 16), (7, 2, 3)])
 def test_c3_queue_shapes(c3, B, spt, g, w):
     _say(f"C3 persistent queue shapes: slots/ticket {spt}, slots/pass {g}, waves/CU {w}; stats invariants")
