@@ -73,6 +73,50 @@ __device__ __attribute__((noinline)) bool disk_rect_f64(const DiskView& S, const
     return np_sum_f64(sq, S.rd) <= S.rp2_64[i];
 }
 
+// The same three predicates for 2-D disks with the coordinates by value, inlined: a pointer to the
+// caller's state vector would keep that vector in scratch memory for the whole kernel, and a call
+// needs a stack frame; the 2-D disk kernels have registers to spare.  The operations are those
+// above for rd = 2: np.dot's fused chain from 0, and np.sum of two squares (-0.0 + a + b, then
+// 0.0 + that).
+__device__ __attribute__((always_inline)) inline bool disk_rr2_f64(const DiskView& S, double xi, double yi, double xj,
+                                                       double yj, int i, int j) {
+#pragma clang fp contract(off)
+    const double d0 = xi - xj, d1 = yi - yj;
+    return __builtin_fma(d1, d1, __builtin_fma(d0, d0, 0.0)) < S.rrT64[i * S.nr + j];
+}
+
+__device__ __attribute__((always_inline)) inline bool disk_rs2_f64(const DiskView& S, double x, double y, int i, int o) {
+#pragma clang fp contract(off)
+    const double d0 = S.sc64[2 * o] - x, d1 = S.sc64[2 * o + 1] - y;
+    return __builtin_fma(d1, d1, __builtin_fma(d0, d0, 0.0)) < S.srT64[i * S.ns + o];
+}
+
+__device__ __attribute__((always_inline)) inline bool disk_rect2_f64(const DiskView& S, double x, double y, int i, int o) {
+#pragma clang fp contract(off)
+    const double e0 = np_clip_f64(x, S.lo64[2 * o], S.hi64[2 * o]) - x;
+    const double e1 = np_clip_f64(y, S.lo64[2 * o + 1], S.hi64[2 * o + 1]) - y;
+    const double s0 = e0 * e0, s1 = e1 * e1;
+    double res = -0.0;
+    res += s0;
+    res += s1;
+    return 0.0 + res <= S.rp2_64[i];
+}
+
+// literal binary64 evaluation of two 2-D disks (used for non-finite coordinates)
+__device__ __attribute__((always_inline)) inline bool disk_collides22_f64(const DiskView& S, double q0, double q1,
+                                                              double q2, double q3, unsigned mask) {
+    if ((mask & 3u) == 3u && disk_rr2_f64(S, q0, q1, q2, q3, 0, 1)) return true;
+    for (int i = 0; i < 2; ++i) {
+        if (!((mask >> i) & 1u)) continue;
+        const double x = i ? q2 : q0, y = i ? q3 : q1;
+        for (int o = 0; o < S.ns; ++o)
+            if (disk_rs2_f64(S, x, y, i, o)) return true;
+        for (int o = 0; o < S.nrect; ++o)
+            if (disk_rect2_f64(S, x, y, i, o)) return true;
+    }
+    return false;
+}
+
 // literal binary64 evaluation (used for non-finite coordinates)
 __device__ __attribute__((noinline)) bool disk_collides_f64(const DiskView& S, const double* q, unsigned mask) {
     for (int i = 0; i < S.nr; ++i)
@@ -114,7 +158,11 @@ __device__ inline bool disk_collides(const DiskView& S, const double* q, unsigne
             finite = finite && __builtin_isfinite(q[k]);
         }
     }
-    if (!finite) return disk_collides_f64(S, q, mask);
+    constexpr bool by_value = RDT == 2 && NRT == 2;  // rechecks without a pointer to q
+    if (!finite) {
+        if constexpr (by_value) return disk_collides22_f64(S, q[0], q[1], q[2], q[3], mask);
+        else return disk_collides_f64(S, q, mask);
+    }
     const float M = disk_margin(B, rd);
 #pragma unroll
     for (int i = 0; i < (NRT ? NRT : MRCC_DISK_MAX_ROBOTS); ++i) {
@@ -134,7 +182,11 @@ __device__ inline bool disk_collides(const DiskView& S, const double* q, unsigne
             if (d2 < T + M) {
                 if (d2 < T - M) return true;
                 ++rechecks;
-                if (disk_rr_f64(S, q, i, j)) return true;
+                if constexpr (by_value) {
+                    if (disk_rr2_f64(S, q[0], q[1], q[2], q[3], 0, 1)) return true;
+                } else if (disk_rr_f64(S, q, i, j)) {
+                    return true;
+                }
             }
         }
     }
@@ -154,7 +206,11 @@ __device__ inline bool disk_collides(const DiskView& S, const double* q, unsigne
             if (d2 < T + M) {
                 if (d2 < T - M) return true;
                 ++rechecks;
-                if (disk_rs_f64(S, q, i, o)) return true;
+                if constexpr (by_value) {
+                    if (disk_rs2_f64(S, q[2 * i], q[2 * i + 1], i, o)) return true;
+                } else if (disk_rs_f64(S, q, i, o)) {
+                    return true;
+                }
             }
         }
         const float T = S.rp2_32[i];
@@ -170,7 +226,11 @@ __device__ inline bool disk_collides(const DiskView& S, const double* q, unsigne
             if (d2 <= T + M) {
                 if (d2 < T - M) return true;
                 ++rechecks;
-                if (disk_rect_f64(S, q, i, o)) return true;
+                if constexpr (by_value) {
+                    if (disk_rect2_f64(S, q[2 * i], q[2 * i + 1], i, o)) return true;
+                } else if (disk_rect_f64(S, q, i, o)) {
+                    return true;
+                }
             }
         }
     }
@@ -179,7 +239,9 @@ __device__ inline bool disk_collides(const DiskView& S, const double* q, unsigne
 
 // --------------------------------------------------------------------------------------------
 // planar oriented boxes: closed 2D separating-axis test.  Box = [x y ux uy vx vy h0 h1].
-__device__ __attribute__((noinline)) bool obb2_overlap_f64(const double* A, const double* B) {
+// inlined into its callers, the rechecks below (noinline themselves), so that their box arrays
+// stay in registers
+__device__ __attribute__((always_inline)) inline bool obb2_overlap_f64(const double* A, const double* B) {
 #pragma clang fp contract(off)
     double tx = B[0] - A[0], ty = B[1] - A[1];
     double C = A[2] * B[2] + A[3] * B[3];

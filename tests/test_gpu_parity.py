@@ -163,6 +163,38 @@ def test_c1_near_contact(c1):
     assert np.array_equal(gpu_configs(sc, Q), o.check_configs(Q))
 
 
+def test_abstract_sphere_near_contact(B):
+    """abstract.test's sphere obstacle (r 0.2 at the origin, abstract_env.py:363-378) and two disks
+    of r 0.1: disk centres within 1e-15 .. 1e-7 of 0.3 from the origin (the binary64 recheck of
+    the disk-sphere pair), disk pairs near 0.2 apart, masks and non-finite values."""
+    _say("abstract.test disk-sphere and disk-disk near contact (binary64 rechecks), masks, NaN")
+    desc = scenes.abstract_test_scene()
+    sc, o = B.Scene(desc), O.Scene(desc)
+    rng = np.random.default_rng(19)
+    n = 20000
+    Q = rng.uniform(-2, 2, (n, 4))
+    eps = rng.choice([-1e-7, -1e-12, -1e-15, 0.0, 1e-15, 1e-12, 1e-7], n)
+    phi = rng.uniform(-math.pi, math.pi, n)
+    k = n // 2
+    Q[:k, 0] = (0.3 + eps[:k]) * np.cos(phi[:k])  # robot 0 touching the sphere
+    Q[:k, 1] = (0.3 + eps[:k]) * np.sin(phi[:k])
+    Q[k:, 2] = Q[k:, 0] + (0.2 + eps[k:]) * np.cos(phi[k:])  # robot 1 touching robot 0
+    Q[k:, 3] = Q[k:, 1] + (0.2 + eps[k:]) * np.sin(phi[k:])
+    ref = o.check_configs(Q)
+    assert 0.2 < ref.mean() < 0.8
+    assert np.array_equal(gpu_configs(sc, Q), ref)
+    for mask in (1, 2):
+        assert np.array_equal(gpu_configs(sc, Q, mask=mask), o.check_configs(Q, mask=mask))
+    Qn = Q[:8].copy()
+    Qn[0, 0] = np.nan
+    Qn[1, 3] = np.inf
+    Qn[2, 1] = -np.inf
+    assert np.array_equal(gpu_configs(sc, Qn), o.check_configs(Qn))
+    q2 = np.clip(Q[:3000] + rng.normal(0, 0.02, (3000, 4)), -2, 2)
+    assert np.array_equal(gpu_edges(sc, Q[:3000], q2, 0.01), o.check_edges(Q[:3000], q2, 0.01))
+    assert np.array_equal(gpu_edges(sc, Q[:300], q2[:300], 0.01, per_call=True), o.check_edges(Q[:300], q2[:300], 0.01))
+
+
 def test_c1_non_finite(c1):
     _say("C1 NaN / inf coordinates follow numpy semantics")
     desc, sc, o, _ = c1
