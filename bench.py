@@ -22,6 +22,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The CPU baselines' OpenMP workers sleep between parallel regions instead of spinning: spinning
+# workers hold the host cores that launch the next measurement's GPU work (the clearance autograd
+# pair measured 18.0 ms after the CPU baselines, 9.7 ms without them, same kernel times).
+os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
 PKG = "multirobot-pathplanning-benchmark_amd"
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X vector FP32 (AMD published)
@@ -137,12 +141,14 @@ def main():
                     help="only the clearance cost + gradient workload (for profiling runs)")
     ap.add_argument("--ur10-only", action="store_true",
                     help="only the four-UR10e mujoco-world workload (for profiling runs)")
+    ap.add_argument("--per-call-only", action="store_true",
+                    help="only the plain per-call form on the headline edges (for kernel traces)")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the headline batch: warmup + timed graph replays (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
-                                   "described in DESIGN.md, not built)"}))
+        print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 (sharding edges over "
+                                   "several GPUs is not built)"}))
         sys.exit(2)
 
     import torch
@@ -194,6 +200,19 @@ def main():
         return
     a = torch.from_numpy(q1).to(dev)
     b = torch.from_numpy(q2).to(dev)
+    if args.per_call_only:  # one launch per edge over the first per_call_edges edges, twice
+        m = min(args.per_call_edges, args.edges)
+        o1 = torch.empty(m, dtype=torch.uint8, device=dev)
+        b1 = sc.edge_batch(a[:m], b[:m], o1, res)
+        sc.check_edges_per_call_device(b1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sc.check_edges_per_call_device(b1)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        print(json.dumps({"per_call_only": True, "edges": m, "us_per_edge": dt / m * 1e6,
+                          "edge_states_per_s": int(np.sum(N[:m] - 2)) / dt}), flush=True)
+        return
     out = torch.empty(args.edges, dtype=torch.uint8, device=dev)
     batch = sc.edge_batch(a, b, out, res)
     plan = backend.EdgePlan(sc, batch, backend.make_tune(use_graph=1))
