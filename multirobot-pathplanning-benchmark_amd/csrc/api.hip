@@ -432,9 +432,11 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
         }
     }
     if (sc->arm_nw == 0) return fail(MRCC_ERR_INVALID, "arm scene needs more than 64 KiB of LDS per wave");
-    // the per-call kernel: one workgroup of as many waves (up to 8) as 64 KiB of LDS holds
+    // the per-call kernel: one workgroup of as many waves as 64 KiB of LDS holds, at most one per
+    // SIMD of the CU (4): a launch is latency-bound, and a second wave on a SIMD slows the first
+    // (the states of one edge usually all collide, so the first round decides most edges)
     sc->arm_nw_edge = 1;
-    for (int nw = 8; nw >= 1; --nw)
+    for (int nw = 4; nw >= 1; --nw)
         if (arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, nw, H.nslot) <= 65536) {
             sc->arm_nw_edge = nw;
             break;

@@ -548,6 +548,44 @@ __device__ inline void arm_stage_scene(void* base, const ArmK& V) {
     __syncthreads();
 }
 
+// The same copy in two halves, for a kernel that computes while the image is in flight: the first
+// K blocks of the image (K 16-byte loads per thread, indices clamped into the image, so the loads
+// are issued unconditionally) into registers, then the stores, the rest of the image and the
+// barrier.
+template <int K>
+struct ArmStage {
+    float4 v[K];
+};
+
+template <int K>
+__device__ inline ArmStage<K> arm_stage_load(const ArmK& V) {
+    ArmStage<K> r;
+    const float4* src = V.img();
+    const int last = (V.img_bytes >> 4) - 1;  // >= 0: the joint table is never empty
+    const int nt = blockDim.x;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = threadIdx.x + k * nt;
+        r.v[k] = src[i < last ? i : last];
+    }
+    return r;
+}
+
+template <int K>
+__device__ inline void arm_stage_store(void* base, const ArmK& V, const ArmStage<K>& r) {
+    float4* dst = (float4*)base;
+    const float4* src = V.img();
+    const int n16 = V.img_bytes >> 4;
+    const int nt = blockDim.x;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = threadIdx.x + k * nt;
+        if (i < n16) dst[i] = r.v[k];
+    }
+    for (int i = threadIdx.x + K * nt; i < n16; i += nt) dst[i] = src[i];
+    __syncthreads();
+}
+
 // LDS written by some lanes of a wave is read by other lanes of the same wave after this point
 // (LDS-only fences: outstanding global loads, such as the next state's prefetched inputs, are
 // not waited for)
@@ -736,10 +774,13 @@ __device__ inline float box_d2(float4 c, float4 lo, float4 hi) {
 // kernel with one state per wave in flight: the per-call form): the kinematics load joint
 // coefficients three joints ahead, and the first round of sphere centres shares one block of
 // loads with the link spheres.
+// `have_tq` (wave-uniform): `tq` already holds sincos_f64(qv) of the lane's coordinate (computed by
+// the caller, for instance while the scene was being staged); otherwise it is computed here.
 template <bool GRASP, bool CHAIN, bool STATS, bool LOWLAT = false>
 __device__ inline bool arm_state_collides(const ArmK& V, const ArmLds& W, const ArmRegs& R, double qv,
                                           int mode, unsigned mask, RecheckT<STATS>& rechecks,
-                                          ArmCountT<STATS>& cnt, bool lazy = false) {
+                                          ArmCountT<STATS>& cnt, bool lazy = false, bool have_tq = false,
+                                          SinCos tq = SinCos{0.0, 1.0}) {
     const int lane = wave_lane();
     if (mode < 0 || mode >= V.nm) return true;
     const int na = arm_mact(V, W, mode);
@@ -755,7 +796,7 @@ __device__ inline bool arm_state_collides(const ArmK& V, const ArmLds& W, const 
     if (lane < Dact) {
         bad = !__builtin_isfinite(qv);
         const bool pr = CHAIN && ((V.prism >> lane) & 1ull);
-        const SinCos t = sincos_f64(qv);
+        const SinCos t = have_tq ? tq : sincos_f64(qv);
         const double cs = pr ? 1.0 : t.c, sn = pr ? 0.0 : t.s;
         double2* f = (double2*)(W.fkc + 8 * lane);
         if (CHAIN) {

@@ -815,8 +815,9 @@ __global__ void __launch_bounds__(256, 3) k_arm_configs(ArmK V, const double* qv
 // collision raises the flag, the others stop before their next state, and thread 0 writes the
 // verdict once every wave is done, so the output needs no initialisation.  Spheres beyond the
 // first 64 are placed when a candidate pair needs them (most states of a per-call stream collide
-// within the first arm).  The rows are loaded before the scene is staged, so the two round trips
-// overlap.
+// within the first arm).  The rows are loaded first, then the scene image; while the image is in
+// flight each wave computes the edge's state count and the joint sines and cosines of its first
+// state, so the staging round trip overlaps the rows' round trip and that arithmetic.
 template <bool GRASP, bool CHAIN>
 __global__ void __launch_bounds__(512) k_arm_edge(ArmK V, EdgeBatchView B) {
     extern __shared__ double lds_arm[];
@@ -828,8 +829,8 @@ __global__ void __launch_bounds__(512) k_arm_edge(ArmK V, EdgeBatchView B) {
     const double* b = row2(B, 0);
     const double av = lane < D ? a[lane * B.ds] : 0.0;
     const double bv = lane < D ? b[lane * B.ds] : 0.0;
+    const auto stage = arm_stage_load<4>(V);  // 16 KiB of the image at 256 threads
     if (threadIdx.x == 0) s_hit = 0;
-    arm_stage_scene(lds_arm, V);  // ends with a barrier
     int N;
     if (B.N || B.N0 > 0) {
         N = B.N ? B.N[0] : B.N0;
@@ -842,26 +843,39 @@ __global__ void __launch_bounds__(512) k_arm_edge(ArmK V, EdgeBatchView B) {
         const double mx = wave_max_f64(__builtin_isnan(m) ? 0.0 : m);
         N = edge_N_from_max(__builtin_isnan(m0) ? m0 : mx, B.res);
     }
-    const ArmLds Wl = arm_lds(lds_arm, V);
-    const ArmRegs R = arm_regs(V, Wl);
     const int nw = blockDim.x >> 6, wv = threadIdx.x >> 6;
     const int mode = batch_mode(B, 0);
     const bool bad_mode = mode < 0 || mode >= V.nm;
     const EdgeWin w = edge_window(N, batch_N_start(B, 0), batch_N_max(B, 0), B.incl);
+    const bool run = !w.err && !bad_mode;
+    // the state of slot `local` (abstract_env.py:341, :348); i < 0: the slot has no state
+    auto state_q = [&](int i) {
+#pragma clang fp contract(off)
+        return av + ((bv - av) / (double)(N - 1)) * (double)i;
+    };
+    // the wave's first state and its joint sines / cosines, before the image is stored
+    const int i0 = run && wv < w.M ? edge_state_index(w, wv, B.incl) : -1;
+    double q0 = 0.0;
+    SinCos t0 = SinCos{0.0, 1.0};
+    if (i0 >= 0) {  // wave-uniform
+        q0 = state_q(i0);
+        t0 = sincos_f64(q0);
+    }
+    arm_stage_store(lds_arm, V, stage);  // ends with a barrier
+    const ArmLds Wl = arm_lds(lds_arm, V);
+    const ArmRegs R = arm_regs(V, Wl);
     NoCount rechecks;
     ArmCountT<false> cnt;
-    if (!w.err && !bad_mode) {
+    if (run) {
         volatile int* hit = &s_hit;
         for (int local = wv; local < w.M; local += nw) {
             if (*hit) break;  // wave-uniform: one LDS word
-            const int i = edge_state_index(w, local, B.incl);
+            const bool first = local == wv;
+            const int i = first ? i0 : edge_state_index(w, local, B.incl);
             if (i < 0) continue;
-            double qv;
-            {
-#pragma clang fp contract(off)
-                qv = av + ((bv - av) / (double)(N - 1)) * (double)i;  // abstract_env.py:341, :348
-            }
-            if (arm_state_collides<GRASP, CHAIN, false, true>(V, Wl, R, qv, mode, B.mask, rechecks, cnt, true)) {
+            const double qv = first ? q0 : state_q(i);
+            if (arm_state_collides<GRASP, CHAIN, false, true>(V, Wl, R, qv, mode, B.mask, rechecks, cnt, true,
+                                                              first, t0)) {
                 if (lane == 0) *hit = 1;
                 break;
             }
