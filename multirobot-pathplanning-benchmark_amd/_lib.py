@@ -78,9 +78,11 @@ SIGNATURES = {
     "mrcc_edge_plan_destroy": (_c.c_int, [_vp]),
     "mrcc_check_edges": (_c.c_int, [_vp, _c.POINTER(EdgeBatch), _c.POINTER(Tune), _vp]),
     "mrcc_check_edge": (_c.c_int, [_vp, _vp, _vp, _c.c_double, _i32, _i32, _i32, _i32, _i32, _u32, _vp, _vp]),
-    "mrcc_check_edge_host": (_c.c_int, [_vp, _dp, _dp, _c.c_double, _i32, _i32, _i32, _i32, _i32, _u32,
+    # the rows of the per-call entry points as addresses (a numpy array's .ctypes.data is the
+    # cheapest pointer to take per call)
+    "mrcc_check_edge_host": (_c.c_int, [_vp, _vp, _vp, _c.c_double, _i32, _i32, _i32, _i32, _i32, _u32,
                                         _c.POINTER(_i32)]),
-    "mrcc_check_config_host": (_c.c_int, [_vp, _dp, _i32, _u32, _c.POINTER(_i32)]),
+    "mrcc_check_config_host": (_c.c_int, [_vp, _vp, _i32, _u32, _c.POINTER(_i32)]),
     "mrcc_check_edges_host": (_c.c_int, [_vp, _dp, _dp, _i64, _c.c_double, _ip, _ip, _ip, _i32, _ip, _u32,
                                          _u8p]),
     "mrcc_check_edges_per_call": (_c.c_int, [_vp, _c.POINTER(EdgeBatch), _vp]),

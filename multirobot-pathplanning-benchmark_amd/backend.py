@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import EdgeBatch, Tune, check, dptr, iptr, lib
-from .configuration import Configuration, NpConfiguration, batch_config_cost, config_cost
+from .configuration import Configuration, NpConfiguration, batch_config_cost, config_cost, edge_num_states
 from .planning_env import BaseProblem, Mode, State, generate_binary_search_indices
 from .scenes import SceneDesc
 
@@ -258,6 +258,9 @@ class MrccEnvironment(BaseProblem):
         self._slices = sl
         self._current_mode = self.start_mode
         self._graphs = None
+        # the per-call entry points' result (one per environment: calls on it are synchronous)
+        self._res = ctypes.c_int32()
+        self._res_p = ctypes.pointer(self._res)
         if manipulation is not None:
             from .modes import SceneGraphs
 
@@ -356,37 +359,43 @@ class MrccEnvironment(BaseProblem):
         if q is None:
             raise ValueError
         v = self._flat(q)
-        res = ctypes.c_int32()
-        check(lib().mrcc_check_config_host(self.scene.handle, dptr(v), self.mode_id(mode), ALL_ROBOTS,
-                                           ctypes.byref(res)))
-        return bool(res.value)
+        check(lib().mrcc_check_config_host(self.scene.handle, v.ctypes.data, self.mode_id(mode), ALL_ROBOTS,
+                                           self._res_p))
+        return bool(self._res.value)
 
     def is_collision_free_for_robot(self, r, q, m=None, collision_tolerance=None, set_mode=True) -> bool:
         """Collisions involving robots r and no robot outside r (rai_base_env.py:515-615)."""
         v = self._flat(q)
-        res = ctypes.c_int32()
-        check(lib().mrcc_check_config_host(self.scene.handle, dptr(v), self.mode_id(m), self.robot_mask(r),
-                                           ctypes.byref(res)))
-        return bool(res.value)
+        check(lib().mrcc_check_config_host(self.scene.handle, v.ctypes.data, self.mode_id(m), self.robot_mask(r),
+                                           self._res_p))
+        return bool(self._res.value)
 
     def is_edge_collision_free(self, q1, q2, mode, resolution=None, tolerance=None,
                                include_endpoints=False, N_start=0, N_max=None, N=None) -> bool:
-        """abstract_env.py:301-354: N states at `resolution`, BFS order, window [N_start, N_max)."""
+        """abstract_env.py:301-354: N states at `resolution`, BFS order, window [N_start, N_max).
+
+        N, when not given, is the reference's (:321-323, the distance by compute_abs_max_reduction's
+        rule); the library computes it in the common case (N_start = 0).  Here it is computed only
+        for the check N_start > N (the reference asserts) and, after a failed call, to raise the
+        reference's exception for a distance that is not a number or infinite."""
         if resolution is None:
             resolution = self.collision_resolution
         a, b = self._flat(q1), self._flat(q2)
-        if N is None:
-            N = max(2, int(float(np.max(np.abs(a - b))) / resolution) + 1)
-        if N_start > N:
+        if N is None and N_start != 0:
+            N = edge_num_states(a, b, resolution)
+        if N is not None and N_start > N:
             assert False
         if N_start < 0:
             raise ValueError("N_start < 0 is not supported")
-        res = ctypes.c_int32()
-        check(lib().mrcc_check_edge_host(self.scene.handle, dptr(a), dptr(b), float(resolution), int(N),
-                                         int(N_start), -1 if N_max is None else int(N_max),
-                                         int(bool(include_endpoints)), self.mode_id(mode), ALL_ROBOTS,
-                                         ctypes.byref(res)))
-        return bool(res.value)
+        rc = lib().mrcc_check_edge_host(self.scene.handle, a.ctypes.data, b.ctypes.data, float(resolution),
+                                        0 if N is None else int(N), int(N_start),
+                                        -1 if N_max is None else int(N_max), int(bool(include_endpoints)),
+                                        self.mode_id(mode), ALL_ROBOTS, self._res_p)
+        if rc != _lib.MRCC_OK:
+            if N is None:
+                edge_num_states(a, b, resolution)  # ValueError / OverflowError as int() in the reference
+            check(rc)
+        return bool(self._res.value)
 
     # ---- batched ---------------------------------------------------------------------------
     def batch_is_collision_free(self, qs, modes=None, robots=None) -> np.ndarray:

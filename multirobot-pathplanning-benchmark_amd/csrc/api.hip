@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <deque>
 #include <mutex>
 #include <string>
@@ -917,8 +918,7 @@ int mrcc_scene_destroy(mrcc_scene* scene) {
     SceneRec* sc = &box->rec;
     (void)hipDeviceSynchronize();
     if (sc->dev_mem) (void)hipFree(sc->dev_mem);
-    if (sc->stage_dev) (void)hipFree(sc->stage_dev);
-    if (sc->stage_host) (void)hipHostFree(sc->stage_host);
+    if (sc->stage_host) (void)hipHostFree(sc->stage_host);  // stage_dev is its device address
     if (sc->batch_dev) (void)hipFree(sc->batch_dev);
     if (sc->batch_host) (void)hipHostFree(sc->batch_host);
     delete box;
@@ -1083,13 +1083,48 @@ int mrcc_check_edges(const mrcc_scene* scene, const mrcc_edge_batch* batch, cons
     return rc;
 }
 
+// The staging block of the host-pointer per-call entry points: pinned, coherent host memory that
+// the kernel reads and writes in place (no copy commands).  Layout (doubles): q1[MAX_D] q2[MAX_D],
+// then the mode id (int32) and the verdict byte.  stage_dev is the device's address of the block.
 static int ensure_stage(SceneRec* sc) {
-    if (sc->stage_dev) return MRCC_OK;
+    if (sc->stage_host) return MRCC_OK;
     const size_t bytes = 8 * (2 * MRCC_MAX_D + 4);
-    HIP_OK(hipMalloc(&sc->stage_dev, bytes));
-    HIP_OK(hipHostMalloc(&sc->stage_host, bytes));
+    void* h = nullptr;
+    HIP_OK(hipHostMalloc(&h, bytes, hipHostMallocCoherent | hipHostMallocMapped));
+    void* d = nullptr;
+    const hipError_t e = hipHostGetDevicePointer(&d, h, 0);
+    if (e != hipSuccess) {
+        (void)hipHostFree(h);
+        return fail(MRCC_ERR_HIP, "hipHostGetDevicePointer failed: %s", hipGetErrorString(e));
+    }
+    memset(h, 0, bytes);
+    sc->stage_host = (double*)h;
+    sc->stage_dev = (double*)d;
     sc->stage_out_dev = (unsigned char*)(sc->stage_dev + 2 * MRCC_MAX_D + 2);
     sc->stage_out_host = (unsigned char*)(sc->stage_host + 2 * MRCC_MAX_D + 2);
+    return MRCC_OK;
+}
+
+#define MRCC_NO_VERDICT 0xFF  // the kernels write 0 or 1
+
+// Waits for the verdict byte that the launch on `st` writes into the staging block: spins on it for
+// up to 200 us (a per-call check takes a few), then waits for the stream.  The byte is written once,
+// after every wave has read its rows, so the next call may refill the block.
+static int wait_verdict(SceneRec* sc, hipStream_t st, int32_t* result) {
+    volatile const unsigned char* p = sc->stage_out_host;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned k = 1;; ++k) {
+        const unsigned char v = *p;
+        if (v != MRCC_NO_VERDICT) {
+            *result = v ? 1 : 0;
+            return MRCC_OK;
+        }
+        if ((k & 63) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(200)) break;
+    }
+    HIP_OK(hipStreamSynchronize(st));
+    const unsigned char v = *p;
+    if (v == MRCC_NO_VERDICT) return fail(MRCC_ERR_HIP, "the per-call kernel wrote no verdict");
+    *result = v ? 1 : 0;
     return MRCC_OK;
 }
 
@@ -1123,15 +1158,15 @@ static EdgeBatchView one_edge_view(const SceneRec* sc, const double* q1, const d
     return v;
 }
 
-// Copies one or two host rows into the scene's pinned and device staging block (synchronous
-// callers only: the previous call has finished reading it).  Layout: q1[MAX_D] q2[MAX_D] ... out.
-static int stage_rows(SceneRec* sc, const double* q1h, const double* q2h, hipStream_t st) {
+// Copies one or two host rows into the scene's staging block and clears its verdict byte
+// (synchronous callers only: the previous call's kernel has finished reading the block).
+static int stage_rows(SceneRec* sc, const double* q1h, const double* q2h) {
     int rc = ensure_stage(sc);
     if (rc != MRCC_OK) return rc;
     double* h = sc->stage_host;
     if (q1h) memcpy(h, q1h, 8 * (size_t)sc->D);
     if (q2h) memcpy(h + MRCC_MAX_D, q2h, 8 * (size_t)sc->D);
-    HIP_OK(hipMemcpyAsync(sc->stage_dev, h, 8 * (2 * MRCC_MAX_D), hipMemcpyHostToDevice, st));
+    *(volatile unsigned char*)sc->stage_out_host = MRCC_NO_VERDICT;
     return MRCC_OK;
 }
 
@@ -1173,16 +1208,13 @@ int mrcc_check_edge_host(const mrcc_scene* scene, const double* q1, const double
     if (N_start > Nh) return fail(MRCC_ERR_INVALID, "N_start %d > N %d (the reference asserts)", N_start, Nh);
     if (Nh > MRCC_MAX_EDGE_STATES) return fail(MRCC_ERR_INVALID, "edge of %d states (at most %d)", Nh, MRCC_MAX_EDGE_STATES);
     hipStream_t st = nullptr;
-    rc = stage_rows(sc, q1, q2, st);
+    rc = stage_rows(sc, q1, q2);
     if (rc != MRCC_OK) return rc;
     const EdgeBatchView v = one_edge_view(sc, sc->stage_dev, sc->stage_dev + MRCC_MAX_D, resolution, N, N_start,
                                           N_max, include_endpoints, mode_id, robot_mask, sc->stage_out_dev);
     launch_edge1(*sc, v, st);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(sc->stage_out_host, sc->stage_out_dev, 1, hipMemcpyDeviceToHost, st));
-    HIP_OK(hipStreamSynchronize(st));
-    *result = sc->stage_out_host[0] ? 1 : 0;
-    return MRCC_OK;
+    return wait_verdict(sc, st, result);
 }
 
 // Pinned and device staging of at least `bytes` for mrcc_check_edges_host (calls are synchronous,
@@ -1292,19 +1324,14 @@ int mrcc_check_config_host(const mrcc_scene* scene, const double* q, int32_t mod
     int rc = check_mode(sc, mode_id);
     if (rc != MRCC_OK) return rc;
     hipStream_t st = nullptr;
-    rc = stage_rows(sc, q, nullptr, st);
+    rc = stage_rows(sc, q, nullptr);
     if (rc != MRCC_OK) return rc;
     // the mode id travels in the staging block's int area (after the two rows)
-    int32_t* ih = (int32_t*)(sc->stage_host + 2 * MRCC_MAX_D);
-    ih[0] = mode_id;
+    ((int32_t*)(sc->stage_host + 2 * MRCC_MAX_D))[0] = mode_id;
     const int32_t* md = (const int32_t*)(sc->stage_dev + 2 * MRCC_MAX_D);
-    HIP_OK(hipMemcpyAsync((void*)md, ih, 4, hipMemcpyHostToDevice, st));
     launch_configs(*sc, sc->stage_dev, 1, sc->D, 1, md, robot_mask, sc->stage_out_dev, 1, st);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(sc->stage_out_host, sc->stage_out_dev, 1, hipMemcpyDeviceToHost, st));
-    HIP_OK(hipStreamSynchronize(st));
-    *result = sc->stage_out_host[0] ? 1 : 0;
-    return MRCC_OK;
+    return wait_verdict(sc, st, result);
 }
 
 int mrcc_check_edges_per_call(const mrcc_scene* scene, const mrcc_edge_batch* batch, void* stream) {

@@ -24,7 +24,8 @@ struct SceneRec {
     // host copies needed by host code
     int arm_na, arm_dof, arm_S, arm_nm;
     int arm_has_grasp;    // some mode holds a box: the evaluator with the held-box phase
-    // staging buffers of the host-pointer per-call entry points (lazily allocated)
+    // staging block of the host-pointer per-call entry points (lazily allocated): pinned coherent
+    // host memory, read and written by the kernels in place; stage_dev is its device address
     double* stage_dev;
     double* stage_host;
     unsigned char* stage_out_dev;

@@ -786,26 +786,37 @@ __global__ void __launch_bounds__(256) k_arm_direct(ArmK V, EdgeBatchView B, con
     arm_stats<true>(stats, rechecks, cnt);
 }
 
+// One wave per configuration.  The wave's first configuration (its row, its mode and the joint
+// sines and cosines) is loaded and computed while the scene image is in flight, as in the per-call
+// edge kernel; a single configuration (the per-call form) is then one staging round trip plus
+// the evaluation.
 template <bool GRASP, bool CHAIN>
 __global__ void __launch_bounds__(256, 3) k_arm_configs(ArmK V, const double* qv, long long n,
                                                      long long rs, long long ds, const int* modes,
                                                      unsigned mask, unsigned char* out) {
     extern __shared__ double lds_arm[];
-    arm_stage_scene(lds_arm, V);
-    const ArmLds Wl = arm_lds(lds_arm, V);
-    const ArmRegs R = arm_regs(V, Wl);
     const int nw = blockDim.x >> 6;
     const int lane = wave_lane();
+    const long long r0 = (long long)blockIdx.x * nw + (threadIdx.x >> 6);
+    // coordinates of inactive arms are loaded too and zeroed once the mode is known
+    const double q0 = r0 < n && lane < V.D ? qv[r0 * rs + (long long)lane * ds] : 0.0;
+    const int mode0 = r0 < n && modes ? modes[r0] : 0;
+    const auto stage = arm_stage_load<4>(V);
+    const SinCos t0 = sincos_f64(q0);
+    arm_stage_store(lds_arm, V, stage);  // ends with a barrier
+    const ArmLds Wl = arm_lds(lds_arm, V);
+    const ArmRegs R = arm_regs(V, Wl);
     NoCount rechecks;
     ArmCountT<false> cnt;
-    for (long long r = (long long)blockIdx.x * nw + (threadIdx.x >> 6); r < n;
-         r += (long long)gridDim.x * nw) {
-        const int mode = modes ? modes[r] : 0;
+    for (long long r = r0; r < n; r += (long long)gridDim.x * nw) {
+        const bool first = r == r0;
+        const int mode = first ? mode0 : (modes ? modes[r] : 0);
         const int md = (mode >= 0 && mode < V.nm) ? mode : 0;
         const int Dact = arm_mact(V, Wl, md) * V.dof;
-        const double q = lane < Dact ? qv[r * rs + (long long)lane * ds] : 0.0;
+        const double q = lane < Dact ? (first ? q0 : qv[r * rs + (long long)lane * ds]) : 0.0;
         // a mode outside [0, nm): verdict 0 (arm_state_collides reports it as a collision)
-        const bool c = arm_state_collides<GRASP, CHAIN, false>(V, Wl, R, q, mode, mask, rechecks, cnt);
+        const bool c = arm_state_collides<GRASP, CHAIN, false>(V, Wl, R, q, mode, mask, rechecks, cnt, false,
+                                                               first, t0);
         if (lane == 0) out[r] = c ? 0 : 1;
     }
 }
@@ -900,6 +911,17 @@ __global__ void __launch_bounds__(256) k_lane_edge(View S, EdgeBatchView B) {
     if (!w.err) {
         const double* a = row1(B, 0);
         const double* b = row2(B, 0);
+        // the rows in registers for the specialised scenes (one round trip, also when the rows
+        // live in host memory); the generic ones read them per state
+        constexpr bool HOLD = MAXD <= 12;
+        double ra[HOLD ? MAXD : 1], rb[HOLD ? MAXD : 1];
+        if constexpr (HOLD) {
+#pragma unroll
+            for (int k = 0; k < MAXD; ++k) {
+                ra[k] = k < D ? a[k * B.ds] : 0.0;
+                rb[k] = k < D ? b[k * B.ds] : 0.0;
+            }
+        }
         volatile int* hit = &s_hit;
         for (int base = 0; base < w.M; base += blockDim.x) {
             if (*hit) break;  // the same LDS word for every lane
@@ -910,8 +932,13 @@ __global__ void __launch_bounds__(256) k_lane_edge(View S, EdgeBatchView B) {
                 if (i >= 0) {
                     double q[MAXD];
 #pragma unroll
-                    for (int k = 0; k < MAXD; ++k)
-                        if (k < D) q[k] = interp_f64(a[k * B.ds], b[k * B.ds], N, i);
+                    for (int k = 0; k < MAXD; ++k) {
+                        if constexpr (HOLD) {
+                            if (k < D) q[k] = interp_f64(ra[k], rb[k], N, i);
+                        } else {
+                            if (k < D) q[k] = interp_f64(a[k * B.ds], b[k * B.ds], N, i);
+                        }
+                    }
                     coll = lane_collides<View, T1, T2>(S, q, B.mask, rechecks);
                 }
             }

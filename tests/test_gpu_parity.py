@@ -461,6 +461,52 @@ def test_c3_non_finite_edges(c3, B):
             gpu_edges(sc, a2, b, 0.05)
 
 
+def test_c3_environment_per_call(c3, B):
+    _say("C3 per-call host path (MrccEnvironment): configs, edges, windows, masks, non-finite lengths")
+    desc, sc, o, rng = c3
+    env = B.MrccEnvironment(desc, collision_resolution=0.05)
+    Q = np.concatenate([near_home(rng, desc, 150, sigma=0.4), scenes.uniform_configs(rng, desc, 50)])
+    for q in Q:
+        assert env.is_collision_free(q, None) == o.config_free(q)
+    assert env.is_collision_free_for_robot(["a1", "a3"], Q[0], None) == o.config_free(Q[0], mask=0b0101)
+    # edges from free starts (short steps: about half of them free) and from random starts
+    s = near_home(rng, desc, 3000, sigma=0.3)
+    a = np.concatenate([s[o.check_configs(s)][:60], near_home(rng, desc, 140, sigma=0.4)])
+    b = np.clip(a + rng.normal(0, 0.1, a.shape), desc.limits[0], desc.limits[1])
+    n_free = 0
+    for i in range(len(a)):
+        got = env.is_edge_collision_free(a[i], b[i], None)
+        assert got == o.edge_free(a[i], b[i], 0.05), i
+        n_free += got
+        if i % 4 == 0:  # windows of the binary order (the path schedule), endpoints included
+            N = pkg.edge_num_states(a[i], b[i], 0.05)
+            ns = int(rng.integers(0, N + 1))
+            nm = ns + int(rng.integers(0, 12))
+            assert env.is_edge_collision_free(a[i], b[i], None, N_start=ns, N_max=nm, include_endpoints=True) == \
+                o.edge_free(a[i], b[i], 0.05, N=N, N_start=ns, N_max=nm, include_endpoints=True), i
+    assert 10 <= n_free <= len(a) - 10  # both outcomes occur
+    # the reference's N rule: a NaN away from coordinate 0 is skipped by the distance (the states
+    # holding it collide); in coordinate 0 int(nan) raises ValueError, an infinite length
+    # OverflowError (abstract_env.py:321-323)
+    a2 = a[0].copy()
+    a2[5] = np.nan
+    assert env.is_edge_collision_free(a2, b[0], None) == o.edge_free(a2, b[0], 0.05)
+    a2 = a[0].copy()
+    a2[0] = np.nan
+    with pytest.raises(ValueError):
+        env.is_edge_collision_free(a2, b[0], None)
+    a2 = a[0].copy()
+    a2[9] = np.inf
+    with pytest.raises(OverflowError):
+        env.is_edge_collision_free(a2, b[0], None)
+    # the staging block is reused call after call: a colliding and a free call alternate
+    free_i = next(i for i in range(len(a)) if o.edge_free(a[i], b[i], 0.05))
+    hit_i = next(i for i in range(len(a)) if not o.edge_free(a[i], b[i], 0.05))
+    for _ in range(20):
+        assert env.is_edge_collision_free(a[free_i], b[free_i], None)
+        assert not env.is_edge_collision_free(a[hit_i], b[hit_i], None)
+
+
 def test_c3_masks(c3):
     _say("C3 per-arm masks")
     desc, sc, o, rng = c3
