@@ -201,9 +201,12 @@ int mrcc_check_edge(const mrcc_scene* scene, const double* q1, const double* q2,
                     double resolution, int32_t N, int32_t N_start, int32_t N_max,
                     int32_t include_endpoints, int32_t mode_id, uint32_t robot_mask,
                     uint8_t* out, void* stream);
-/* The planners' call pattern on host memory: q1, q2 (or q) are host double[D]; inputs travel
- * through a pinned staging buffer of the scene, one kernel launch, synchronous.  *result = 1 when
- * collision free, 0 otherwise (is_edge_collision_free / is_collision_free of one call). */
+/* The planners' call pattern on host memory: q1, q2 (or q) are host double[D]; they are copied
+ * into a pinned, coherent staging block of the scene that the kernel reads in place and into which
+ * it writes the verdict (no copy commands); one kernel launch on the null stream; the call returns
+ * once the verdict has arrived (spinning on it for up to 200 us, then waiting for the stream).
+ * *result = 1 when collision free, 0 otherwise (is_edge_collision_free / is_collision_free of one
+ * call).  Calls on one scene must not overlap (they share the staging block). */
 int mrcc_check_edge_host(const mrcc_scene* scene, const double* q1, const double* q2,
                          double resolution, int32_t N, int32_t N_start, int32_t N_max,
                          int32_t include_endpoints, int32_t mode_id, uint32_t robot_mask,
