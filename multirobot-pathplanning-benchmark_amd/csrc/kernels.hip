@@ -309,6 +309,18 @@ __device__ inline long long claim_chunk(const WV& W, int& q, long long nchunks) 
     return -1;
 }
 
+// The verdict byte of an edge in the persistent kernels, as every XCD sees it: another wave
+// writes an edge's 0 when one of its states collides, and the later slots of the edge are skipped
+// when they read it.  Each XCD has its own L2 and plain accesses may stay in it (a plain read can
+// return an XCD's stale copy until the kernel ends); agent-scope accesses reach the level that all
+// XCDs share.
+__device__ inline bool edge_dead(const unsigned char* out, int e) {
+    return __hip_atomic_load(out + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+}
+__device__ inline void edge_kill(unsigned char* out, int e) {
+    __hip_atomic_store(out + e, (unsigned char)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 struct SlotRef {
     int e;      // edge, -1 when the slot is padding
     int local;  // slot within the edge
@@ -419,7 +431,7 @@ __global__ void __launch_bounds__(256) k_lane_persistent(View S, EdgeBatchView B
             SlotRef ref = slot_ref(W, p, s, lgG, B.n);
             ++n_slots;
             if (ref.e < 0) continue;
-            if (B.out[ref.e] == 0) { ++n_dead; continue; }
+            if (edge_dead(B.out, ref.e)) { ++n_dead; continue; }
             EdgeWin w = win_of(W, ref.e, B.incl);
             const int i = edge_state_index(w, ref.local, B.incl);
             if (i < 0) continue;
@@ -431,7 +443,7 @@ __global__ void __launch_bounds__(256) k_lane_persistent(View S, EdgeBatchView B
             ++n_eval;
             const int mode = B.modes ? B.modes[ref.e] : 0;
             (void)mode;
-            if (lane_collides<View, T1, T2>(S, q, B.mask, rechecks)) B.out[ref.e] = 0;
+            if (lane_collides<View, T1, T2>(S, q, B.mask, rechecks)) edge_kill(B.out, ref.e);
         }
     }
     if (W.stats) {
@@ -466,7 +478,7 @@ __global__ void __launch_bounds__(64) k_lane_direct(View S, EdgeBatchView B, con
         const double* a = row1(B, e);
         const double* b = row2(B, e);
         for (int base = u * 64; base < w.M; base += wpe * 64) {
-            if (B.out[e] == 0) break;  // wave-uniform: same address
+            if (edge_dead(B.out, (int)e)) break;  // wave-uniform: same address (other workgroups of the edge)
             const int local = base + lane;
             bool coll = false;
             if (local < w.M) {
@@ -481,7 +493,7 @@ __global__ void __launch_bounds__(64) k_lane_direct(View S, EdgeBatchView B, con
                 }
             }
             if (__any(coll)) {
-                if (lane == 0) B.out[e] = 0;
+                if (lane == 0) edge_kill(B.out, (int)e);
                 break;
             }
         }
@@ -599,7 +611,7 @@ __device__ inline void arm_persistent_body(const ArmK& V, const ArmQueue& B, con
                 const int4 rec = W.erec[ee];
                 const int S = rec.z, Mv = rec.y, Nv = rec.x, w0 = erec_w0(rec);
                 const int full = erec_full(rec);
-                const unsigned char o = B.out[ee];
+                const unsigned char o = edge_dead(B.out, ee) ? 0 : 1;
                 const int md0 = batch_mode(B, ee);
                 const int local = k * S + pl;
                 if (local < Mv) {
@@ -702,7 +714,7 @@ __device__ inline void arm_persistent_body(const ArmK& V, const ArmQueue& B, con
                                                                      __builtin_popcount(recent & 0xffffu) >= 8);
             recent = (recent << 1) | (col ? 1u : 0u);
             if (col) {
-                if (lane == 0) B.out[ej] = 0;
+                if (lane == 0) edge_kill(B.out, ej);
                 deadm |= __ballot(e == ej);
             }
         }
@@ -767,7 +779,7 @@ __global__ void __launch_bounds__(256) k_arm_direct(ArmK V, EdgeBatchView B, con
         }
         bool hit = false;
         for (int local = u; local < w.M; local += wpe) {
-            if (!init_out && B.out[e] == 0) break;
+            if (!init_out && edge_dead(B.out, (int)e)) break;
             const int i = edge_state_index(w, local, B.incl);
             if (i < 0) continue;
             const double qv = arm_load_state(B, e, N, i, arm_mact(V, Wl, md) * V.dof);
@@ -778,7 +790,7 @@ __global__ void __launch_bounds__(256) k_arm_direct(ArmK V, EdgeBatchView B, con
             }
         }
         if (wave_lane() == 0) {
-            if (hit) B.out[e] = 0;
+            if (hit) edge_kill(B.out, (int)e);
             else if (init_out) B.out[e] = 1;
         }
     }
