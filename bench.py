@@ -590,7 +590,12 @@ def secondary_workloads(args, backend, scenes, dev):
         n_states, N = logical_states(q1, q2, res)
         a, b = torch.from_numpy(q1).to(dev), torch.from_numpy(q2).to(dev)
         o_ = torch.empty(len(q1), dtype=torch.uint8, device=dev)
-        plan = backend.EdgePlan(sc, sc.edge_batch(a, b, o_, res), backend.make_tune(use_graph=1))
+        # execution shape from the autotuner (queue shapes and the single launch, timed on this batch)
+        autotune = importlib.import_module(f"{PKG}.autotune")
+        lbatch = sc.edge_batch(a, b, o_, res)
+        ltune, ltune_ms = autotune.autotune(sc, lbatch, float(np.mean(N)), use_cache=False)
+        lbest = min(ltune_ms, key=ltune_ms.get)
+        plan = backend.EdgePlan(sc, lbatch, ltune)
         plan.run()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -602,6 +607,8 @@ def secondary_workloads(args, backend, scenes, dev):
         got = o_.cpu().numpy().astype(bool)
         pc_rate, pc_got, pc_m, pc_us = per_call_rate(sc, q1, q2, res, N, dev)
         ent = {"edges": len(q1), "edge_states": n_states, "gpu_batched": gpu_rate, "unit": "edge-states/s",
+               "autotuned": {"choice": lbest, "ms": ltune_ms[lbest], "candidates": len(ltune_ms),
+                             "default_shape_ms": ltune_ms.get("G8/t4/w16")},
                "gpu_per_call": {"value": pc_rate, "edges": pc_m, "us_per_edge": pc_us,
                                 "matches": bool(np.array_equal(pc_got, got[:pc_m])),
                                 "note": "one launch per edge (mrcc_check_edges_per_call)"}}

@@ -1,7 +1,9 @@
 """Autotuner of the batched edge check.
 
 Times candidate execution shapes (states per pass G, queue slots per ticket, persistent waves per
-CU) on the caller's own batch with HIP events and keeps the fastest.  Choices are cached in a JSON
+CU, and the single launch without a queue: one workgroup per edge walking its states in rounds,
+which wins on small batches of long edges that mostly collide early) on the caller's own batch
+with HIP events and keeps the fastest.  Choices are cached in a JSON
 file keyed by the GPU architecture and a workload signature (scene kind, dimension, sphere count,
 edge count and mean state count, both rounded to powers of two), so a process pays the search
 once per workload shape.
@@ -19,10 +21,17 @@ from .backend import EdgePlan, make_tune
 
 CACHE = os.environ.get("MRCC_AUTOTUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "mrcc_autotune.json"))
 
+DIRECT = dict(persistent=0, sorted=0)  # the single launch: no queue, one workgroup per edge
 LANE_GRID = [dict(slots_per_pass=g, slots_per_ticket=t, waves_per_cu=w)
-             for g in (4, 8, 16, 32) for t in (1, 2) for w in (8, 16, 32)]
+             for g in (4, 8, 16, 32) for t in (1, 2, 4) for w in (8, 16)] + [DIRECT]
 ARM_GRID = [dict(slots_per_pass=g, slots_per_ticket=t, waves_per_cu=w)
-            for g in (1, 2, 4, 8) for t in (8, 16, 32) for w in (12, 16)]
+            for g in (1, 2, 4, 8) for t in (8, 16, 32) for w in (12, 16)] + [DIRECT]
+
+
+def candidate_name(cand: dict) -> str:
+    if not cand.get("persistent", 1):
+        return "direct"
+    return "G{slots_per_pass}/t{slots_per_ticket}/w{waves_per_cu}".format(**cand)
 
 
 def _p2(x: float) -> int:
@@ -78,7 +87,7 @@ def autotune(scene, batch, mean_states: float, reps: int = 3, grid=None, use_cac
         e1.record()
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / reps
-        name = "G{slots_per_pass}/t{slots_per_ticket}/w{waves_per_cu}".format(**cand)
+        name = candidate_name(cand)
         timings[name] = ms
         if verbose:
             print(f"[autotune] {name}: {ms:.3f} ms", flush=True)

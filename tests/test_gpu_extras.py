@@ -79,8 +79,9 @@ def test_autotune_choice_is_correct(arms):
     out = torch.empty(3000, dtype=torch.uint8, device="cuda")
     batch = env.scene.edge_batch(a, b, out, 0.05)
     grid = [dict(slots_per_pass=g, slots_per_ticket=t, waves_per_cu=w) for g, t, w in ((1, 1, 8), (4, 2, 16), (8, 4, 24))]
+    grid.append(at.DIRECT)
     tune, timings = at.autotune(env.scene, batch, mean_states=5.0, reps=2, grid=grid, use_cache=False)
-    assert len(timings) == 3
+    assert len(timings) == 4 and "direct" in timings
     for cand in grid:
         got = gpu_edges(env.scene, q1, q2, 0.05, tune=backend().make_tune(**cand))
         assert np.array_equal(got, ref), cand
