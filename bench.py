@@ -539,6 +539,13 @@ def lane_roofline(args, backend, sc, desc, rng, dev, res, threads, oracle_scene)
            "frac": ach / PEAK_FP32_TFLOPS, "flops_per_state": fl, "states_per_launch": f_states,
            "edges": len(fq1), "ms_per_launch": split["kernel_ms"],
            "kernel": "k_lane_persistent, all-free edges (every state evaluated)"}
+    # the single launch (one workgroup of 64 lanes per edge, states in rounds of 64) on the same set
+    dsplit = timed_split(backend.EdgePlan(sc, sc.edge_batch(fa, fb, fo, res),
+                                          backend.make_tune(persistent=0, sorted=0, use_graph=0, time_kernels=1)),
+                         args.steps)
+    dach = f_states * fl / (dsplit["kernel_ms"] * 1e-3) / 1e12
+    ent["direct"] = {"achieved": dach, "frac": dach / PEAK_FP32_TFLOPS, "ms_per_launch": dsplit["kernel_ms"],
+                     "kernel": "k_lane_direct, the same all-free edges"}
     if oracle_scene is not None:
         k = min(len(fq1), 16384)
         ref, n_ev = oracle_scene.check_edges(fq1[:k], fq2[:k], res, threads=threads, count=True)
