@@ -582,7 +582,19 @@ __device__ inline void arm_stage_store(void* base, const ArmK& V, const ArmStage
         const int i = threadIdx.x + k * nt;
         if (i < n16) dst[i] = r.v[k];
     }
-    for (int i = threadIdx.x + K * nt; i < n16; i += nt) dst[i] = src[i];
+    // the rest, four loads in flight per thread
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int i0 = threadIdx.x + K * nt; i0 < n16; i0 += 4 * nt) {
+        const int i1 = i0 + nt, i2 = i1 + nt, i3 = i2 + nt;
+        const float4 a = src[i0];
+        const float4 b = i1 < n16 ? src[i1] : z;
+        const float4 c = i2 < n16 ? src[i2] : z;
+        const float4 d = i3 < n16 ? src[i3] : z;
+        dst[i0] = a;
+        if (i1 < n16) dst[i1] = b;
+        if (i2 < n16) dst[i2] = c;
+        if (i3 < n16) dst[i3] = d;
+    }
     __syncthreads();
 }
 

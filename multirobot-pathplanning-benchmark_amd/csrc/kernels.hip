@@ -1155,8 +1155,10 @@ void launch_configs(const SceneRec& sc, const double* q, long long n, long long 
             // one wave per configuration up to 4 * max_grid waves
             long long g = n < (long long)max_grid * 4 ? n : (long long)max_grid * 4;
             if (g < 1) g = 1;
-            const int nw = g < sc.arm_nw ? 1 : sc.arm_nw;
-            const size_t lds = nw == 1 ? sc.arm_lds1 : sc.arm_lds;
+            // a full workgroup also for fewer configurations than waves: the extra waves only help
+            // stage the scene (one round trip of four loads per thread instead of several)
+            const int nw = sc.arm_nw;
+            const size_t lds = sc.arm_lds;
             const int blocks = (int)((g + nw - 1) / nw);
 #define MRCC_ARM_CONFIGS(G, C) k_arm_configs<G, C><<<blocks, 64 * nw, lds, st>>>(sc.armk, q, n, rs, ds, modes, mask, out)
             if (sc.arm.chain) {
