@@ -938,32 +938,45 @@ def clearance_workload(args, backend, scenes, dev, threads):
     bwd = lambda: lib.check(lib.lib().mrcc_clearance_backward(sc.handle, ctypes.c_void_p(qf.data_ptr()), n, D,
                                                               ctypes.c_float(0.02), ctypes.c_void_p(gc.data_ptr()),
                                                               ctypes.c_void_p(gq.data_ptr()), st))
-    kms = {}
-    for name, f in (("forward", fwd), ("backward", bwd)):
-        f()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        e0.record()
-        for _ in range(reps):
+    def kernel_times():
+        out = {}
+        for name, f in (("forward", fwd), ("backward", bwd)):
             f()
-        e1.record()
-        torch.cuda.synchronize()
-        kms[name] = e0.elapsed_time(e1) / reps
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(reps):
+                f()
+            e1.record()
+            torch.cuda.synchronize()
+            out[name] = e0.elapsed_time(e1) / reps
+        return out
+
+    kms = kernel_times()  # as shipped: pairs culled with the links' bounding spheres
+    cost_culled = cost.clone()
+    sc.set_broadphase(False)  # the full pair set, for the roofline
+    kms_full = kernel_times()
+    sc.set_broadphase(True)
+    full_cost_diff = float((cost - cost_culled).abs().max())
     fl_fwd, pinfo = arm_flops_per_state(desc)
     # backward: the forward's pair set again (recomputed) + per sphere and joint of its chain the
     # cross product z x (c - o) and its dot with the force (about 18 flops; a dof/2 chain on average)
     fl_bwd = fl_fwd + 18 * pinfo["spheres"] * desc.dof // 2
-    ach_f = n * fl_fwd / (kms["forward"] * 1e-3) / 1e12
-    ach_b = n * fl_bwd / (kms["backward"] * 1e-3) / 1e12
+    ach_f = n * fl_fwd / (kms_full["forward"] * 1e-3) / 1e12
+    ach_b = n * fl_bwd / (kms_full["backward"] * 1e-3) / 1e12
     ent = {"paths": P, "waypoints": P * W, "dof": D, "ms_forward_backward": fb * 1e3,
            "waypoints_per_s": P * W / fb, "mean_cost": float(c.detach().mean()),
            "kernel_ms": kms,
+           "kernel_ms_full_pair_set": kms_full,
+           "full_pair_set_max_cost_difference": full_cost_diff,
            "roofline": {"bound": "valu", "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                         "forward": {"achieved": ach_f, "frac": ach_f / PEAK_FP32_TFLOPS, "flops_per_waypoint": fl_fwd},
                         "backward": {"achieved": ach_b, "frac": ach_b / PEAK_FP32_TFLOPS,
                                      "flops_per_waypoint": fl_bwd},
-                        "kernel": "k_clearance (one waypoint per wavefront, full pair set, square root only "
-                                  "for pairs within reach)"},
+                        "kernel": "k_clearance with the broad phase off (the full pair set: lane per sphere, "
+                                  "every partner sphere and box, square root only for pairs within reach); "
+                                  "as shipped the pairs are culled with the links' bounding spheres "
+                                  "(kernel_ms)"},
            "note": "ms_forward_backward is wall time of the autograd pair (forward, sum, backward, "
                    "gradient accumulation) from Python; kernel_ms are HIP events around the two library "
                    "kernels alone"}

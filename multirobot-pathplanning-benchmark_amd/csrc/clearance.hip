@@ -3,8 +3,9 @@
 //   cost(q) = sum over the checked pair set of max(0, m + r_i + r_j - |c_i - c_j|)^2
 //           + sum over spheres x boxes of max(0, m + r_i - dist(c_i, box))^2
 // with the pair set of the boolean check (links >= 2 apart in one arm, all links of different
-// arms).  One waypoint per wavefront.  The backward recomputes the forward kinematics, gathers the
-// force on every sphere centre (d cost / d c) in LDS and maps it to the joints with
+// arms), culled exactly with the links' bounding spheres.  One waypoint per wavefront.  The
+// backward recomputes the forward kinematics, gathers the force on every sphere centre
+// (d cost / d c) in LDS and maps it to the joints with
 // d c / d q_k = z_{k-1} x (c - o_{k-1}) (revolute joints) or z_{k-1} (prismatic chain joints) for
 // spheres on links >= k of the same arm.
 #include "engine.hpp"
@@ -18,6 +19,8 @@ struct ClrLds {
     float4* sph;   // [S]
     float* F;      // [S*3] forces (backward)
     float* red;    // [64]
+    float4* lk;    // [L] link bounding spheres (x, y, z, R)
+    int* cand;     // [64] candidates of one culling round
 };
 
 __device__ inline ClrLds clr_lds(void* base, const ArmView& V) {
@@ -29,6 +32,8 @@ __device__ inline ClrLds clr_lds(void* base, const ArmView& V) {
     W.sph = (float4*)(((uintptr_t)(W.X + 12 * V.L) + 15) & ~uintptr_t(15));
     W.F = (float*)(W.sph + V.S);
     W.red = W.F + 3 * V.S;
+    W.lk = (float4*)(((uintptr_t)(W.red + 64) + 15) & ~uintptr_t(15));
+    W.cand = (int*)(W.lk + V.L);
     return W;
 }
 
@@ -92,8 +97,33 @@ __device__ inline void clr_fk(const ArmView& V, const ClrLds& W, const float* q)
     __syncthreads();
 }
 
-// Accumulates the cost of the pair set; when `grad`, also the forces in W.F (zeroed here).
-__device__ inline float clr_pairs(const ArmView& V, const ClrLds& W, float m, bool grad) {
+// Slack of the link-level cull (metres): covers the binary32 rounding of the link and sphere
+// centres (micrometres) with a wide margin.
+#define CLR_CULL_EPS 1e-4f
+
+// One penetration term: cost += pen^2 for pen = R - d > 0, and the force on the first sphere
+// (fx, fy, fz += g e, g = -2 pen / d) when `grad`.  The square root only for pairs within reach:
+// d < R implies e2 < R^2 (1 + 2^-21), so the bound passes every pair with pen > 0.
+__device__ inline bool clr_term(float ex, float ey, float ez, float R, bool grad, float& cost, float& gx,
+                                float& gy, float& gz) {
+    const float e2 = ex * ex + ey * ey + ez * ez;
+    if (!(e2 <= R * R * 1.0001f)) return false;
+    const float d = sqrtf(e2);
+    const float pen = R - d;
+    if (!(pen > 0.f)) return false;
+    cost += pen * pen;
+    if (!grad || !(d > 0.f)) return false;
+    const float g = -2.f * pen / d;
+    gx = g * ex;
+    gy = g * ey;
+    gz = g * ez;
+    return true;
+}
+
+// The full pair set without culling (the scene's broad phase off): lane = sphere i, every partner
+// j >= sph_lo[i] and every box in turn.  Accumulates the cost; when `grad`, also the forces in W.F
+// (zeroed here).
+__device__ inline float clr_pairs_all(const ArmView& V, const ClrLds& W, float m, bool grad) {
     const int lane = threadIdx.x;
     const int S = V.S;
     if (grad) {
@@ -170,6 +200,118 @@ __device__ inline float clr_pairs(const ArmView& V, const ClrLds& W, float m, bo
     return total;
 }
 
+// Accumulates the cost of the pair set; when `grad`, also the forces in W.F (zeroed here).
+// Terms are nonzero only for pairs within reach, so both parts are culled with the links'
+// bounding spheres first (the triangle inequality: a sphere of link A lies within R_A - r of A's
+// centre, so no pair of two links farther apart than m + R_A + R_B, and no sphere of a link
+// farther than m + R_A from a box, penetrates); the surviving (link, box) and (link, link)
+// candidates are expanded to their spheres, 8 x 8 lanes at a time.  With the scene's broad phase
+// off (V.broadphase = 0) the full pair set is walked instead (clr_pairs_all).
+__device__ inline float clr_pairs(const ArmView& V, const ClrLds& W, float m, bool grad) {
+    if (!V.broadphase) return clr_pairs_all(V, W, m, grad);
+    const int lane = threadIdx.x;
+    const int S = V.S, L = V.L;
+    if (grad)
+        for (int t = lane; t < 3 * S; t += 64) W.F[t] = 0.f;
+    for (int l = lane; l < L; l += 64) {
+        const float* X = W.X + 12 * l;
+        const double* b = V.link_bc64 + 3 * l;
+        const float b0 = (float)b[0], b1 = (float)b[1], b2 = (float)b[2];
+        W.lk[l] = make_float4(X[0] * b0 + X[1] * b1 + X[2] * b2 + X[9], X[3] * b0 + X[4] * b1 + X[5] * b2 + X[10],
+                              X[6] * b0 + X[7] * b1 + X[8] * b2 + X[11], V.link_R32[l]);
+    }
+    __syncthreads();
+    float cost = 0.f;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    // 1. spheres vs boxes
+    const int nlb = L * V.nb;
+    for (int t0 = 0; t0 < nlb; t0 += 64) {
+        const int t = t0 + lane;
+        bool cand = false;
+        int code = 0;
+        if (t < nlb) {
+            const int l = t / V.nb, b = t - l * V.nb;
+            const float4 c = W.lk[l];
+            const float* bx = V.box32 + 8 * b;
+            const float ex = c.x - fminf(fmaxf(c.x, bx[0]), bx[4]);
+            const float ey = c.y - fminf(fmaxf(c.y, bx[1]), bx[5]);
+            const float ez = c.z - fminf(fmaxf(c.z, bx[2]), bx[6]);
+            const float R = m + c.w + CLR_CULL_EPS;
+            cand = V.link_first[l + 1] > V.link_first[l] && ex * ex + ey * ey + ez * ez <= R * R;
+            code = l | (b << 8);
+        }
+        const unsigned long long mk = __ballot(cand);
+        if (mk == 0ull) continue;  // wave-uniform
+        if (cand) W.cand[__popcll(mk & below)] = code;
+        __syncthreads();
+        const int n = __popcll(mk);
+        for (int c0 = 0; c0 < n; c0 += 8) {  // 8 candidates x 8 spheres per round
+            const int g = lane >> 3, k = lane & 7;
+            if (c0 + g >= n) continue;
+            const int cd = W.cand[c0 + g];
+            const int l = cd & 255, b = cd >> 8;
+            const int s = V.link_first[l] + k;
+            if (s >= V.link_first[l + 1]) continue;
+            const float4 ci = W.sph[s];
+            const float* bx = V.box32 + 8 * b;
+            float gx, gy, gz;
+            if (clr_term(ci.x - fminf(fmaxf(ci.x, bx[0]), bx[4]), ci.y - fminf(fmaxf(ci.y, bx[1]), bx[5]),
+                         ci.z - fminf(fmaxf(ci.z, bx[2]), bx[6]), m + ci.w, grad, cost, gx, gy, gz)) {
+                atomicAdd(&W.F[3 * s], gx);
+                atomicAdd(&W.F[3 * s + 1], gy);
+                atomicAdd(&W.F[3 * s + 2], gz);
+            }
+        }
+        __syncthreads();  // W.cand is rewritten by the next round
+    }
+    // 2. sphere pairs of the link pairs in the pair set
+    for (int p0 = 0; p0 < V.NP; p0 += 64) {
+        const int pi = p0 + lane;
+        bool cand = false;
+        int y = 0;
+        if (pi < V.NP) {
+            const int2 pr = V.lpairs2[pi];
+            const float4 A = W.lk[(pr.x >> 8) & 255], B = W.lk[pr.x & 255];
+            const float dx = A.x - B.x, dy = A.y - B.y, dz = A.z - B.z;
+            const float R = m + A.w + B.w + CLR_CULL_EPS;
+            cand = dx * dx + dy * dy + dz * dz <= R * R;
+            y = pr.y;
+        }
+        const unsigned long long mk = __ballot(cand);
+        if (mk == 0ull) continue;  // wave-uniform
+        if (cand) W.cand[__popcll(mk & below)] = y;
+        __syncthreads();
+        const int n = __popcll(mk);
+        for (int c = 0; c < n; ++c) {  // one link pair per round: lane (i, j) of its 8 x 8 spheres
+            const int yc = W.cand[c];
+            const int fl = yc & 511, nl = (yc >> 9) & 15, fm = (yc >> 13) & 511, nm = (yc >> 22) & 15;
+            if ((lane >> 3) >= nl || (lane & 7) >= nm) continue;
+            const int i = fl + (lane >> 3), j = fm + (lane & 7);
+            const float4 ci = W.sph[i], cj = W.sph[j];
+            float gx, gy, gz;
+            if (clr_term(ci.x - cj.x, ci.y - cj.y, ci.z - cj.z, m + ci.w + cj.w, grad, cost, gx, gy, gz)) {
+                atomicAdd(&W.F[3 * i], gx);
+                atomicAdd(&W.F[3 * i + 1], gy);
+                atomicAdd(&W.F[3 * i + 2], gz);
+                atomicAdd(&W.F[3 * j], -gx);
+                atomicAdd(&W.F[3 * j + 1], -gy);
+                atomicAdd(&W.F[3 * j + 2], -gz);
+            }
+        }
+        __syncthreads();
+    }
+    // wave sum
+    W.red[lane] = cost;
+    __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) {
+        if (lane < off) W.red[lane] += W.red[lane + off];
+        __syncthreads();
+    }
+    const float total = W.red[0];
+    __syncthreads();
+    return total;
+}
+
 __global__ void __launch_bounds__(64) k_clearance(ArmView V, const float* q, long long n, float m,
                                                   const float* grad_cost, float* cost, float* grad_q) {
     extern __shared__ float lds_clr[];
@@ -222,7 +364,7 @@ __global__ void __launch_bounds__(64) k_clearance(ArmView V, const float* q, lon
 
 size_t clearance_lds_bytes(const ArmView& V) {
     return sizeof(float) * (2 * V.D + 12 * V.L) + 16 + sizeof(float4) * V.S + sizeof(float) * 3 * V.S +
-           sizeof(float) * 64;
+           sizeof(float) * 64 + 16 + sizeof(float4) * V.L + sizeof(int) * 64;
 }
 
 void launch_clearance(const SceneRec& sc, const float* q, long long n, float margin,

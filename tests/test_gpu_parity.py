@@ -609,11 +609,13 @@ def test_empty_and_errors(c1, B):
 
 # ---------------------------------------------------------------------------------------------
 # differentiable clearance (C4)
-def test_clearance_cost_and_grad(B):
-    _say("clearance cost vs FP64 oracle; gradient vs FP64 central FD")
+@pytest.mark.parametrize("bp", [True, False], ids=["culled", "full_pair_set"])
+def test_clearance_cost_and_grad(B, bp):
+    _say(f"clearance cost vs FP64 oracle; gradient vs FP64 central FD (link cull {bp})")
     rng = np.random.default_rng(15)
     desc = scenes.quad_arms(rng=rng)
     sc, o = B.Scene(desc), O.Scene(desc)
+    sc.set_broadphase(bp)
     Q = near_home(rng, desc, 64, sigma=0.6)
     qd = torch.from_numpy(Q.astype(np.float32)).cuda()
     cost = torch.empty(64, dtype=torch.float32, device="cuda")
