@@ -307,6 +307,30 @@ def test_c2_sparse_scene_and_masks(B):
     assert np.array_equal(gpu_edges(sc, q1, q2, 0.01), ref)
 
 
+def test_c2_environment_per_call(B):
+    _say("C2 per-call host path (MrccEnvironment): configs, edges, windows, masks")
+    rng = np.random.default_rng(21)
+    desc = scenes.planar_boxes(rng=rng, n_obstacles=5)
+    env, o = B.MrccEnvironment(desc, collision_resolution=0.01), O.Scene(desc)
+    Q = scenes.uniform_configs(rng, desc, 200)
+    for q in Q:
+        assert env.is_collision_free(q, None) == o.config_free(q)
+        assert env.is_collision_free_for_robot(["a2", "a3"], q, None) == o.config_free(q, mask=0b0110)
+    q1, q2 = scenes.lognormal_edges(rng, desc, 200, 0.01)
+    n_free = 0
+    for i in range(len(q1)):
+        got = env.is_edge_collision_free(q1[i], q2[i], None)
+        assert got == o.edge_free(q1[i], q2[i], 0.01), i
+        n_free += got
+        if i % 3 == 0:
+            N = pkg.edge_num_states(q1[i], q2[i], 0.01)
+            ns = int(rng.integers(0, N + 1))
+            nm = ns + int(rng.integers(0, 12))
+            assert env.is_edge_collision_free(q1[i], q2[i], None, N_start=ns, N_max=nm) == \
+                o.edge_free(q1[i], q2[i], 0.01, N=N, N_start=ns, N_max=nm), i
+    assert 10 <= n_free <= len(q1) - 10
+
+
 # ---------------------------------------------------------------------------------------------
 # C3: 4 x 7-DoF arms
 @pytest.fixture(scope="module")
