@@ -445,6 +445,29 @@ int build_arm_device(SceneRec* sc, const ArmHost& H) {
     sc->arm_lds_edge = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, sc->arm_nw_edge, H.nslot);
     sc->arm_lds = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, sc->arm_nw, H.nslot);
     sc->arm_lds1 = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, 1, H.nslot);
+    // the persistent kernel: the most resident waves per CU, counting both budgets: its registers
+    // allow 4 waves per SIMD (16 per CU), 3 with held boxes (12), and the LDS holds what fits in
+    // 160 KiB, one staged scene per workgroup.  gfx950 gives a workgroup up to 160 KiB, so scenes
+    // with held boxes, whose waves need the most LDS (an 8-arm scene: 14 KB per wave), take
+    // workgroups of up to MRCC_PERSISTENT_GRASP_NW waves and share one scene among more of them.
+    {
+        const int reg_waves = V.grasp ? 12 : 16;
+        const int nw_max = V.grasp ? MRCC_PERSISTENT_GRASP_NW : 4;
+        int bestw = 0;
+        for (int nw = 1; nw <= nw_max; ++nw) {
+            const size_t b = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, nw, H.nslot);
+            if (b > 163840) break;
+            const int wgs = std::min((int)(163840 / b), reg_waves / nw);
+            if (wgs * nw > bestw) {
+                bestw = wgs * nw;
+                sc->arm_nw_pers = nw;
+                sc->arm_wg_pers = wgs;
+            }
+        }
+        if (bestw == 0) return fail(MRCC_ERR_INVALID, "arm scene needs more than 160 KiB of LDS per wave");
+        sc->arm_lds_pers = arm_lds_bytes(V.D, L, na, nb, S, V.NP, V.nm, V.nclu, V.grasp, V.chain, sc->arm_nw_pers, H.nslot);
+        sc->arm_blocks_pers = sc->arm_wg_pers * device_cus();
+    }
     sc->arm_na = na;
     sc->arm_dof = dof;
     sc->arm_S = S;

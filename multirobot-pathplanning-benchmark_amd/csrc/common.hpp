@@ -16,6 +16,7 @@
 #define MRCC_MAX_ARMS 8
 #define MRCC_MAX_DOF 8
 #define MRCC_MAX_LINKS (MRCC_MAX_ARMS * MRCC_MAX_DOF)
+#define MRCC_PERSISTENT_GRASP_NW 10  // largest workgroup (waves) of the persistent kernel with held boxes
 #define MRCC_MODES_LDS_MAX 1024  // per-mode arm counts of up to this many modes live in LDS
 #define MRCC_MAX_SPHERES 512
 #define MRCC_MAX_D 64

@@ -17,6 +17,10 @@ struct SceneRec {
     size_t arm_lds;       // dynamic LDS bytes of the arm kernels (arm_nw waves per workgroup)
     size_t arm_lds1;      // dynamic LDS bytes of an arm kernel with one wave per workgroup
     int arm_nw;           // waves per workgroup of the arm kernels
+    int arm_nw_pers;      // waves per workgroup of the persistent kernel
+    int arm_wg_pers;      // its resident workgroups per CU (registers and LDS)
+    int arm_blocks_pers;  // its largest useful grid: resident workgroups on every CU
+    size_t arm_lds_pers;  // its dynamic LDS bytes
     int arm_nw_edge;      // waves of the per-call kernel's one workgroup
     size_t arm_lds_edge;  // its dynamic LDS bytes
     void* dev_mem;        // one allocation holding every array the views point to

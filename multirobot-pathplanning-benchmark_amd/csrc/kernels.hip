@@ -739,10 +739,12 @@ __device__ inline void arm_persistent_body(const ArmK& V, const ArmQueue& B, con
     arm_stats<STATS>(W.stats, rechecks, cnt);
 }
 
-// Scenes without held boxes fit 4 waves per SIMD (<= 128 VGPRs); the grasp variant 3.  DH and chain
-// joints, held boxes and statistics are template parameters.
+// Scenes without held boxes fit 4 waves per SIMD (<= 128 VGPRs), in workgroups of up to 4 waves;
+// the grasp variant 3, in workgroups of up to MRCC_PERSISTENT_GRASP_NW waves (10: 3, 3, 2 and 2 on
+// the CU's SIMDs).  DH and chain joints, held boxes and statistics are template parameters.
 template <bool GRASP, bool CHAIN, bool STATS>
-__global__ void __launch_bounds__(256, GRASP ? 3 : 4) k_arm_persistent(ArmK V, ArmQueue Q, int lgG, int spt) {
+__global__ void __launch_bounds__(GRASP ? 64 * MRCC_PERSISTENT_GRASP_NW : 256, GRASP ? 3 : 4)
+    k_arm_persistent(ArmK V, ArmQueue Q, int lgG, int spt) {
     arm_persistent_body<GRASP, CHAIN, STATS>(V, Q, Q, lgG, spt);
 }
 
@@ -1039,9 +1041,10 @@ void launch_plan_run(const SceneRec& sc, const EdgeBatchView& B, const PlanWork&
                 k_lane_persistent<PlanarView, 0, 0><<<grid, 256, 0, st>>>(sc.planar, B, W, lgG, spt);
             break;
         case KIND_ARMS: {
-            // grid counts waves; workgroups of arm_nw waves
-            const int nw = sc.arm_nw;
-            const int blocks = (grid + nw - 1) / nw;
+            // grid counts waves; workgroups of arm_nw_pers waves, no more of them than can be resident
+            const int nw = sc.arm_nw_pers;
+            int blocks = (grid + nw - 1) / nw;
+            if (blocks > sc.arm_blocks_pers) blocks = sc.arm_blocks_pers;
             const bool stats = W.stats != nullptr;
             ArmQueue Q;
             Q.q1 = B.q1;
@@ -1063,8 +1066,8 @@ void launch_plan_run(const SceneRec& sc, const EdgeBatchView& B, const PlanWork&
             Q.mask = B.mask;
 #define MRCC_ARM_PERSISTENT(G, C)                                                                     \
     do {                                                                                          \
-        if (stats) k_arm_persistent<G, C, true><<<blocks, 64 * nw, sc.arm_lds, st>>>(sc.armk, Q, lgG, spt);  \
-        else k_arm_persistent<G, C, false><<<blocks, 64 * nw, sc.arm_lds, st>>>(sc.armk, Q, lgG, spt);      \
+        if (stats) k_arm_persistent<G, C, true><<<blocks, 64 * nw, sc.arm_lds_pers, st>>>(sc.armk, Q, lgG, spt);  \
+        else k_arm_persistent<G, C, false><<<blocks, 64 * nw, sc.arm_lds_pers, st>>>(sc.armk, Q, lgG, spt);      \
     } while (0)
             if (sc.arm.chain) {
                 if (sc.arm_has_grasp) MRCC_ARM_PERSISTENT(true, true);
