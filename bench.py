@@ -959,9 +959,11 @@ def clearance_workload(args, backend, scenes, dev, threads):
     sc.set_broadphase(True)
     full_cost_diff = float((cost - cost_culled).abs().max())
     fl_fwd, pinfo = arm_flops_per_state(desc)
-    # backward: the forward's pair set again (recomputed) + per sphere and joint of its chain the
-    # cross product z x (c - o) and its dot with the force (about 18 flops; a dof/2 chain on average)
-    fl_bwd = fl_fwd + 18 * pinfo["spheres"] * desc.dof // 2
+    # backward: the forward's pair set again (recomputed), then per sphere c x f and f summed into
+    # its link (12 flops), per joint the suffix sums over its arm's links (6 flops a link, dof/2
+    # links on average) and z . (T - o x F) (15 flops)
+    L_ = desc.n_arms * desc.dof
+    fl_bwd = fl_fwd + 12 * pinfo["spheres"] + 6 * L_ * (desc.dof + 1) // 2 + 15 * L_
     ach_f = n * fl_fwd / (kms_full["forward"] * 1e-3) / 1e12
     ach_b = n * fl_bwd / (kms_full["backward"] * 1e-3) / 1e12
     ent = {"paths": P, "waypoints": P * W, "dof": D, "ms_forward_backward": fb * 1e3,

@@ -21,6 +21,7 @@ struct ClrLds {
     float* red;    // [64]
     float4* lk;    // [L] link bounding spheres (x, y, z, R)
     int* cand;     // [64] candidates of one culling round
+    float* tf;     // [L*6] per link: sum of c x f and sum of f over its spheres (backward)
 };
 
 __device__ inline ClrLds clr_lds(void* base, const ArmView& V) {
@@ -34,6 +35,7 @@ __device__ inline ClrLds clr_lds(void* base, const ArmView& V) {
     W.red = W.F + 3 * V.S;
     W.lk = (float4*)(((uintptr_t)(W.red + 64) + 15) & ~uintptr_t(15));
     W.cand = (int*)(W.lk + V.L);
+    W.tf = (float*)(W.cand + 64);
     return W;
 }
 
@@ -326,7 +328,26 @@ __global__ void __launch_bounds__(64) k_clearance(ArmView V, const float* q, lon
         } else {
             __syncthreads();
             // lane (arm a, joint k): g = z_{k-1} . sum_{s on links >= k} (c_s - o_{k-1}) x f_s
+            //   = z_{k-1} . (T_k - o_{k-1} x F_k), T_k and F_k the sums of c_s x f_s and of f_s
+            //   over the arm's links >= k: per-link sums first (lane per link), then suffix sums
             const int lane = threadIdx.x;
+            for (int l = lane; l < V.L; l += 64) {
+                float t0 = 0.f, t1 = 0.f, t2 = 0.f, f0 = 0.f, f1 = 0.f, f2 = 0.f;
+                for (int s = V.link_first[l]; s < V.link_first[l + 1]; ++s) {
+                    const float4 c = W.sph[s];
+                    const float a0 = W.F[3 * s], a1 = W.F[3 * s + 1], a2 = W.F[3 * s + 2];
+                    t0 += c.y * a2 - c.z * a1;
+                    t1 += c.z * a0 - c.x * a2;
+                    t2 += c.x * a1 - c.y * a0;
+                    f0 += a0;
+                    f1 += a1;
+                    f2 += a2;
+                }
+                float* tf = W.tf + 6 * l;
+                tf[0] = t0; tf[1] = t1; tf[2] = t2;
+                tf[3] = f0; tf[4] = f1; tf[5] = f2;
+            }
+            __syncthreads();
             if (lane < V.D) {
                 const int a = lane / V.dof, k = lane % V.dof;
                 float z0, z1, z2, o0, o1, o2;
@@ -339,20 +360,16 @@ __global__ void __launch_bounds__(64) k_clearance(ArmView V, const float* q, lon
                     z0 = P[2]; z1 = P[5]; z2 = P[8];
                     o0 = P[9]; o1 = P[10]; o2 = P[11];
                 }
-                float g = 0.f;
-                const int s0 = V.link_first[a * V.dof + k], s1 = V.link_first[(a + 1) * V.dof];
-                const bool pr = (V.prism >> lane) & 1ull;
-                for (int s = s0; s < s1; ++s) {
-                    if (pr) {  // d c / d q = z: g = z . f
-                        g += z0 * W.F[3 * s] + z1 * W.F[3 * s + 1] + z2 * W.F[3 * s + 2];
-                        continue;
-                    }
-                    const float4 cs = W.sph[s];
-                    const float w0 = cs.x - o0, w1 = cs.y - o1, w2 = cs.z - o2;
-                    const float f0 = W.F[3 * s], f1 = W.F[3 * s + 1], f2 = W.F[3 * s + 2];
-                    // z . (w x f)
-                    g += z0 * (w1 * f2 - w2 * f1) + z1 * (w2 * f0 - w0 * f2) + z2 * (w0 * f1 - w1 * f0);
+                float T0 = 0.f, T1 = 0.f, T2 = 0.f, F0 = 0.f, F1 = 0.f, F2 = 0.f;
+                for (int l = (a + 1) * V.dof - 1; l >= a * V.dof + k; --l) {
+                    const float* tf = W.tf + 6 * l;
+                    T0 += tf[0]; T1 += tf[1]; T2 += tf[2];
+                    F0 += tf[3]; F1 += tf[4]; F2 += tf[5];
                 }
+                const bool pr = (V.prism >> lane) & 1ull;  // prismatic: d c / d q = z, g = z . F
+                const float g = pr ? z0 * F0 + z1 * F1 + z2 * F2
+                                   : z0 * (T0 - (o1 * F2 - o2 * F1)) + z1 * (T1 - (o2 * F0 - o0 * F2)) +
+                                         z2 * (T2 - (o0 * F1 - o1 * F0));
                 grad_q[r * V.D + lane] = grad_cost[r] * g;
             }
         }
@@ -364,7 +381,7 @@ __global__ void __launch_bounds__(64) k_clearance(ArmView V, const float* q, lon
 
 size_t clearance_lds_bytes(const ArmView& V) {
     return sizeof(float) * (2 * V.D + 12 * V.L) + 16 + sizeof(float4) * V.S + sizeof(float) * 3 * V.S +
-           sizeof(float) * 64 + 16 + sizeof(float4) * V.L + sizeof(int) * 64;
+           sizeof(float) * 64 + 16 + sizeof(float4) * V.L + sizeof(int) * 64 + sizeof(float) * 6 * V.L;
 }
 
 void launch_clearance(const SceneRec& sc, const float* q, long long n, float margin,
