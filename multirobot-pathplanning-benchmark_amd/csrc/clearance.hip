@@ -22,6 +22,7 @@ struct ClrLds {
     float4* lk;    // [L] link bounding spheres (x, y, z, R)
     int* cand;     // [64] candidates of one culling round
     float* tf;     // [L*6] per link: sum of c x f and sum of f over its spheres (backward)
+    int* cand2;    // [64] second-level candidates (link, box) of the clustered box test
 };
 
 __device__ inline ClrLds clr_lds(void* base, const ArmView& V) {
@@ -36,6 +37,7 @@ __device__ inline ClrLds clr_lds(void* base, const ArmView& V) {
     W.lk = (float4*)(((uintptr_t)(W.red + 64) + 15) & ~uintptr_t(15));
     W.cand = (int*)(W.lk + V.L);
     W.tf = (float*)(W.cand + 64);
+    W.cand2 = (int*)(W.tf + 6 * V.L);
     return W;
 }
 
@@ -225,46 +227,94 @@ __device__ inline float clr_pairs(const ArmView& V, const ClrLds& W, float m, bo
     __syncthreads();
     float cost = 0.f;
     const unsigned long long below = (1ull << lane) - 1ull;
-    // 1. spheres vs boxes
-    const int nlb = L * V.nb;
-    for (int t0 = 0; t0 < nlb; t0 += 64) {
-        const int t = t0 + lane;
-        bool cand = false;
-        int code = 0;
-        if (t < nlb) {
-            const int l = t / V.nb, b = t - l * V.nb;
-            const float4 c = W.lk[l];
-            const float* bx = V.box32 + 8 * b;
-            const float ex = c.x - fminf(fmaxf(c.x, bx[0]), bx[4]);
-            const float ey = c.y - fminf(fmaxf(c.y, bx[1]), bx[5]);
-            const float ez = c.z - fminf(fmaxf(c.z, bx[2]), bx[6]);
-            const float R = m + c.w + CLR_CULL_EPS;
-            cand = V.link_first[l + 1] > V.link_first[l] && ex * ex + ey * ey + ez * ez <= R * R;
-            code = l | (b << 8);
+    // the spheres of (link, box) candidate `cd` (lane k of its group of 8 takes the link's sphere k)
+    auto sphere_box = [&](int cd, int k) {
+        const int l = cd & 255, b = cd >> 8;
+        const int s = V.link_first[l] + k;
+        if (s >= V.link_first[l + 1]) return;
+        const float4 ci = W.sph[s];
+        const float* bx = V.box32 + 8 * b;
+        float gx, gy, gz;
+        if (clr_term(ci.x - fminf(fmaxf(ci.x, bx[0]), bx[4]), ci.y - fminf(fmaxf(ci.y, bx[1]), bx[5]),
+                     ci.z - fminf(fmaxf(ci.z, bx[2]), bx[6]), m + ci.w, grad, cost, gx, gy, gz)) {
+            atomicAdd(&W.F[3 * s], gx);
+            atomicAdd(&W.F[3 * s + 1], gy);
+            atomicAdd(&W.F[3 * s + 2], gz);
         }
-        const unsigned long long mk = __ballot(cand);
-        if (mk == 0ull) continue;  // wave-uniform
-        if (cand) W.cand[__popcll(mk & below)] = code;
-        __syncthreads();
-        const int n = __popcll(mk);
-        for (int c0 = 0; c0 < n; c0 += 8) {  // 8 candidates x 8 spheres per round
-            const int g = lane >> 3, k = lane & 7;
-            if (c0 + g >= n) continue;
-            const int cd = W.cand[c0 + g];
-            const int l = cd & 255, b = cd >> 8;
-            const int s = V.link_first[l] + k;
-            if (s >= V.link_first[l + 1]) continue;
-            const float4 ci = W.sph[s];
-            const float* bx = V.box32 + 8 * b;
-            float gx, gy, gz;
-            if (clr_term(ci.x - fminf(fmaxf(ci.x, bx[0]), bx[4]), ci.y - fminf(fmaxf(ci.y, bx[1]), bx[5]),
-                         ci.z - fminf(fmaxf(ci.z, bx[2]), bx[6]), m + ci.w, grad, cost, gx, gy, gz)) {
-                atomicAdd(&W.F[3 * s], gx);
-                atomicAdd(&W.F[3 * s + 1], gy);
-                atomicAdd(&W.F[3 * s + 2], gz);
+    };
+    // squared distance from the link sphere's centre to a box [lo, hi]
+    auto box_e2 = [](float4 c, const float* lo, const float* hi) {
+        const float ex = c.x - fminf(fmaxf(c.x, lo[0]), hi[0]);
+        const float ey = c.y - fminf(fmaxf(c.y, lo[1]), hi[1]);
+        const float ez = c.z - fminf(fmaxf(c.z, lo[2]), hi[2]);
+        return ex * ex + ey * ey + ez * ez;
+    };
+    // 1. spheres vs boxes.  Scenes with box clusters (at most 8 boxes each): (link, cluster) pairs
+    //    lane-parallel, the boxes of the candidate clusters 8 lanes per pair, then the spheres of
+    //    the candidate (link, box) pairs 8 lanes per pair; otherwise (link, box) pairs directly.
+    if (V.nclu > 0) {
+        const int nlc = L * V.nclu;
+        for (int t0 = 0; t0 < nlc; t0 += 64) {
+            const int t = t0 + lane;
+            bool cand = false;
+            int code = 0;
+            if (t < nlc) {
+                const int l = t / V.nclu, c = t - l * V.nclu;
+                const float4 lk = W.lk[l];
+                const float R = m + lk.w + CLR_CULL_EPS;
+                cand = V.link_first[l + 1] > V.link_first[l] &&
+                       box_e2(lk, V.clu32 + 8 * c, V.clu32 + 8 * c + 4) <= R * R;
+                code = l | (c << 8);
             }
+            const unsigned long long mk = __ballot(cand);
+            if (mk == 0ull) continue;  // wave-uniform
+            if (cand) W.cand[__popcll(mk & below)] = code;
+            __syncthreads();
+            const int nA = __popcll(mk);
+            for (int a0 = 0; a0 < nA; a0 += 8) {  // 8 (link, cluster) candidates x 8 boxes
+                const int g = lane >> 3, j = lane & 7;
+                const bool ga = a0 + g < nA;
+                const int c2 = W.cand[ga ? a0 + g : a0];
+                const int l = c2 & 255, cl = c2 >> 8;
+                const int first = __float_as_int(V.clu32[8 * cl + 3]), count = __float_as_int(V.clu32[8 * cl + 7]);
+                const bool act = ga && j < count;
+                const int b = first + (act ? j : 0);
+                const float4 lk = W.lk[l];
+                const float R = m + lk.w + CLR_CULL_EPS;
+                const bool bc = act && box_e2(lk, V.box32 + 8 * b, V.box32 + 8 * b + 4) <= R * R;
+                const unsigned long long bm = __ballot(bc);
+                if (bm == 0ull) continue;  // wave-uniform
+                if (bc) W.cand2[__popcll(bm & below)] = l | (b << 8);
+                __syncthreads();
+                const int nB = __popcll(bm);
+                for (int c0 = 0; c0 < nB; c0 += 8)  // 8 (link, box) candidates x 8 spheres
+                    if (c0 + (lane >> 3) < nB) sphere_box(W.cand2[c0 + (lane >> 3)], lane & 7);
+                __syncthreads();  // W.cand2 is rewritten by the next group
+            }
+            __syncthreads();  // W.cand is rewritten by the next round
         }
-        __syncthreads();  // W.cand is rewritten by the next round
+    } else {
+        const int nlb = L * V.nb;
+        for (int t0 = 0; t0 < nlb; t0 += 64) {
+            const int t = t0 + lane;
+            bool cand = false;
+            int code = 0;
+            if (t < nlb) {
+                const int l = t / V.nb, b = t - l * V.nb;
+                const float4 lk = W.lk[l];
+                const float R = m + lk.w + CLR_CULL_EPS;
+                cand = V.link_first[l + 1] > V.link_first[l] && box_e2(lk, V.box32 + 8 * b, V.box32 + 8 * b + 4) <= R * R;
+                code = l | (b << 8);
+            }
+            const unsigned long long mk = __ballot(cand);
+            if (mk == 0ull) continue;  // wave-uniform
+            if (cand) W.cand[__popcll(mk & below)] = code;
+            __syncthreads();
+            const int n = __popcll(mk);
+            for (int c0 = 0; c0 < n; c0 += 8)  // 8 candidates x 8 spheres per round
+                if (c0 + (lane >> 3) < n) sphere_box(W.cand[c0 + (lane >> 3)], lane & 7);
+            __syncthreads();  // W.cand is rewritten by the next round
+        }
     }
     // 2. sphere pairs of the link pairs in the pair set
     for (int p0 = 0; p0 < V.NP; p0 += 64) {
@@ -381,7 +431,8 @@ __global__ void __launch_bounds__(64) k_clearance(ArmView V, const float* q, lon
 
 size_t clearance_lds_bytes(const ArmView& V) {
     return sizeof(float) * (2 * V.D + 12 * V.L) + 16 + sizeof(float4) * V.S + sizeof(float) * 3 * V.S +
-           sizeof(float) * 64 + 16 + sizeof(float4) * V.L + sizeof(int) * 64 + sizeof(float) * 6 * V.L;
+           sizeof(float) * 64 + 16 + sizeof(float4) * V.L + sizeof(int) * 64 + sizeof(float) * 6 * V.L +
+           sizeof(int) * 64;
 }
 
 void launch_clearance(const SceneRec& sc, const float* q, long long n, float margin,
