@@ -41,6 +41,12 @@ __device__ inline ClrLds clr_lds(void* base, const ArmView& V) {
     return W;
 }
 
+// value of lane (lane & ~3) + J of the lane's quad (DPP quad_perm broadcast; the quad's lanes active)
+template <int J>
+__device__ inline float quad_bcast_f32(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), J * 0x55, 0xf, 0xf, false));
+}
+
 // forward kinematics in binary32 and sphere centres into LDS
 __device__ inline void clr_fk(const ArmView& V, const ClrLds& W, const float* q) {
     const int lane = threadIdx.x;
@@ -52,41 +58,40 @@ __device__ inline void clr_fk(const ArmView& V, const ClrLds& W, const float* q)
         W.sn[lane] = pr ? 0.f : s;
     }
     __syncthreads();
-    if (lane < V.na) {
-        float X[12];
-        for (int k = 0; k < 12; ++k) X[k] = (float)V.base[12 * lane + k];
-        for (int k = 0; k < V.dof; ++k) {
-            const int l = lane * V.dof + k;
-            const float cs = W.cs[l], sn = W.sn[l];
-            float A[3][4];
-            if (V.chain) {  // A = Rz(q) C (revolute), Tz(q) C (prismatic)
-                const double* C = V.dh + 12 * l;
-                for (int c = 0; c < 4; ++c) {
-                    const float c0 = (float)C[c], c1 = (float)C[4 + c];
-                    A[0][c] = cs * c0 - sn * c1;
-                    A[1][c] = sn * c0 + cs * c1;
-                    A[2][c] = (float)C[8 + c];
+    // lane (arm a, row r, column c) carries entry (r, c) of its arm's transform through the chain,
+    // arms 5 at a time (60 lanes); the quad of a row shares the row's new entries (DPP broadcasts)
+    {
+        const int c = lane & 3, rr = (lane >> 2) % 3, ag = lane / 12;
+        for (int a0 = 0; a0 < V.na; a0 += 5) {
+            const int a = a0 + ag;
+            if (ag < 5 && a < V.na) {
+                const double* B = V.base + 12 * a;
+                float R0 = (float)B[3 * rr], R1 = (float)B[3 * rr + 1], R2 = (float)B[3 * rr + 2], t = (float)B[9 + rr];
+                for (int k = 0; k < V.dof; ++k) {
+                    const int l = a * V.dof + k;
+                    const float cs = W.cs[l], sn = W.sn[l];
+                    float A0, A1, A2;  // column c of the joint's step A (3 x 4)
+                    if (V.chain) {  // A = Rz(q) C (revolute), Tz(q) C (prismatic)
+                        const double* C = V.dh + 12 * l;
+                        const float c0 = (float)C[c], c1 = (float)C[4 + c];
+                        A0 = cs * c0 - sn * c1;
+                        A1 = sn * c0 + cs * c1;
+                        A2 = (float)C[8 + c];
+                        if (c == 3 && ((V.prism >> l) & 1ull)) A2 += q[l];
+                    } else {
+                        const double* pd = V.dh + 4 * l;
+                        const float d = (float)pd[0], al = (float)pd[1], ca = (float)pd[2], sa = (float)pd[3];
+                        A0 = c == 0 ? cs : (c == 1 ? -sn * ca : (c == 2 ? sn * sa : al * cs));
+                        A1 = c == 0 ? sn : (c == 1 ? cs * ca : (c == 2 ? -cs * sa : al * sn));
+                        A2 = c == 0 ? 0.f : (c == 1 ? sa : (c == 2 ? ca : d));
+                    }
+                    const float y = R0 * A0 + R1 * A1 + R2 * A2 + (c == 3 ? t : 0.f);
+                    W.X[12 * l + (c < 3 ? 3 * rr + c : 9 + rr)] = y;
+                    R0 = quad_bcast_f32<0>(y);
+                    R1 = quad_bcast_f32<1>(y);
+                    R2 = quad_bcast_f32<2>(y);
+                    t = quad_bcast_f32<3>(y);
                 }
-                if ((V.prism >> l) & 1ull) A[2][3] += q[l];
-            } else {
-                const double* p = V.dh + 4 * l;
-                const float d = (float)p[0], a = (float)p[1], ca = (float)p[2], sa = (float)p[3];
-                A[0][0] = cs; A[0][1] = -sn * ca; A[0][2] = sn * sa; A[0][3] = a * cs;
-                A[1][0] = sn; A[1][1] = cs * ca; A[1][2] = -cs * sa; A[1][3] = a * sn;
-                A[2][0] = 0.f; A[2][1] = sa; A[2][2] = ca; A[2][3] = d;
-            }
-            float Y[12];
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-                    Y[3 * r + c] = X[3 * r] * A[0][c] + X[3 * r + 1] * A[1][c] + X[3 * r + 2] * A[2][c];
-                Y[9 + r] = X[3 * r] * A[0][3] + X[3 * r + 1] * A[1][3] + X[3 * r + 2] * A[2][3] + X[9 + r];
-            }
-#pragma unroll
-            for (int t = 0; t < 12; ++t) {
-                X[t] = Y[t];
-                W.X[12 * l + t] = Y[t];
             }
         }
     }
