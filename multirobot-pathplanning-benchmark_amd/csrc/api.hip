@@ -944,6 +944,7 @@ int mrcc_scene_destroy(mrcc_scene* scene) {
     if (sc->stage_host) (void)hipHostFree(sc->stage_host);  // stage_dev is its device address
     if (sc->batch_dev) (void)hipFree(sc->batch_dev);
     if (sc->batch_host) (void)hipHostFree(sc->batch_host);
+    if (sc->list_host) (void)hipHostFree(sc->list_host);  // list_dev is its device address
     delete box;
     return MRCC_OK;
 }
@@ -1130,24 +1131,51 @@ static int ensure_stage(SceneRec* sc) {
 
 #define MRCC_NO_VERDICT 0xFF  // the kernels write 0 or 1
 
-// Waits for the verdict byte that the launch on `st` writes into the staging block: spins on it for
-// up to 200 us (a per-call check takes a few), then waits for the stream.  The byte is written once,
-// after every wave has read its rows, so the next call may refill the block.
-static int wait_verdict(SceneRec* sc, hipStream_t st, int32_t* result) {
-    volatile const unsigned char* p = sc->stage_out_host;
+// Spins until the byte *p, which a kernel writes into pinned host memory, differs from `unset`, for
+// up to 200 us (a per-call check takes a few), then waits for the stream (which also reports an
+// error of the work queued on it).
+static int wait_host_byte(volatile const unsigned char* p, unsigned char unset, hipStream_t st) {
     const auto t0 = std::chrono::steady_clock::now();
     for (unsigned k = 1;; ++k) {
-        const unsigned char v = *p;
-        if (v != MRCC_NO_VERDICT) {
-            *result = v ? 1 : 0;
-            return MRCC_OK;
-        }
+        if (*p != unset) return MRCC_OK;
         if ((k & 63) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(200)) break;
     }
     HIP_OK(hipStreamSynchronize(st));
-    const unsigned char v = *p;
-    if (v == MRCC_NO_VERDICT) return fail(MRCC_ERR_HIP, "the per-call kernel wrote no verdict");
-    *result = v ? 1 : 0;
+    if (*p == unset) return fail(MRCC_ERR_HIP, "the kernel wrote no result");
+    return MRCC_OK;
+}
+
+// Waits for the verdict byte that the launch on `st` writes into the staging block.  The byte is
+// written once, after every wave has read its rows, so the next call may refill the block.
+static int wait_verdict(SceneRec* sc, hipStream_t st, int32_t* result) {
+    const int rc = wait_host_byte(sc->stage_out_host, MRCC_NO_VERDICT, st);
+    if (rc != MRCC_OK) return rc;
+    *result = *(volatile const unsigned char*)sc->stage_out_host ? 1 : 0;
+    return MRCC_OK;
+}
+
+#define MRCC_HOST_LIST_ZC 64  // host edge lists up to this length use the zero-copy block
+
+// The short-list block: rows, int arrays and verdicts of up to MRCC_HOST_LIST_ZC edges, then the
+// completion flag.
+static size_t list_block_bytes() {
+    return 2 * 8 * (size_t)MRCC_HOST_LIST_ZC * MRCC_MAX_D + 4 * 4 * (size_t)MRCC_HOST_LIST_ZC +
+           MRCC_HOST_LIST_ZC + 16 + 16;
+}
+
+static int ensure_list_stage(SceneRec* sc) {
+    if (sc->list_host) return MRCC_OK;
+    void* h = nullptr;
+    HIP_OK(hipHostMalloc(&h, list_block_bytes(), hipHostMallocCoherent | hipHostMallocMapped));
+    void* d = nullptr;
+    const hipError_t e = hipHostGetDevicePointer(&d, h, 0);
+    if (e != hipSuccess) {
+        (void)hipHostFree(h);
+        return fail(MRCC_ERR_HIP, "hipHostGetDevicePointer failed: %s", hipGetErrorString(e));
+    }
+    memset(h, 0, list_block_bytes());
+    sc->list_host = (unsigned char*)h;
+    sc->list_dev = (unsigned char*)d;
     return MRCC_OK;
 }
 
@@ -1302,17 +1330,22 @@ int mrcc_check_edges_host(const mrcc_scene* scene, const double* q1, const doubl
         if (ints[j]) o += 4 * (size_t)n;
     }
     const size_t o_out = o, bytes = o + (size_t)n;
-    int rc = ensure_batch_stage(sc, bytes);
+    // short lists (a planner's candidates): the kernel reads the rows from pinned host memory and
+    // writes the verdicts there, no copy commands; longer lists: one copy each way
+    const bool zc = n <= MRCC_HOST_LIST_ZC;
+    int rc = zc ? ensure_list_stage(sc) : ensure_batch_stage(sc, bytes);
     if (rc != MRCC_OK) return rc;
-    unsigned char* h = sc->batch_host;
+    unsigned char* h = zc ? sc->list_host : sc->batch_host;
+    unsigned char* d = zc ? sc->list_dev : sc->batch_dev;
+    const size_t o_flag = bytes;
     memcpy(h, q1, rows);
     memcpy(h + rows, q2, rows);
     for (int j = 0; j < 4; ++j)
         if (ints[j]) memcpy(h + off[j], ints[j], 4 * (size_t)n);
     memset(h + o_out, 1, (size_t)n);  // verdicts start free; waves write 0 on a collision
     hipStream_t st = nullptr;
-    HIP_OK(hipMemcpyAsync(sc->batch_dev, h, bytes, hipMemcpyHostToDevice, st));
-    unsigned char* d = sc->batch_dev;
+    if (zc) *(volatile unsigned char*)(h + o_flag) = 0;
+    else HIP_OK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, st));
     EdgeBatchView v;
     memset(&v, 0, sizeof(v));
     v.Nm0 = -1;  // N_max None unless given
@@ -1333,9 +1366,19 @@ int mrcc_check_edges_host(const mrcc_scene* scene, const double* q1, const doubl
     const int wpe = n >= 2048 ? 1 : (int)std::min<int64_t>(64, std::max<int64_t>(1, 2048 / n));
     launch_direct(*sc, v, nullptr, n, nullptr, wpe, 0, 32 * device_cus(), nullptr, st);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(h + o_out, d + o_out, (size_t)n, hipMemcpyDeviceToHost, st));
-    HIP_OK(hipStreamSynchronize(st));
-    for (int64_t i = 0; i < n; ++i) result[i] = h[o_out + i] ? 1 : 0;
+    if (zc) {
+        // the flag kernel runs after every wave of the check (stream order) and its write reaches
+        // host memory after theirs
+        launch_flag(d + o_flag, st);
+        HIP_OK(hipGetLastError());
+        rc = wait_host_byte(h + o_flag, 0, st);
+        if (rc != MRCC_OK) return rc;
+    } else {
+        HIP_OK(hipMemcpyAsync(h + o_out, d + o_out, (size_t)n, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+    }
+    volatile const unsigned char* out = h + o_out;
+    for (int64_t i = 0; i < n; ++i) result[i] = out[i] ? 1 : 0;
     return MRCC_OK;
 }
 

@@ -38,6 +38,11 @@ struct SceneRec {
     unsigned char* batch_dev;
     unsigned char* batch_host;
     size_t batch_cap;
+    // block of the short host lists (at most MRCC_HOST_LIST_ZC edges): pinned coherent host memory
+    // that the kernels read and write in place; list_dev is its device address; a flag byte after
+    // the verdicts is set by a one-thread kernel after the check
+    unsigned char* list_host;
+    unsigned char* list_dev;
 };
 
 void launch_plan_build(const SceneRec& sc, const EdgeBatchView& B, const PlanWork& W, int lgG,
@@ -50,6 +55,8 @@ void launch_direct(const SceneRec& sc, const EdgeBatchView& B, const int* list, 
 void launch_fill_u8(unsigned char* p, long long n, unsigned char v, hipStream_t st);
 // plain per-call form: edge 0 of B in one workgroup (k_arm_edge / k_lane_edge)
 void launch_edge1(const SceneRec& sc, const EdgeBatchView& B, hipStream_t st);
+// one thread writes 1 to *flag (after the work queued before it on the stream)
+void launch_flag(unsigned char* flag, hipStream_t st);
 void launch_configs(const SceneRec& sc, const double* q, long long n, long long rs, long long ds,
                     const int* modes, unsigned mask, unsigned char* out, int max_grid,
                     hipStream_t st);

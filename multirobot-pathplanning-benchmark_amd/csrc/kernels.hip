@@ -1216,6 +1216,12 @@ void launch_arm_fk(const SceneRec& sc, const double* q, long long n, long long r
     k_arm_fk<<<grid_for(t, 128), 128, 0, st>>>(sc.arm, q, n, rs, ds, out);
 }
 
+__global__ void k_set_flag(unsigned char* flag) {
+    if (threadIdx.x == 0) *flag = 1;
+}
+
+void launch_flag(unsigned char* flag, hipStream_t st) { k_set_flag<<<1, 64, 0, st>>>(flag); }
+
 void launch_sincos(const double* x, long long n, double* s, double* c, hipStream_t st) {
     k_sincos<<<grid_for(n, 256), 256, 0, st>>>(x, n, s, c);
 }
