@@ -216,8 +216,10 @@ int mrcc_check_config_host(const mrcc_scene* scene, const double* q, int32_t mod
 /* A short list of edges on host memory (a planner's candidate list: find_parent / rewire
  * rrtstar_base.py:1339-1446, PRM edges prm_graph.py:663-684, EIT windows planner_eitstar.py:630-665):
  * rows q1[i*D ..], q2[i*D ..] (host double), optional per-edge host int32 arrays N (<= 0: from the
- * resolution), N_start, N_max (< 0: None), mode_ids (NULL: defaults).  One pinned staging copy
- * each way and one launch (several wavefronts per edge for short lists), synchronous.
+ * resolution), N_start, N_max (< 0: None), mode_ids (NULL: defaults).  Lists of up to 64 edges
+ * go through a pinned coherent block of the scene that the kernel reads and writes in place (the
+ * host spins on a flag written after the check); longer lists through one pinned staging copy each
+ * way.  One launch (several wavefronts per edge for short lists), synchronous.
  * result[i] = 1 when edge i is collision free (is_edge_collision_free, planning_env.py:1746-1763),
  * 0 otherwise.  N_start > N (the reference asserts) or N_start < 0 -> MRCC_ERR_INVALID. */
 int mrcc_check_edges_host(const mrcc_scene* scene, const double* q1, const double* q2, int64_t n,
