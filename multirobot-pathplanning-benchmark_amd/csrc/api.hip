@@ -1301,6 +1301,7 @@ int mrcc_check_edges_host(const mrcc_scene* scene, const double* q1, const doubl
     // given N array are replaced by the resolution's N
     std::vector<int32_t> Nfill;
     if (N) Nfill.assign(N, N + n);
+    int maxN = 2;  // the longest edge's state count (an upper bound of every window's)
     for (int64_t i = 0; i < n; ++i) {
         int Nh = N ? N[i] : 0;
         if (Nh <= 0) {
@@ -1320,6 +1321,7 @@ int mrcc_check_edges_host(const mrcc_scene* scene, const double* q1, const doubl
         if (Nh > MRCC_MAX_EDGE_STATES)
             return fail(MRCC_ERR_INVALID, "edge %lld of %d states (at most %d)", (long long)i, Nh, MRCC_MAX_EDGE_STATES);
         if (N) Nfill[i] = Nh;
+        maxN = std::max(maxN, Nh);
     }
     // block: q1 rows, q2 rows, then the int32 arrays given, then one result byte per edge
     const size_t rows = 8 * (size_t)n * D;
@@ -1362,8 +1364,10 @@ int mrcc_check_edges_host(const mrcc_scene* scene, const double* q1, const doubl
     v.incl = include_endpoints;
     v.mask = robot_mask;
     v.out = d + o_out;
-    // several waves per edge for short lists (states u, u + wpe, ... of the edge), one for long ones
-    const int wpe = n >= 2048 ? 1 : (int)std::min<int64_t>(64, std::max<int64_t>(1, 2048 / n));
+    // several waves per edge for short lists (states u, u + wpe, ... of the edge), one for long
+    // ones; no more than the longest edge has states (a wave without one would only stage the scene)
+    int wpe = n >= 2048 ? 1 : (int)std::min<int64_t>(64, std::max<int64_t>(1, 2048 / n));
+    wpe = std::max(1, std::min(wpe, maxN));
     launch_direct(*sc, v, nullptr, n, nullptr, wpe, 0, 32 * device_cus(), nullptr, st);
     HIP_OK(hipGetLastError());
     if (zc) {
