@@ -140,9 +140,12 @@ __device__ inline float disk_margin(float B, int rd) {
     return 0x1p-18f * t * t * (float)(rd > 2 ? rd : 2) * 0.5f;
 }
 
-template <int RDT, int NRT>
+// SPLIT: the state's tests split over G lanes, this lane taking the robot pairs when part == 0
+// and the spheres and rectangles part, part + G, ...; without it every test.
+template <int RDT, int NRT, bool SPLIT = false>
 __device__ inline bool disk_collides(const DiskView& S, const double* q, unsigned mask,
-                                     unsigned& rechecks) {
+                                     unsigned& rechecks, int part_ = 0, int G_ = 1) {
+    const int part = SPLIT ? part_ : 0, G = SPLIT ? G_ : 1;
     constexpr int MAXD = (RDT ? RDT : MRCC_DISK_MAX_RD) * (NRT ? NRT : MRCC_DISK_MAX_ROBOTS);
     const int rd = RDT ? RDT : S.rd;
     const int nr = NRT ? NRT : S.nr;
@@ -166,7 +169,7 @@ __device__ inline bool disk_collides(const DiskView& S, const double* q, unsigne
     const float M = disk_margin(B, rd);
 #pragma unroll
     for (int i = 0; i < (NRT ? NRT : MRCC_DISK_MAX_ROBOTS); ++i) {
-        if (i >= nr) break;
+        if (i >= nr || part != 0) break;
 #pragma unroll
         for (int j = i + 1; j < (NRT ? NRT : MRCC_DISK_MAX_ROBOTS); ++j) {
             if (j >= nr) break;
@@ -194,7 +197,7 @@ __device__ inline bool disk_collides(const DiskView& S, const double* q, unsigne
     for (int i = 0; i < (NRT ? NRT : MRCC_DISK_MAX_ROBOTS); ++i) {
         if (i >= nr) break;
         if (!((mask >> i) & 1u)) continue;
-        for (int o = 0; o < S.ns; ++o) {
+        for (int o = part; o < S.ns; o += G) {
             float d2 = 0.0f;
 #pragma unroll
             for (int k = 0; k < (RDT ? RDT : MRCC_DISK_MAX_RD); ++k) {
@@ -214,7 +217,7 @@ __device__ inline bool disk_collides(const DiskView& S, const double* q, unsigne
             }
         }
         const float T = S.rp2_32[i];
-        for (int o = 0; o < S.nrect; ++o) {
+        for (int o = part; o < S.nrect; o += G) {
             float d2 = 0.0f;
 #pragma unroll
             for (int k = 0; k < (RDT ? RDT : MRCC_DISK_MAX_RD); ++k) {
@@ -310,33 +313,50 @@ __device__ inline float obb2_sep_f32(float ax, float ay, float ac, float as, flo
 // value is off by ~2^-22 (B + H); M = 2^-18 (1 + B) keeps a factor >= 8.
 __device__ inline float planar_margin(float B) { return 0x1p-18f * (1.0f + B); }
 
-template <int NRT>
+// One planar state: 3 binary64 coordinates per robot (x, y, yaw) in q; non-finite coordinates are
+// reported as collisions.
+// SPLIT: the state's tests split over G lanes, this lane taking the robot pairs when part == 0 and
+// the obstacles part, part + G, ...; without it every test.
+// POSED: the binary32 poses (x, y, cos, sin of the yaw, each rounded from binary64) come from
+// pose[robot], computed once per robot by the caller, and bad says that a coordinate is not
+// finite; q serves the binary64 rechecks only.
+template <int NRT, bool SPLIT = false, bool POSED = false>
 __device__ inline bool planar_collides(const PlanarView& S, const double* q, unsigned mask,
-                                       unsigned& rechecks) {
+                                       unsigned& rechecks, int part_ = 0, int G_ = 1,
+                                       const float4* pose = nullptr, bool bad = false) {
     constexpr int NR_MAX = NRT ? NRT : MRCC_PLANAR_MAX_ROBOTS;
     const int nr = NRT ? NRT : S.nr;
+    const int part = SPLIT ? part_ : 0, G = SPLIT ? G_ : 1;
     float x[NR_MAX], y[NR_MAX], c[NR_MAX], s[NR_MAX], h0[NR_MAX], h1[NR_MAX];
     float B = S.Bscene;
     bool finite = true;
 #pragma unroll
     for (int i = 0; i < NR_MAX; ++i) {
         if (i >= nr) break;
-        finite = finite && __builtin_isfinite(q[3 * i]) && __builtin_isfinite(q[3 * i + 1]) &&
-                 __builtin_isfinite(q[3 * i + 2]);
-        SinCos t = sincos_f64(q[3 * i + 2]);
-        x[i] = (float)q[3 * i];
-        y[i] = (float)q[3 * i + 1];
-        c[i] = (float)t.c;
-        s[i] = (float)t.s;
+        if constexpr (POSED) {
+            const float4 p = pose[i];
+            x[i] = p.x;
+            y[i] = p.y;
+            c[i] = p.z;
+            s[i] = p.w;
+        } else {
+            finite = finite && __builtin_isfinite(q[3 * i]) && __builtin_isfinite(q[3 * i + 1]) &&
+                     __builtin_isfinite(q[3 * i + 2]);
+            SinCos t = sincos_f64(q[3 * i + 2]);
+            x[i] = (float)q[3 * i];
+            y[i] = (float)q[3 * i + 1];
+            c[i] = (float)t.c;
+            s[i] = (float)t.s;
+        }
         h0[i] = S.rh32[2 * i];
         h1[i] = S.rh32[2 * i + 1];
         B = fmaxf(B, fmaxf(fabsf(x[i]), fabsf(y[i])));
     }
-    if (!finite) return true;  // non-finite coordinates are reported as collisions
+    if (POSED ? bad : !finite) return true;
     const float M = planar_margin(B);
 #pragma unroll
     for (int i = 0; i < NR_MAX; ++i) {
-        if (i >= nr) break;
+        if (i >= nr || part != 0) break;
 #pragma unroll
         for (int j = i + 1; j < NR_MAX; ++j) {
             if (j >= nr) break;
@@ -351,7 +371,7 @@ __device__ inline bool planar_collides(const PlanarView& S, const double* q, uns
             }
         }
     }
-    for (int o = 0; o < S.no; ++o) {
+    for (int o = part; o < S.no; o += G) {
         const float* ob = S.ob32 + 8 * o;
         const float ox = ob[0], oy = ob[1], oc = ob[2], os = ob[3], oh0 = ob[6], oh1 = ob[7];
 #pragma unroll

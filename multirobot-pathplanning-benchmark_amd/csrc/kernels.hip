@@ -911,50 +911,215 @@ __global__ void __launch_bounds__(512) k_arm_edge(ArmK V, EdgeBatchView B) {
     if (threadIdx.x == 0) B.out[0] = (w.err || bad_mode || s_hit) ? 0 : 1;
 }
 
-// Plain per-call form of the lane-per-state scenes: one workgroup of 4 waves, one state per lane,
-// 256 states per round; the verdict through LDS as in k_arm_edge.
-template <class View, int T1, int T2>
-__global__ void __launch_bounds__(256) k_lane_edge(View S, EdgeBatchView B) {
+// ---- per-call lane kernels: the scene's binary32 tables staged in LDS ----------------------------
+// A per-call launch starts with cold caches, and the evaluators read their tables inside loops with
+// early exits, one dependent round trip per cache line (100 planar obstacles: 25 of them).  The
+// staged form issues every load of the tables at once, before the edge rows, and the evaluators
+// then read LDS.  The binary64 tables of the rechecks stay in global memory.
+
+#define LANE_STAGE_FLOATS 8192  // 32 KiB of LDS; larger scenes run unstaged
+
+// One table: K loads per thread issued by load(), stored by store() (the rest of a longer table
+// after them, K loads in flight per thread).
+template <int K>
+struct TabPre {
+    float v[K];
+    __device__ void load(const float* src, int n) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int e = (int)threadIdx.x + k * (int)blockDim.x;
+            v[k] = e < n ? src[e] : 0.0f;
+        }
+    }
+    __device__ void store(float* dst, const float* src, int n) const {
+        const int nt = (int)blockDim.x;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int e = (int)threadIdx.x + k * nt;
+            if (e < n) dst[e] = v[k];
+        }
+        for (int e0 = (int)threadIdx.x + K * nt; e0 < n; e0 += K * nt) {
+            float r[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const int e = e0 + k * nt;
+                r[k] = e < n ? src[e] : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (e0 + k * nt < n) dst[e0 + k * nt] = r[k];
+        }
+    }
+};
+
+__host__ __device__ inline int pad4(int n) { return (n + 3) & ~3; }
+
+template <class View> struct LaneStage;
+
+// planar scenes: obstacle boxes (8 floats each), then the robot half-extents
+template <>
+struct LaneStage<PlanarView> {
+    TabPre<4> ob;
+    TabPre<1> rh;
+    __host__ __device__ static int floats(const PlanarView& S) { return pad4(8 * S.no) + 2 * S.nr; }
+    __device__ void load(const PlanarView& S) {
+        ob.load(S.ob32, 8 * S.no);
+        rh.load(S.rh32, 2 * S.nr);
+    }
+    __device__ void store(float* t, const PlanarView& S) const {
+        ob.store(t, S.ob32, 8 * S.no);
+        rh.store(t + pad4(8 * S.no), S.rh32, 2 * S.nr);
+    }
+    __device__ static PlanarView redirect(PlanarView S, const float* t) {
+        S.rh32 = t + pad4(8 * S.no);
+        S.ob32 = t;
+        return S;
+    }
+};
+
+// disk scenes: robot-pair thresholds, sphere centres, robot-sphere thresholds, rectangle corners,
+// robot radii squared
+template <>
+struct LaneStage<DiskView> {
+    TabPre<1> rr, lo, hi, rp;
+    TabPre<2> sc, sr;
+    struct Off { int rr, sc, sr, lo, hi, rp, end; };
+    __host__ __device__ static Off offsets(const DiskView& S) {
+        Off o;
+        o.rr = 0;
+        o.sc = pad4(S.nr * S.nr);
+        o.sr = o.sc + pad4(S.ns * S.rd);
+        o.lo = o.sr + pad4(S.nr * S.ns);
+        o.hi = o.lo + pad4(S.nrect * S.rd);
+        o.rp = o.hi + pad4(S.nrect * S.rd);
+        o.end = o.rp + S.nr;
+        return o;
+    }
+    __host__ __device__ static int floats(const DiskView& S) { return offsets(S).end; }
+    __device__ void load(const DiskView& S) {
+        rr.load(S.rrT32, S.nr * S.nr);
+        sc.load(S.sc32, S.ns * S.rd);
+        sr.load(S.srT32, S.nr * S.ns);
+        lo.load(S.lo32, S.nrect * S.rd);
+        hi.load(S.hi32, S.nrect * S.rd);
+        rp.load(S.rp2_32, S.nr);
+    }
+    __device__ void store(float* t, const DiskView& S) const {
+        const Off o = offsets(S);
+        rr.store(t + o.rr, S.rrT32, S.nr * S.nr);
+        sc.store(t + o.sc, S.sc32, S.ns * S.rd);
+        sr.store(t + o.sr, S.srT32, S.nr * S.ns);
+        lo.store(t + o.lo, S.lo32, S.nrect * S.rd);
+        hi.store(t + o.hi, S.hi32, S.nrect * S.rd);
+        rp.store(t + o.rp, S.rp2_32, S.nr);
+    }
+    __device__ static DiskView redirect(DiskView S, const float* t) {
+        const Off o = offsets(S);
+        S.rrT32 = t + o.rr;
+        S.sc32 = t + o.sc;
+        S.srT32 = t + o.sr;
+        S.lo32 = t + o.lo;
+        S.hi32 = t + o.hi;
+        S.rp2_32 = t + o.rp;
+        return S;
+    }
+};
+
+// the scene's tables fit the staged kernel's LDS table
+template <class View>
+inline bool lane_stage_fits(const View& S) { return LaneStage<View>::floats(S) <= LANE_STAGE_FLOATS; }
+
+template <bool STAGED, class View>
+__device__ inline View lane_view(const View& S, const float* t) {
+    if constexpr (STAGED) return LaneStage<View>::redirect(S, t);
+    else return S;
+}
+
+// binary64 interpolation step of one coordinate, (b - a) / (N - 1), as interp_f64 forms it
+__device__ inline double interp_step_f64(double a, double b, int N) {
+#pragma clang fp contract(off)
+    return (b - a) / (double)(N - 1);
+}
+
+template <class View> __device__ inline int lane_pose_robots(const View&) { return 1; }
+template <> __device__ inline int lane_pose_robots<PlanarView>(const PlanarView& S) { return S.nr; }
+
+// Plain per-call form of the lane-per-state scenes: one workgroup of NT threads (1024 for the
+// specialised scenes), the verdict through LDS as in k_arm_edge.
+// - the interpolation steps (b - a) / (N - 1) once per coordinate, in LDS;
+// - each round takes R states; a state's obstacle tests split over G lanes (a power of two, at
+//   most 64, R G <= NT), so that a free state, which runs every test, does not set the launch's
+//   latency alone;
+// - planar scenes: before the tests, one lane per (state, robot) forms that robot's binary32 pose
+//   (binary64 interpolation and sincos) in LDS, once instead of once per lane of the state.
+template <class View, int T1, int T2, bool STAGED, int NT>
+__global__ void __launch_bounds__(NT) k_lane_edge(View S0, EdgeBatchView B) {
     constexpr int MAXD = LaneDim<View, T1, T2>::value;
+    constexpr bool PLANAR = std::is_same<View, PlanarView>::value;
     __shared__ int s_hit;
+    __shared__ float s_tab[STAGED ? LANE_STAGE_FLOATS : 1];
+    __shared__ double s_a[MAXD], s_dir[MAXD];
+    __shared__ float4 s_pose[PLANAR ? NT : 1];
+    __shared__ unsigned char s_bad[PLANAR ? NT : 1];
     if (threadIdx.x == 0) s_hit = 0;
-    __syncthreads();
-    const int D = S.D;
+    LaneStage<View> pre;
+    if constexpr (STAGED) pre.load(S0);  // every load of the tables in flight before the rows
+    const int D = S0.D;
     const int N = edge_N_dev(B, 0, D);
     const EdgeWin w = edge_window(N, batch_N_start(B, 0), batch_N_max(B, 0), B.incl);
+    const int t = (int)threadIdx.x;
+    if (t < D && t < MAXD) {
+        const double av = row1(B, 0)[t * B.ds], bv = row2(B, 0)[t * B.ds];
+        s_a[t] = av;
+        s_dir[t] = interp_step_f64(av, bv, N);
+    }
+    if constexpr (STAGED) pre.store(s_tab, S0);
+    __syncthreads();  // s_hit, the steps and the staged tables
+    const View S = lane_view<STAGED>(S0, s_tab);
     unsigned rechecks = 0;
-    if (!w.err) {
-        const double* a = row1(B, 0);
-        const double* b = row2(B, 0);
-        // the rows in registers for the specialised scenes (one round trip, also when the rows
-        // live in host memory); the generic ones read them per state
-        constexpr bool HOLD = MAXD <= 12;
-        double ra[HOLD ? MAXD : 1], rb[HOLD ? MAXD : 1];
-        if constexpr (HOLD) {
-#pragma unroll
-            for (int k = 0; k < MAXD; ++k) {
-                ra[k] = k < D ? a[k * B.ds] : 0.0;
-                rb[k] = k < D ? b[k * B.ds] : 0.0;
-            }
-        }
+    if (!w.err && w.M > 0) {
+        const int nrp = PLANAR ? (T1 ? T1 : lane_pose_robots(S0)) : 1;
+        const int R = max(1, min(w.M, NT / nrp));
+        int lgG = 0;
+        while (lgG < 6 && (R << (lgG + 1)) <= NT) ++lgG;
+        const int G = 1 << lgG, part = t & (G - 1), sl = t >> lgG;
         volatile int* hit = &s_hit;
-        for (int base = 0; base < w.M; base += blockDim.x) {
+        for (int base = 0; base < w.M; base += R) {
             if (*hit) break;  // the same LDS word for every lane
-            const int local = base + (int)threadIdx.x;
+            if constexpr (PLANAR) {
+                if (t < R * nrp) {
+                    const int ps = t / nrp, r = t - ps * nrp, local = base + ps;
+                    if (local < w.M) {
+                        const int i = edge_state_index(w, local, B.incl);
+                        if (i >= 0) {
+                            const double px = interp_dir_f64(s_a[3 * r], s_dir[3 * r], i);
+                            const double py = interp_dir_f64(s_a[3 * r + 1], s_dir[3 * r + 1], i);
+                            const double th = interp_dir_f64(s_a[3 * r + 2], s_dir[3 * r + 2], i);
+                            const SinCos sc = sincos_f64(th);
+                            s_pose[t] = make_float4((float)px, (float)py, (float)sc.c, (float)sc.s);
+                            s_bad[t] = !(__builtin_isfinite(px) && __builtin_isfinite(py) && __builtin_isfinite(th));
+                        }
+                    }
+                }
+                __syncthreads();
+            }
             bool coll = false;
-            if (local < w.M) {
+            const int local = base + sl;
+            if (sl < R && local < w.M) {
                 const int i = edge_state_index(w, local, B.incl);
                 if (i >= 0) {
                     double q[MAXD];
 #pragma unroll
-                    for (int k = 0; k < MAXD; ++k) {
-                        if constexpr (HOLD) {
-                            if (k < D) q[k] = interp_f64(ra[k], rb[k], N, i);
-                        } else {
-                            if (k < D) q[k] = interp_f64(a[k * B.ds], b[k * B.ds], N, i);
-                        }
+                    for (int k = 0; k < MAXD; ++k)
+                        if (k < D) q[k] = interp_dir_f64(s_a[k], s_dir[k], i);
+                    if constexpr (PLANAR) {
+                        bool bad = false;
+                        for (int r = 0; r < nrp; ++r) bad = bad || s_bad[sl * nrp + r];
+                        coll = planar_collides<T1, true, true>(S, q, B.mask, rechecks, part, G, s_pose + sl * nrp,
+                                                               bad);
+                    } else {
+                        coll = disk_collides<T1, T2, true>(S, q, B.mask, rechecks, part, G);
                     }
-                    coll = lane_collides<View, T1, T2>(S, q, B.mask, rechecks);
                 }
             }
             if (__any(coll)) *hit = 1;
@@ -1180,18 +1345,30 @@ void launch_configs(const SceneRec& sc, const double* q, long long n, long long 
 
 void launch_edge1(const SceneRec& sc, const EdgeBatchView& B, hipStream_t st) {
     switch (sc.kind) {
-        case KIND_DISKS:
-            if (sc.disk.rd == 2 && sc.disk.nr == 2)
-                k_lane_edge<DiskView, 2, 2><<<1, 256, 0, st>>>(sc.disk, B);
-            else
-                k_lane_edge<DiskView, 0, 0><<<1, 256, 0, st>>>(sc.disk, B);
+        case KIND_DISKS: {
+            const bool stg = lane_stage_fits(sc.disk);
+            if (sc.disk.rd == 2 && sc.disk.nr == 2) {
+                if (stg) k_lane_edge<DiskView, 2, 2, true, 1024><<<1, 1024, 0, st>>>(sc.disk, B);
+                else k_lane_edge<DiskView, 2, 2, false, 1024><<<1, 1024, 0, st>>>(sc.disk, B);
+            } else if (stg) {
+                k_lane_edge<DiskView, 0, 0, true, 256><<<1, 256, 0, st>>>(sc.disk, B);
+            } else {
+                k_lane_edge<DiskView, 0, 0, false, 256><<<1, 256, 0, st>>>(sc.disk, B);
+            }
             break;
-        case KIND_PLANAR:
-            if (sc.planar.nr == 4)
-                k_lane_edge<PlanarView, 4, 0><<<1, 256, 0, st>>>(sc.planar, B);
-            else
-                k_lane_edge<PlanarView, 0, 0><<<1, 256, 0, st>>>(sc.planar, B);
+        }
+        case KIND_PLANAR: {
+            const bool stg = lane_stage_fits(sc.planar);
+            if (sc.planar.nr == 4) {
+                if (stg) k_lane_edge<PlanarView, 4, 0, true, 1024><<<1, 1024, 0, st>>>(sc.planar, B);
+                else k_lane_edge<PlanarView, 4, 0, false, 1024><<<1, 1024, 0, st>>>(sc.planar, B);
+            } else if (stg) {
+                k_lane_edge<PlanarView, 0, 0, true, 256><<<1, 256, 0, st>>>(sc.planar, B);
+            } else {
+                k_lane_edge<PlanarView, 0, 0, false, 256><<<1, 256, 0, st>>>(sc.planar, B);
+            }
             break;
+        }
         case KIND_ARMS: {
             const int nw = sc.arm_nw_edge;
             const size_t lds = sc.arm_lds_edge;

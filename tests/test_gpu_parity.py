@@ -331,6 +331,47 @@ def test_c2_environment_per_call(B):
     assert 10 <= n_free <= len(q1) - 10
 
 
+def _lane_scene(kind, rng):
+    if kind == "disk_many_spheres":  # 2 x 3000 binary32 sphere floats + thresholds: beyond 32 KiB
+        c = rng.uniform(-2, 2, (3000, 2))
+        lim = np.array([[-2.0] * 4, [2.0] * 4])
+        return scenes.disks_scene([0.02, 0.02], 2, lim, spheres=[(c[i], 0.005) for i in range(3000)])
+    if kind == "disk3d":  # 3 robots in 3-D with spheres and boxes (the generic disk evaluator)
+        lim = np.array([[-2.0] * 9, [2.0] * 9])
+        sph = [(rng.uniform(-2, 2, 3), rng.uniform(0.1, 0.4)) for _ in range(12)]
+        rec = [(rng.uniform(-2, 2, 3), rng.uniform(0.2, 0.8, 3)) for _ in range(10)]
+        return scenes.disks_scene([0.1, 0.15, 0.2], 3, lim, spheres=sph, rects=rec)
+    if kind == "disk_c1":
+        return scenes.planar_disks(rng=rng)
+    nr, no = {"planar4_100": (4, 100), "planar4_300": (4, 300), "planar3_100": (3, 100),
+              "planar4_1100": (4, 1100)}[kind]
+    return scenes.planar_boxes(rng=rng, n_robots=nr, n_obstacles=no)
+
+
+@pytest.mark.parametrize("kind", ["planar4_100", "planar4_300", "planar3_100", "planar4_1100",
+                                  "disk_c1", "disk3d", "disk_many_spheres"])
+def test_lane_per_call_scene_sizes(B, kind):
+    """The per-call lane kernel stages the binary32 tables in LDS up to 8192 floats (planar boxes up
+    to 1019 obstacles) and reads global memory beyond; planar edges split a state's obstacles
+    over up to 64 lanes when the edge has few states.  Edge lengths from 1 to ~300 states,
+    masks, against the oracle."""
+    rng = np.random.default_rng(sum(map(ord, kind)))
+    desc = _lane_scene(kind, rng)
+    sc, o = B.Scene(desc), O.Scene(desc)
+    lo, hi = desc.limits
+    n = 400
+    q1 = rng.uniform(lo, hi, (n, desc.dim))
+    step = np.exp(rng.uniform(np.log(0.003), np.log(3.0), (n, 1)))
+    d = rng.normal(size=(n, desc.dim))
+    d /= np.max(np.abs(d), axis=1, keepdims=True)
+    q2 = np.clip(q1 + step * d, lo, hi)
+    ref = o.check_edges(q1, q2, 0.01)
+    _say(f"{kind}: per-call edges vs oracle ({ref.mean():.3f} free)")
+    assert np.array_equal(gpu_edges(sc, q1, q2, 0.01, per_call=True), ref)
+    m = 0b0101
+    assert np.array_equal(gpu_edges(sc, q1, q2, 0.01, mask=m, per_call=True), o.check_edges(q1, q2, 0.01, mask=m))
+
+
 # ---------------------------------------------------------------------------------------------
 # C3: 4 x 7-DoF arms
 @pytest.fixture(scope="module")
