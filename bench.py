@@ -142,6 +142,8 @@ def main():
                     help="only the four-UR10e mujoco-world workload (for profiling runs)")
     ap.add_argument("--per-call-only", action="store_true",
                     help="only the plain per-call form on the headline edges (for kernel traces)")
+    ap.add_argument("--lane-per-call-only", action="store_true",
+                    help="only the plain per-call form on the C1 and C2 edges (for kernel traces)")
     ap.add_argument("--headline-only", action="store_true",
                     help="only the headline batch: warmup + timed graph replays (for counter runs)")
     args = ap.parse_args()
@@ -165,6 +167,13 @@ def main():
     if args.ur10_only:
         ent = ur10_workload(args, backend, scenes, dev, len(os.sched_getaffinity(0)))
         print(json.dumps({"ur10_only": True, **(ent or {"error": "fixture missing"})}), flush=True)
+        return
+    if args.lane_per_call_only:
+        for name, desc, q1, q2, res in lane_workloads(scenes):
+            n_states, N = logical_states(q1, q2, res)
+            rate, _, m, us = per_call_rate(backend.Scene(desc), q1, q2, res, N, dev, m=args.per_call_edges)
+            print(json.dumps({"lane_per_call_only": True, "workload": name, "edges": m, "us_per_edge": us,
+                              "edge_states_per_s": rate}), flush=True)
         return
     if args.cost_only:
         ent = cost_workload(args, dev, len(os.sched_getaffinity(0)))
@@ -574,6 +583,20 @@ def lane_roofline(args, backend, sc, desc, rng, dev, res, threads, oracle_scene)
     return ent
 
 
+def lane_workloads(scenes):
+    """C1 (2 disks, 20 rectangles, 10,000 random pairs) and C2 (4 planar boxes, 100 obstacles,
+    262,144 lognormal edges), both at resolution 0.01."""
+    for name, mk, gen in (("c1_planar_disks_10k_edges", scenes.planar_disks, "pairs"),
+                          ("c2_planar_boxes_262k_edges", scenes.planar_boxes, "lognormal")):
+        rng = np.random.default_rng(1)
+        desc = mk(rng=rng)
+        if gen == "pairs":
+            q1, q2 = scenes.random_pair_edges(rng, desc, 10000)
+        else:
+            q1, q2 = scenes.lognormal_edges(rng, desc, 262144, 0.01)
+        yield name, desc, q1, q2, 0.01
+
+
 def secondary_workloads(args, backend, scenes, dev):
     """C1 (the reference's abstract environment) and C2 (planar boxes): batched GPU rate, plain
     per-call rate, all-free roofline, CPU oracle on the host cores, and for C1 the numpy
@@ -584,16 +607,9 @@ def secondary_workloads(args, backend, scenes, dev):
 
     out = {}
     threads = len(os.sched_getaffinity(0))
-    for name, mk, gen, res in (
-            ("c1_planar_disks_10k_edges", scenes.planar_disks, "pairs", 0.01),
-            ("c2_planar_boxes_262k_edges", scenes.planar_boxes, "lognormal", 0.01)):
-        rng = np.random.default_rng(1)
-        desc = mk(rng=rng)
+    for name, desc, q1, q2, res in lane_workloads(scenes):
+        gen = "pairs" if name.startswith("c1") else "lognormal"
         sc = backend.Scene(desc)
-        if gen == "pairs":
-            q1, q2 = scenes.random_pair_edges(rng, desc, 10000)
-        else:
-            q1, q2 = scenes.lognormal_edges(rng, desc, 262144, res)
         n_states, N = logical_states(q1, q2, res)
         a, b = torch.from_numpy(q1).to(dev), torch.from_numpy(q2).to(dev)
         o_ = torch.empty(len(q1), dtype=torch.uint8, device=dev)

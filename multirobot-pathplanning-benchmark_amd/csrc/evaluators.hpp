@@ -374,15 +374,29 @@ __device__ inline bool planar_collides(const PlanarView& S, const double* q, uns
     for (int o = part; o < S.no; o += G) {
         const float* ob = S.ob32 + 8 * o;
         const float ox = ob[0], oy = ob[1], oc = ob[2], os = ob[3], oh0 = ob[6], oh1 = ob[7];
+        // every robot's separation, then one branch on the least (fminf skips NaN, as v <= M
+        // does); the generic form (up to 16 robots) branches per robot, which needs fewer registers
+        float v[NR_MAX];
+        float vmin = NRT ? __builtin_inff() : -__builtin_inff();
+        if constexpr (NRT != 0) {
 #pragma unroll
-        for (int i = 0; i < NR_MAX; ++i) {
-            if (i >= nr) break;
-            if (!((mask >> i) & 1u)) continue;
-            float v = obb2_sep_f32(x[i], y[i], c[i], s[i], h0[i], h1[i], ox, oy, oc, os, oh0, oh1);
-            if (v <= M) {
-                if (v < -M) return true;
-                ++rechecks;
-                if (planar_ro_f64(S, q[3 * i], q[3 * i + 1], q[3 * i + 2], i, o)) return true;
+            for (int i = 0; i < NR_MAX; ++i) {
+                v[i] = obb2_sep_f32(x[i], y[i], c[i], s[i], h0[i], h1[i], ox, oy, oc, os, oh0, oh1);
+                if ((mask >> i) & 1u) vmin = fminf(vmin, v[i]);
+            }
+        }
+        if (vmin <= M) {
+#pragma unroll
+            for (int i = 0; i < NR_MAX; ++i) {
+                if (i >= nr) break;
+                if (!((mask >> i) & 1u)) continue;
+                if constexpr (NRT == 0)
+                    v[i] = obb2_sep_f32(x[i], y[i], c[i], s[i], h0[i], h1[i], ox, oy, oc, os, oh0, oh1);
+                if (v[i] <= M) {
+                    if (v[i] < -M) return true;
+                    ++rechecks;
+                    if (planar_ro_f64(S, q[3 * i], q[3 * i + 1], q[3 * i + 2], i, o)) return true;
+                }
             }
         }
     }
