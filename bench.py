@@ -281,9 +281,9 @@ def main():
     sc.check_edges_per_call_device(batch3)
     torch.cuda.synchronize()
     pc_time = time.perf_counter() - t0
-    per_call_rate = pc_states / pc_time
+    per_call_value = pc_states / pc_time
     per_call_match = bool(np.array_equal(out3.cpu().numpy().astype(bool), gpu_free[:m]))
-    log(f"plain per-call: {m} launches in {pc_time * 1e3:.1f} ms -> {per_call_rate:.4g} edge-states/s; "
+    log(f"plain per-call: {m} launches in {pc_time * 1e3:.1f} ms -> {per_call_value:.4g} edge-states/s; "
         f"single launch {single_rate:.4g}")
     # the same per-call form as the planners reach it: MrccEnvironment.is_edge_collision_free from
     # Python on host configurations (rows and verdict in the scene's pinned block, one launch per call)
@@ -407,7 +407,7 @@ def main():
                                         "run_split_ms": rf["f_split"],
                                         "note": "as shipped; flops are the tests it executes"}},
         "cpu_baseline": cpu,
-        "plain_per_call": {"value": per_call_rate, "unit": "edge-states/s", "edges": m, "matches": per_call_match,
+        "plain_per_call": {"value": per_call_value, "unit": "edge-states/s", "edges": m, "matches": per_call_match,
                            "us_per_edge": pc_time / m * 1e6,
                            "note": "one kernel launch per edge on device rows (mrcc_check_edges_per_call)",
                            "python_env": {"value": py_rate, "edges": mp, "us_per_call": py_time / mp * 1e6,
@@ -423,7 +423,7 @@ def main():
         "mismatches": None if mism is None else mism + sum(v.get("mismatches", 0) for v in secondary.values()),
         "checks_compared": checked + sum(v.get("checks_compared", 0) for v in secondary.values()),
         "speedup_vs_cpu": (value / cpu["value"]) if cpu else None,
-        "speedup_vs_per_call": value / per_call_rate,
+        "speedup_vs_per_call": value / per_call_value,
         "secondary": secondary,
     }
     print(json.dumps(line), flush=True)
