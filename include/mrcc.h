@@ -206,7 +206,10 @@ int mrcc_check_edge(const mrcc_scene* scene, const double* q1, const double* q2,
  * it writes the verdict (no copy commands); one kernel launch on the null stream; the call returns
  * once the verdict has arrived (spinning on it for up to 200 us, then waiting for the stream).
  * *result = 1 when collision free, 0 otherwise (is_edge_collision_free / is_collision_free of one
- * call).  Calls on one scene must not overlap (they share the staging block). */
+ * call).  Arm edges of more than two rounds of states (a round: one state per wave of the
+ * workgroup) run on several workgroups, one state per wave up to 64 states, followed by a
+ * one-thread flag kernel that the call spins on.  Calls on one scene must not overlap (they
+ * share the staging block). */
 int mrcc_check_edge_host(const mrcc_scene* scene, const double* q1, const double* q2,
                          double resolution, int32_t N, int32_t N_start, int32_t N_max,
                          int32_t include_endpoints, int32_t mode_id, uint32_t robot_mask,

@@ -1263,6 +1263,28 @@ int mrcc_check_edge_host(const mrcc_scene* scene, const double* q1, const double
     if (rc != MRCC_OK) return rc;
     const EdgeBatchView v = one_edge_view(sc, sc->stage_dev, sc->stage_dev + MRCC_MAX_D, resolution, N, N_start,
                                           N_max, include_endpoints, mode_id, robot_mask, sc->stage_out_dev);
+    // Arm edges of more than two rounds of the one-workgroup kernel (a round checks one state per
+    // wave): the same kernel on several workgroups, one state per wave for up to 64 states, the
+    // verdict byte preset to free (a workgroup writes 0 on a collision), then the flag kernel,
+    // whose write reaches host memory after the workgroups' (stream order).  A free edge then
+    // costs about one state's evaluation instead of one per round.
+    if (sc->kind == KIND_ARMS) {
+        const EdgeWin wh = edge_window(Nh, N_start, N_max < 0 ? -1 : N_max, include_endpoints);
+        if (!wh.err && wh.M > 2 * sc->arm_nw_edge) {
+            volatile unsigned char* flag = sc->stage_out_host + 8;  // the block's last 8 bytes
+            *(volatile unsigned char*)sc->stage_out_host = 1;
+            *flag = 0;
+            const int nw = sc->arm_nw_edge;
+            launch_arm_edge_spread(*sc, v, (std::min(64, wh.M) + nw - 1) / nw, st);
+            HIP_OK(hipGetLastError());
+            launch_flag(sc->stage_out_dev + 8, st);
+            HIP_OK(hipGetLastError());
+            rc = wait_host_byte(flag, 0, st);
+            if (rc != MRCC_OK) return rc;
+            *result = *(volatile const unsigned char*)sc->stage_out_host ? 1 : 0;
+            return MRCC_OK;
+        }
+    }
     launch_edge1(*sc, v, st);
     HIP_OK(hipGetLastError());
     return wait_verdict(sc, st, result);

@@ -839,7 +839,10 @@ __global__ void __launch_bounds__(256, 3) k_arm_configs(ArmK V, const double* qv
 // Plain per-call form: one launch checks edge 0 of B with one workgroup of nw waves; wave w takes
 // the edge's slots w, w + nw, ...  The waves share the verdict through LDS: a wave that finds a
 // collision raises the flag, the others stop before their next state, and thread 0 writes the
-// verdict once every wave is done, so the output needs no initialisation.  Spheres beyond the
+// verdict once every wave is done, so the output needs no initialisation.  With several
+// workgroups (long edges from the host entry point), wave w of workgroup g takes the slots
+// g nw + w, then every G nw further; the caller presets the verdict to 1, a workgroup writes 0
+// only, and completion is the caller's to observe (stream order).  Spheres beyond the
 // first 64 are placed when a candidate pair needs them (most states of a per-call stream collide
 // within the first arm).  The rows are loaded first, then the scene image; while the image is in
 // flight each wave computes the edge's state count and the joint sines and cosines of its first
@@ -870,6 +873,7 @@ __global__ void __launch_bounds__(512) k_arm_edge(ArmK V, EdgeBatchView B) {
         N = edge_N_from_max(__builtin_isnan(m0) ? m0 : mx, B.res);
     }
     const int nw = blockDim.x >> 6, wv = threadIdx.x >> 6;
+    const int g0 = (int)blockIdx.x * nw + wv, gstep = (int)gridDim.x * nw;  // the wave's first slot, stride
     const int mode = batch_mode(B, 0);
     const bool bad_mode = mode < 0 || mode >= V.nm;
     const EdgeWin w = edge_window(N, batch_N_start(B, 0), batch_N_max(B, 0), B.incl);
@@ -880,7 +884,7 @@ __global__ void __launch_bounds__(512) k_arm_edge(ArmK V, EdgeBatchView B) {
         return av + ((bv - av) / (double)(N - 1)) * (double)i;
     };
     // the wave's first state and its joint sines / cosines, before the image is stored
-    const int i0 = run && wv < w.M ? edge_state_index(w, wv, B.incl) : -1;
+    const int i0 = run && g0 < w.M ? edge_state_index(w, g0, B.incl) : -1;
     double q0 = 0.0;
     SinCos t0 = SinCos{0.0, 1.0};
     if (i0 >= 0) {  // wave-uniform
@@ -894,9 +898,9 @@ __global__ void __launch_bounds__(512) k_arm_edge(ArmK V, EdgeBatchView B) {
     ArmCountT<false> cnt;
     if (run) {
         volatile int* hit = &s_hit;
-        for (int local = wv; local < w.M; local += nw) {
+        for (int local = g0; local < w.M; local += gstep) {
             if (*hit) break;  // wave-uniform: one LDS word
-            const bool first = local == wv;
+            const bool first = local == g0;
             const int i = first ? i0 : edge_state_index(w, local, B.incl);
             if (i < 0) continue;
             const double qv = first ? q0 : state_q(i);
@@ -908,7 +912,11 @@ __global__ void __launch_bounds__(512) k_arm_edge(ArmK V, EdgeBatchView B) {
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) B.out[0] = (w.err || bad_mode || s_hit) ? 0 : 1;
+    if (threadIdx.x == 0) {
+        const bool coll = w.err || bad_mode || s_hit;
+        if (gridDim.x == 1) B.out[0] = coll ? 0 : 1;
+        else if (coll) B.out[0] = 0;
+    }
 }
 
 // ---- per-call lane kernels: the scene's binary32 tables staged in LDS ----------------------------
@@ -1341,6 +1349,21 @@ void launch_configs(const SceneRec& sc, const double* q, long long n, long long 
             break;
         }
     }
+}
+
+void launch_arm_edge_spread(const SceneRec& sc, const EdgeBatchView& B, int blocks, hipStream_t st) {
+    const int nw = sc.arm_nw_edge;
+    const size_t lds = sc.arm_lds_edge;
+#define MRCC_ARM_EDGE(G, C) k_arm_edge<G, C><<<blocks, 64 * nw, lds, st>>>(sc.armk, B)
+    if (sc.arm.chain) {
+        if (sc.arm_has_grasp) MRCC_ARM_EDGE(true, true);
+        else MRCC_ARM_EDGE(false, true);
+    } else if (sc.arm_has_grasp) {
+        MRCC_ARM_EDGE(true, false);
+    } else {
+        MRCC_ARM_EDGE(false, false);
+    }
+#undef MRCC_ARM_EDGE
 }
 
 void launch_edge1(const SceneRec& sc, const EdgeBatchView& B, hipStream_t st) {

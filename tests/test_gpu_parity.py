@@ -572,6 +572,49 @@ def test_c3_environment_per_call(c3, B):
         assert not env.is_edge_collision_free(a[hit_i], b[hit_i], None)
 
 
+def test_c3_environment_long_edges(c3, B):
+    """The one-edge host call spreads an arm edge of more than two rounds of states over several
+    workgroups (verdict preset to free, a flag kernel after the check); shorter edges and windows
+    keep the one-workgroup kernel.  Edges of 3 to ~150 states from free starts, windows on both
+    sides of the switch, a robot mask, and calls of both kinds alternating on the staging block."""
+    _say("C3 per-call host path, long edges (several workgroups) vs oracle")
+    desc, sc, o, rng = c3
+    env = B.MrccEnvironment(desc, collision_resolution=0.05)
+    s = near_home(rng, desc, 4000, sigma=0.3)
+    a = np.concatenate([s[o.check_configs(s)][:60], near_home(rng, desc, 60, sigma=0.45)])
+    d = rng.normal(size=a.shape)
+    d[:60] = 0.0  # from the free starts: one joint moves (long edges that stay free occur)
+    d[np.arange(60), rng.integers(0, desc.dim, 60)] = rng.choice([-1.0, 1.0], 60)
+    d /= np.max(np.abs(d), axis=1, keepdims=True)
+    states = np.exp(rng.uniform(np.log(2.0), np.log(150.0), (len(a), 1)))  # 3 .. 150 states
+    b = np.clip(a + states * 0.05 * d, desc.limits[0], desc.limits[1])
+    got = np.array([env.is_edge_collision_free(a[i], b[i], None) for i in range(len(a))])
+    ref = o.check_edges(a, b, 0.05)
+    assert np.array_equal(got, ref)
+    assert 10 <= ref.sum() <= len(a) - 10
+    N = np.array([pkg.edge_num_states(a[i], b[i], 0.05) for i in range(len(a))])
+    assert (N > 20).sum() > 20 and (N < 8).sum() > 5
+    for i in range(0, len(a), 3):
+        ns = int(rng.integers(0, N[i] + 1))
+        nm = ns + int(rng.choice([3, 8, 9, 40, 200]))
+        for incl in (False, True):
+            assert env.is_edge_collision_free(a[i], b[i], None, N_start=ns, N_max=nm, include_endpoints=incl) == \
+                o.edge_free(a[i], b[i], 0.05, N=int(N[i]), N_start=ns, N_max=nm, include_endpoints=incl), (i, ns, nm)
+    res = ctypes.c_int32(-1)
+    for i in range(0, len(a), 5):  # robot mask through the C entry point
+        ai, bi = np.ascontiguousarray(a[i]), np.ascontiguousarray(b[i])
+        pkg._lib.check(pkg.lib().mrcc_check_edge_host(sc.handle, ai.ctypes.data, bi.ctypes.data, ctypes.c_double(0.05),
+                                                      0, 0, -1, 0, 0, 0b0101, ctypes.byref(res)))
+        assert bool(res.value) == o.edge_free(a[i], b[i], 0.05, mask=0b0101), i
+    long_free = [i for i in range(len(a)) if ref[i] and N[i] > 20]
+    long_hit = [i for i in range(len(a)) if not ref[i] and N[i] > 20]
+    short = [i for i in range(len(a)) if N[i] < 8]
+    assert long_free and long_hit and short
+    for _ in range(10):
+        for i in (long_free[0], long_hit[0], short[0]):
+            assert env.is_edge_collision_free(a[i], b[i], None) == ref[i]
+
+
 def test_c3_masks(c3):
     _say("C3 per-arm masks")
     desc, sc, o, rng = c3
