@@ -326,6 +326,20 @@ def main():
         free_check = {"edges": kf, "states_evaluated_by_oracle": int(fn), "not_free_by_oracle": int(np.sum(~fref)),
                       "seconds": time.perf_counter() - t0}
         log(f"all-free set vs oracle: {free_check}")
+        # the planners' per-call form on long free edges (every state evaluated): the oracle-free
+        # edges of this set with more than 20 states
+        _, fN8 = logical_states(rf["fq1"][:kf], rf["fq2"][:kf], res)
+        sel = np.where(fref & (fN8 > 20))[0][:300]
+        if len(sel):
+            env.is_edge_collision_free(rf["fq1"][sel[0]], rf["fq2"][sel[0]], None)
+            t0 = time.perf_counter()
+            got = [env.is_edge_collision_free(rf["fq1"][i], rf["fq2"][i], None) for i in sel]
+            dt = time.perf_counter() - t0
+            free_check["python_env_long_free"] = {
+                "edges": int(len(sel)), "mean_N": float(fN8[sel].mean()), "us_per_call": dt / len(sel) * 1e6,
+                "matches": bool(all(got)),
+                "note": "MrccEnvironment.is_edge_collision_free on oracle-free edges of more than 20 states"}
+            log(f"per-call on long free edges: {free_check['python_env_long_free']}")
 
     # ---- CPU baseline: the oracle on the host cores -------------------------------------------
     cpu = None
