@@ -83,8 +83,8 @@ SIGNATURES = {
     "mrcc_check_edge_host": (_c.c_int, [_vp, _vp, _vp, _c.c_double, _i32, _i32, _i32, _i32, _i32, _u32,
                                         _c.POINTER(_i32)]),
     "mrcc_check_config_host": (_c.c_int, [_vp, _vp, _i32, _u32, _c.POINTER(_i32)]),
-    "mrcc_check_edges_host": (_c.c_int, [_vp, _dp, _dp, _i64, _c.c_double, _ip, _ip, _ip, _i32, _ip, _u32,
-                                         _u8p]),
+    "mrcc_check_edges_host": (_c.c_int, [_vp, _vp, _vp, _i64, _c.c_double, _vp, _vp, _vp, _i32, _vp, _u32,
+                                         _vp]),
     "mrcc_check_edges_per_call": (_c.c_int, [_vp, _c.POINTER(EdgeBatch), _vp]),
     "mrcc_check_edges_noalloc": (_c.c_int, [_vp, _c.POINTER(EdgeBatch), _c.POINTER(Tune), _vp, _vp]),
     "mrcc_bfs_indices": (_c.c_int, [_i32, _ip]),

@@ -296,6 +296,29 @@ class MrccEnvironment(BaseProblem):
             pass
         return mid
 
+    def _rows(self, qs) -> np.ndarray:
+        """Configurations (an [n, D] array or a sequence of configurations) as contiguous float64
+        rows; np.array over the rows (np.stack costs several times more for short lists)."""
+        if isinstance(qs, np.ndarray):
+            return np.ascontiguousarray(qs, dtype=np.float64)
+        if len(qs) == 0:
+            return np.zeros((0, self.scene.D))
+        return np.array([self._flat(q) for q in qs], dtype=np.float64)
+
+    def _mode_ids(self, modes) -> np.ndarray:
+        """int32 mode_id of each entry, once per distinct object (a candidate list repeats one mode)."""
+        n = len(modes)
+        if n and all(m is modes[0] for m in modes):
+            return np.full(n, self.mode_id(modes[0]), dtype=np.int32)
+        seen = {}
+        out = np.empty(n, dtype=np.int32)
+        for k, m in enumerate(modes):
+            v = seen.get(id(m))
+            if v is None:
+                v = seen[id(m)] = self.mode_id(m)
+            out[k] = v
+        return out
+
     def mode_id(self, mode) -> int:
         """Device mode id of a mode: an int as is; in a manipulation problem, the device mode of
         the Mode's scene graph; otherwise a Mode by its ``mrcc_mode_id``.  A Mode without one is
@@ -400,8 +423,7 @@ class MrccEnvironment(BaseProblem):
     # ---- batched ---------------------------------------------------------------------------
     def batch_is_collision_free(self, qs, modes=None, robots=None) -> np.ndarray:
         torch = _torch()
-        Q = np.ascontiguousarray(np.stack([self._flat(q) for q in qs]) if not isinstance(qs, np.ndarray) else qs,
-                                 dtype=np.float64)
+        Q = self._rows(qs)
         if Q.ndim != 2 or Q.shape[1] != self.scene.D:
             raise ValueError(f"configurations: need [n, {self.scene.D}], got {Q.shape}")
         n = Q.shape[0]
@@ -413,7 +435,7 @@ class MrccEnvironment(BaseProblem):
         out = torch.empty(n, dtype=torch.uint8, device="cuda")
         md = None
         if modes is not None:
-            md = torch.from_numpy(np.asarray([self.mode_id(m) for m in modes], dtype=np.int32)).cuda()
+            md = torch.from_numpy(np.asarray(self._mode_ids(modes), dtype=np.int32)).cuda()
         mask = ALL_ROBOTS if robots is None else self.robot_mask(robots)
         self.scene.check_configs_device(qd, out, md, mask)
         return out.cpu().numpy().astype(bool)
@@ -442,10 +464,7 @@ class MrccEnvironment(BaseProblem):
         torch = _torch()
         if resolution is None:
             resolution = self.collision_resolution
-        A = np.ascontiguousarray(q1s if isinstance(q1s, np.ndarray) else np.stack([self._flat(q) for q in q1s]),
-                                 dtype=np.float64)
-        B = np.ascontiguousarray(q2s if isinstance(q2s, np.ndarray) else np.stack([self._flat(q) for q in q2s]),
-                                 dtype=np.float64)
+        A, B = self._rows(q1s), self._rows(q2s)
         if A.ndim != 2 or A.shape != B.shape or A.shape[1] != self.scene.D:
             raise ValueError(f"q1s, q2s: need two [n, {self.scene.D}] arrays, got {A.shape} and {B.shape}")
         n = A.shape[0]
@@ -467,17 +486,18 @@ class MrccEnvironment(BaseProblem):
             hv = lambda v: None if v is None else np.ascontiguousarray(np.broadcast_to(v, (n,)), dtype=np.int32)
             Nh, Sh, Mh = hv(N), hv(N_start), hv(N_max)
             Mh = None if Mh is None else np.where(Mh < 0, -1, Mh).astype(np.int32)
-            mh = None if modes is None else np.asarray([self.mode_id(m) for m in modes], dtype=np.int32)
+            mh = None if modes is None else np.asarray(self._mode_ids(modes), dtype=np.int32)
             res = np.empty(n, dtype=np.uint8)
-            p = lambda v: None if v is None else iptr(v)
-            check(lib().mrcc_check_edges_host(self.scene.handle, dptr(A), dptr(B), n, float(resolution), p(Nh),
-                                              p(Sh), p(Mh), int(bool(include_endpoints)), p(mh), mask,
-                                              res.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+            # plain addresses (c_void_p): ctypes.cast, which data_as uses, costs microseconds a call
+            p = lambda v: None if v is None else v.ctypes.data
+            check(lib().mrcc_check_edges_host(self.scene.handle, A.ctypes.data, B.ctypes.data, n, float(resolution),
+                                              p(Nh), p(Sh), p(Mh), int(bool(include_endpoints)), p(mh), mask,
+                                              res.ctypes.data))
             return res.astype(bool)
         a, b = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
         out = torch.empty(n, dtype=torch.uint8, device="cuda")
         Nt, St, Mt = iv(N), iv(N_start), iv(N_max)  # N_max entries < 0 mean None
-        md = iv(None if modes is None else [self.mode_id(m) for m in modes])
+        md = iv(None if modes is None else self._mode_ids(modes))
         batch = self.scene.edge_batch(a, b, out, resolution, Nt, St, Mt, include_endpoints, md, mask)
         self.scene.check_edges_device(batch, tune)
         return out.cpu().numpy().astype(bool)

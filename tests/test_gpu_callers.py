@@ -121,19 +121,17 @@ def test_sparse_checks_match_loop():
 # ---------------------------------------------------------------------------------------------
 # mrcc_check_edges_host (short lists on host memory) against the oracle
 def _host_edges(sc, q1, q2, res, N=None, N_start=None, N_max=None, incl=False, modes=None, mask=0xFFFFFFFF):
-    import ctypes
     L = pkg.lib()
     n = len(q1)
     q1 = np.ascontiguousarray(q1, dtype=np.float64)
     q2 = np.ascontiguousarray(q2, dtype=np.float64)
     iv = lambda v: None if v is None else np.ascontiguousarray(v, dtype=np.int32)
     arrs = [iv(N), iv(N_start), iv(N_max), iv(modes)]
-    ptr = lambda v: None if v is None else v.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    ptr = lambda v: None if v is None else v.ctypes.data
     out = np.full(n, 7, dtype=np.uint8)
-    rc = L.mrcc_check_edges_host(sc.handle, q1.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                 q2.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n, float(res),
+    rc = L.mrcc_check_edges_host(sc.handle, q1.ctypes.data, q2.ctypes.data, n, float(res),
                                  ptr(arrs[0]), ptr(arrs[1]), ptr(arrs[2]), int(incl), ptr(arrs[3]), mask,
-                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+                                 out.ctypes.data)
     return rc, out
 
 
