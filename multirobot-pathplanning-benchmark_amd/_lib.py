@@ -93,7 +93,7 @@ SIGNATURES = {
     "mrcc_sincos": (_c.c_int, [_vp, _i64, _vp, _vp, _vp]),
     "mrcc_clearance_forward": (_c.c_int, [_vp, _vp, _i64, _i32, _c.c_float, _vp, _vp]),
     "mrcc_clearance_backward": (_c.c_int, [_vp, _vp, _i64, _i32, _c.c_float, _vp, _vp, _vp]),
-    "mrcc_config_cost": (_c.c_int, [_vp, _i64, _vp, _i64, _i64, _i32, _ip, _i32, _i32, _i32, _c.c_double,
+    "mrcc_config_cost": (_c.c_int, [_vp, _i64, _vp, _i64, _i64, _i32, _vp, _i32, _i32, _i32, _c.c_double,
                                     _vp, _vp]),
 }
 

@@ -148,7 +148,7 @@ def _config_cost(P, Q, slices, metric: int, reduction: int, w: float):
     if n:
         _lib.check(_lib.lib().mrcc_config_cost(
             ctypes.c_void_p(Pd.data_ptr()), ps, ctypes.c_void_p(Qd.data_ptr()), Qd.stride(0), n, D,
-            sl.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(sl) // 2, metric, reduction, float(w),
+            sl.ctypes.data, len(sl) // 2, metric, reduction, float(w),
             ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
     return out if on_device else out.cpu().numpy()
 
