@@ -336,6 +336,14 @@ def _lane_scene(kind, rng):
         c = rng.uniform(-2, 2, (3000, 2))
         lim = np.array([[-2.0] * 4, [2.0] * 4])
         return scenes.disks_scene([0.02, 0.02], 2, lim, spheres=[(c[i], 0.005) for i in range(3000)])
+    if kind == "disk2_1500":  # 6,000 floats of sphere tables: staged, through the tail loop
+        c = rng.uniform(-2, 2, (1500, 2))
+        lim = np.array([[-2.0] * 4, [2.0] * 4])
+        return scenes.disks_scene([0.03, 0.03], 2, lim, spheres=[(c[i], 0.01) for i in range(1500)])
+    if kind == "disk3d_300":  # generic evaluator, 1,800 floats of sphere tables at 256 threads
+        lim = np.array([[-2.0] * 9, [2.0] * 9])
+        sph = [(rng.uniform(-2, 2, 3), rng.uniform(0.02, 0.08)) for _ in range(300)]
+        return scenes.disks_scene([0.1, 0.15, 0.2], 3, lim, spheres=sph)
     if kind == "disk3d":  # 3 robots in 3-D with spheres and boxes (the generic disk evaluator)
         lim = np.array([[-2.0] * 9, [2.0] * 9])
         sph = [(rng.uniform(-2, 2, 3), rng.uniform(0.1, 0.4)) for _ in range(12)]
@@ -344,17 +352,19 @@ def _lane_scene(kind, rng):
     if kind == "disk_c1":
         return scenes.planar_disks(rng=rng)
     nr, no = {"planar4_100": (4, 100), "planar4_300": (4, 300), "planar3_100": (3, 100),
-              "planar4_1100": (4, 1100)}[kind]
+              "planar4_700": (4, 700), "planar3_300": (3, 300), "planar4_1100": (4, 1100)}[kind]
     return scenes.planar_boxes(rng=rng, n_robots=nr, n_obstacles=no)
 
 
-@pytest.mark.parametrize("kind", ["planar4_100", "planar4_300", "planar3_100", "planar4_1100",
-                                  "disk_c1", "disk3d", "disk_many_spheres"])
+@pytest.mark.parametrize("kind", ["planar4_100", "planar4_300", "planar3_100", "planar4_700", "planar3_300",
+                                  "planar4_1100", "disk_c1", "disk3d", "disk2_1500", "disk3d_300",
+                                  "disk_many_spheres"])
 def test_lane_per_call_scene_sizes(B, kind):
     """The per-call lane kernel stages the binary32 tables in LDS up to 8192 floats (planar boxes up
-    to 1019 obstacles) and reads global memory beyond; planar edges split a state's obstacles
-    over up to 64 lanes when the edge has few states.  Edge lengths from 1 to ~300 states,
-    masks, against the oracle."""
+    to 1019 obstacles) and reads global memory beyond; four loads per thread are issued first and
+    longer tables finish in a tail loop (planar4_700 at 1024 threads, planar3_300 at 256); planar
+    edges split a state's obstacles over up to 64 lanes when the edge has few states.  Edge
+    lengths from 1 to ~300 states, masks, against the oracle."""
     rng = np.random.default_rng(sum(map(ord, kind)))
     desc = _lane_scene(kind, rng)
     sc, o = B.Scene(desc), O.Scene(desc)
