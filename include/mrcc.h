@@ -208,7 +208,8 @@ int mrcc_check_edge(const mrcc_scene* scene, const double* q1, const double* q2,
  * *result = 1 when collision free, 0 otherwise (is_edge_collision_free / is_collision_free of one
  * call).  Arm edges of more than two rounds of states (a round: one state per wave of the
  * workgroup) run on several workgroups, one state per wave up to 64 states, followed by a
- * one-thread flag kernel that the call spins on.  Calls on one scene must not overlap (they
+ * one-thread flag kernel that the call spins on.  A configuration of a disk or planar scene is
+ * checked by one workgroup whose lanes split its obstacle tests.  Calls on one scene must not overlap (they
  * share the staging block). */
 int mrcc_check_edge_host(const mrcc_scene* scene, const double* q1, const double* q2,
                          double resolution, int32_t N, int32_t N_start, int32_t N_max,

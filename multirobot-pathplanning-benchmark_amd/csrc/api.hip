@@ -1418,6 +1418,15 @@ int mrcc_check_config_host(const mrcc_scene* scene, const double* q, int32_t mod
     hipStream_t st = nullptr;
     rc = stage_rows(sc, q, nullptr);
     if (rc != MRCC_OK) return rc;
+    if (sc->kind == KIND_DISKS || sc->kind == KIND_PLANAR) {
+        // one workgroup whose lanes split the configuration's obstacle tests (one lane walking
+        // every obstacle of a free configuration set the call's latency)
+        const EdgeBatchView v = one_edge_view(sc, sc->stage_dev, sc->stage_dev, 1.0, 0, 0, -1, 0, mode_id,
+                                              robot_mask, sc->stage_out_dev);
+        launch_config1(*sc, v, st);
+        HIP_OK(hipGetLastError());
+        return wait_verdict(sc, st, result);
+    }
     // the mode id travels in the staging block's int area (after the two rows)
     ((int32_t*)(sc->stage_host + 2 * MRCC_MAX_D))[0] = mode_id;
     const int32_t* md = (const int32_t*)(sc->stage_dev + 2 * MRCC_MAX_D);

@@ -57,6 +57,9 @@ void launch_fill_u8(unsigned char* p, long long n, unsigned char v, hipStream_t 
 void launch_edge1(const SceneRec& sc, const EdgeBatchView& B, hipStream_t st);
 // one thread writes 1 to *flag (after the work queued before it on the stream)
 void launch_flag(unsigned char* flag, hipStream_t st);
+// one configuration (row q1 of B) of a disk or planar scene: the per-call lane kernel, obstacles
+// split over lanes
+void launch_config1(const SceneRec& sc, const EdgeBatchView& B, hipStream_t st);
 // k_arm_edge on `blocks` workgroups (blocks > 1: the verdict must be preset to 1)
 void launch_arm_edge_spread(const SceneRec& sc, const EdgeBatchView& B, int blocks, hipStream_t st);
 void launch_configs(const SceneRec& sc, const double* q, long long n, long long rs, long long ds,

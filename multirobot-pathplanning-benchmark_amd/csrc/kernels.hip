@@ -1060,7 +1060,9 @@ template <> __device__ inline int lane_pose_robots<PlanarView>(const PlanarView&
 //   latency alone;
 // - planar scenes: before the tests, one lane per (state, robot) forms that robot's binary32 pose
 //   (binary64 interpolation and sincos) in LDS, once instead of once per lane of the state.
-template <class View, int T1, int T2, bool STAGED, int NT>
+// CONFIG: the row q1 itself is the one state to check (the one-configuration host call), taken as
+// is, with no interpolation.
+template <class View, int T1, int T2, bool STAGED, int NT, bool CONFIG = false>
 __global__ void __launch_bounds__(NT) k_lane_edge(View S0, EdgeBatchView B) {
     constexpr int MAXD = LaneDim<View, T1, T2>::value;
     constexpr bool PLANAR = std::is_same<View, PlanarView>::value;
@@ -1073,13 +1075,14 @@ __global__ void __launch_bounds__(NT) k_lane_edge(View S0, EdgeBatchView B) {
     LaneStage<View> pre;
     if constexpr (STAGED) pre.load(S0);  // every load of the tables in flight before the rows
     const int D = S0.D;
-    const int N = edge_N_dev(B, 0, D);
-    const EdgeWin w = edge_window(N, batch_N_start(B, 0), batch_N_max(B, 0), B.incl);
+    const int N = CONFIG ? 3 : edge_N_dev(B, 0, D);
+    const EdgeWin w = CONFIG ? edge_window(3, 0, -1, 0)  // one slot, state 1
+                             : edge_window(N, batch_N_start(B, 0), batch_N_max(B, 0), B.incl);
     const int t = (int)threadIdx.x;
     if (t < D && t < MAXD) {
-        const double av = row1(B, 0)[t * B.ds], bv = row2(B, 0)[t * B.ds];
+        const double av = row1(B, 0)[t * B.ds];
         s_a[t] = av;
-        s_dir[t] = interp_step_f64(av, bv, N);
+        s_dir[t] = CONFIG ? 0.0 : interp_step_f64(av, row2(B, 0)[t * B.ds], N);
     }
     if constexpr (STAGED) pre.store(s_tab, S0);
     __syncthreads();  // s_hit, the steps and the staged tables
@@ -1100,9 +1103,9 @@ __global__ void __launch_bounds__(NT) k_lane_edge(View S0, EdgeBatchView B) {
                     if (local < w.M) {
                         const int i = edge_state_index(w, local, B.incl);
                         if (i >= 0) {
-                            const double px = interp_dir_f64(s_a[3 * r], s_dir[3 * r], i);
-                            const double py = interp_dir_f64(s_a[3 * r + 1], s_dir[3 * r + 1], i);
-                            const double th = interp_dir_f64(s_a[3 * r + 2], s_dir[3 * r + 2], i);
+                            const double px = CONFIG ? s_a[3 * r] : interp_dir_f64(s_a[3 * r], s_dir[3 * r], i);
+                            const double py = CONFIG ? s_a[3 * r + 1] : interp_dir_f64(s_a[3 * r + 1], s_dir[3 * r + 1], i);
+                            const double th = CONFIG ? s_a[3 * r + 2] : interp_dir_f64(s_a[3 * r + 2], s_dir[3 * r + 2], i);
                             const SinCos sc = sincos_f64(th);
                             s_pose[t] = make_float4((float)px, (float)py, (float)sc.c, (float)sc.s);
                             s_bad[t] = !(__builtin_isfinite(px) && __builtin_isfinite(py) && __builtin_isfinite(th));
@@ -1119,7 +1122,7 @@ __global__ void __launch_bounds__(NT) k_lane_edge(View S0, EdgeBatchView B) {
                     double q[MAXD];
 #pragma unroll
                     for (int k = 0; k < MAXD; ++k)
-                        if (k < D) q[k] = interp_dir_f64(s_a[k], s_dir[k], i);
+                        if (k < D) q[k] = CONFIG ? s_a[k] : interp_dir_f64(s_a[k], s_dir[k], i);
                     if constexpr (PLANAR) {
                         bool bad = false;
                         for (int r = 0; r < nrp; ++r) bad = bad || s_bad[sl * nrp + r];
@@ -1364,6 +1367,31 @@ void launch_arm_edge_spread(const SceneRec& sc, const EdgeBatchView& B, int bloc
         MRCC_ARM_EDGE(false, false);
     }
 #undef MRCC_ARM_EDGE
+}
+
+// one configuration (row q1 of B, edge 0) of a disk or planar scene with the per-call lane kernel
+void launch_config1(const SceneRec& sc, const EdgeBatchView& B, hipStream_t st) {
+    if (sc.kind == KIND_DISKS) {
+        const bool stg = lane_stage_fits(sc.disk);
+        if (sc.disk.rd == 2 && sc.disk.nr == 2) {
+            if (stg) k_lane_edge<DiskView, 2, 2, true, 1024, true><<<1, 1024, 0, st>>>(sc.disk, B);
+            else k_lane_edge<DiskView, 2, 2, false, 1024, true><<<1, 1024, 0, st>>>(sc.disk, B);
+        } else if (stg) {
+            k_lane_edge<DiskView, 0, 0, true, 256, true><<<1, 256, 0, st>>>(sc.disk, B);
+        } else {
+            k_lane_edge<DiskView, 0, 0, false, 256, true><<<1, 256, 0, st>>>(sc.disk, B);
+        }
+    } else if (sc.kind == KIND_PLANAR) {
+        const bool stg = lane_stage_fits(sc.planar);
+        if (sc.planar.nr == 4) {
+            if (stg) k_lane_edge<PlanarView, 4, 0, true, 1024, true><<<1, 1024, 0, st>>>(sc.planar, B);
+            else k_lane_edge<PlanarView, 4, 0, false, 1024, true><<<1, 1024, 0, st>>>(sc.planar, B);
+        } else if (stg) {
+            k_lane_edge<PlanarView, 0, 0, true, 256, true><<<1, 256, 0, st>>>(sc.planar, B);
+        } else {
+            k_lane_edge<PlanarView, 0, 0, false, 256, true><<<1, 256, 0, st>>>(sc.planar, B);
+        }
+    }
 }
 
 void launch_edge1(const SceneRec& sc, const EdgeBatchView& B, hipStream_t st) {

@@ -380,6 +380,20 @@ def test_lane_per_call_scene_sizes(B, kind):
     assert np.array_equal(gpu_edges(sc, q1, q2, 0.01, per_call=True), ref)
     m = 0b0101
     assert np.array_equal(gpu_edges(sc, q1, q2, 0.01, mask=m, per_call=True), o.check_edges(q1, q2, 0.01, mask=m))
+    # the one-configuration host call runs the same kernel on the row itself (no interpolation):
+    # free and colliding configurations, -0.0 coordinates, non-finite values, masks
+    Q = np.concatenate([q1[:150], 0.5 * (q1[:150] + q2[:150])])
+    Q[::7, 0] = -0.0
+    Q[3, 1] = np.nan
+    Q[5, -1] = np.inf
+    res = ctypes.c_int32(-1)
+    for mask in (0xFFFFFFFF, m):
+        got = []
+        for q in Q:
+            qc = np.ascontiguousarray(q)
+            pkg._lib.check(pkg.lib().mrcc_check_config_host(sc.handle, qc.ctypes.data, 0, mask, ctypes.byref(res)))
+            got.append(bool(res.value))
+        assert np.array_equal(np.array(got), o.check_configs(Q, mask=mask)), mask
 
 
 # ---------------------------------------------------------------------------------------------
