@@ -1066,7 +1066,7 @@ template <class View, int T1, int T2, bool STAGED, int NT, bool CONFIG = false>
 __global__ void __launch_bounds__(NT) k_lane_edge(View S0, EdgeBatchView B) {
     constexpr int MAXD = LaneDim<View, T1, T2>::value;
     constexpr bool PLANAR = std::is_same<View, PlanarView>::value;
-    __shared__ int s_hit;
+    __shared__ int s_hit, s_N;
     __shared__ float s_tab[STAGED ? LANE_STAGE_FLOATS : 1];
     __shared__ double s_a[MAXD], s_dir[MAXD];
     __shared__ float4 s_pose[PLANAR ? NT : 1];
@@ -1074,18 +1074,36 @@ __global__ void __launch_bounds__(NT) k_lane_edge(View S0, EdgeBatchView B) {
     if (threadIdx.x == 0) s_hit = 0;
     LaneStage<View> pre;
     if constexpr (STAGED) pre.load(S0);  // every load of the tables in flight before the rows
-    const int D = S0.D;
-    const int N = CONFIG ? 3 : edge_N_dev(B, 0, D);
-    const EdgeWin w = CONFIG ? edge_window(3, 0, -1, 0)  // one slot, state 1
-                             : edge_window(N, batch_N_start(B, 0), batch_N_max(B, 0), B.incl);
+    const int D = S0.D;  // <= MAXD <= 48: one wave holds every coordinate
     const int t = (int)threadIdx.x;
-    if (t < D && t < MAXD) {
-        const double av = row1(B, 0)[t * B.ds];
-        s_a[t] = av;
-        s_dir[t] = CONFIG ? 0.0 : interp_step_f64(av, row2(B, 0)[t * B.ds], N);
+    if (t < 64) {
+        // wave 0 reads the rows once (lane k: coordinate k; one round trip, also when the rows
+        // live in host memory), forms the state count by edge_N_dev's rule (a NaN in coordinate 0
+        // sticks, a NaN elsewhere is skipped) and the steps
+        const double av = t < D ? row1(B, 0)[t * B.ds] : 0.0;
+        const double bv = (!CONFIG && t < D) ? row2(B, 0)[t * B.ds] : 0.0;
+        int N = 3;
+        if constexpr (!CONFIG) {
+            if (B.N || B.N0 > 0) {
+                N = B.N ? B.N[0] : B.N0;
+            } else {
+                const double m = t < D ? __builtin_fabs(av - bv) : 0.0;
+                const double m0 = readlane_f64(m, 0);
+                const double mx = wave_max_f64(__builtin_isnan(m) ? 0.0 : m);
+                N = edge_N_from_max(__builtin_isnan(m0) ? m0 : mx, B.res);
+            }
+        }
+        if (t < D && t < MAXD) {
+            s_a[t] = av;
+            s_dir[t] = CONFIG ? 0.0 : interp_step_f64(av, bv, N);
+        }
+        if (t == 0) s_N = N;
     }
     if constexpr (STAGED) pre.store(s_tab, S0);
-    __syncthreads();  // s_hit, the steps and the staged tables
+    __syncthreads();  // s_hit, the state count, the steps and the staged tables
+    const int N = s_N;
+    const EdgeWin w = CONFIG ? edge_window(3, 0, -1, 0)  // one slot, state 1
+                             : edge_window(N, batch_N_start(B, 0), batch_N_max(B, 0), B.incl);
     const View S = lane_view<STAGED>(S0, s_tab);
     unsigned rechecks = 0;
     if (!w.err && w.M > 0) {
@@ -1369,57 +1387,60 @@ void launch_arm_edge_spread(const SceneRec& sc, const EdgeBatchView& B, int bloc
 #undef MRCC_ARM_EDGE
 }
 
-// one configuration (row q1 of B, edge 0) of a disk or planar scene with the per-call lane kernel
-void launch_config1(const SceneRec& sc, const EdgeBatchView& B, hipStream_t st) {
+// The per-call lane kernel for edge 0 of B (CONFIG: row q1 itself as the one state).  Scenes of
+// at most LANE_SMALL_SCENE obstacles run one 256-thread workgroup: their states are quick, and
+// 1024 threads (16 waves to start, larger barriers) cost more than the split saves there.
+#define LANE_SMALL_SCENE 8
+
+template <bool CONFIG>
+static void launch_lane1(const SceneRec& sc, const EdgeBatchView& B, hipStream_t st) {
+#define MRCC_LANE1(VIEW, T1, T2, STG, NT, S) k_lane_edge<VIEW, T1, T2, STG, NT, CONFIG><<<1, NT, 0, st>>>(S, B)
     if (sc.kind == KIND_DISKS) {
         const bool stg = lane_stage_fits(sc.disk);
+        const bool small = sc.disk.ns + sc.disk.nrect <= LANE_SMALL_SCENE;
         if (sc.disk.rd == 2 && sc.disk.nr == 2) {
-            if (stg) k_lane_edge<DiskView, 2, 2, true, 1024, true><<<1, 1024, 0, st>>>(sc.disk, B);
-            else k_lane_edge<DiskView, 2, 2, false, 1024, true><<<1, 1024, 0, st>>>(sc.disk, B);
+            if (small) {
+                if (stg) MRCC_LANE1(DiskView, 2, 2, true, 256, sc.disk);
+                else MRCC_LANE1(DiskView, 2, 2, false, 256, sc.disk);
+            } else if (stg) {
+                MRCC_LANE1(DiskView, 2, 2, true, 1024, sc.disk);
+            } else {
+                MRCC_LANE1(DiskView, 2, 2, false, 1024, sc.disk);
+            }
         } else if (stg) {
-            k_lane_edge<DiskView, 0, 0, true, 256, true><<<1, 256, 0, st>>>(sc.disk, B);
+            MRCC_LANE1(DiskView, 0, 0, true, 256, sc.disk);
         } else {
-            k_lane_edge<DiskView, 0, 0, false, 256, true><<<1, 256, 0, st>>>(sc.disk, B);
+            MRCC_LANE1(DiskView, 0, 0, false, 256, sc.disk);
         }
     } else if (sc.kind == KIND_PLANAR) {
         const bool stg = lane_stage_fits(sc.planar);
+        const bool small = sc.planar.no <= LANE_SMALL_SCENE;
         if (sc.planar.nr == 4) {
-            if (stg) k_lane_edge<PlanarView, 4, 0, true, 1024, true><<<1, 1024, 0, st>>>(sc.planar, B);
-            else k_lane_edge<PlanarView, 4, 0, false, 1024, true><<<1, 1024, 0, st>>>(sc.planar, B);
+            if (small) {
+                if (stg) MRCC_LANE1(PlanarView, 4, 0, true, 256, sc.planar);
+                else MRCC_LANE1(PlanarView, 4, 0, false, 256, sc.planar);
+            } else if (stg) {
+                MRCC_LANE1(PlanarView, 4, 0, true, 1024, sc.planar);
+            } else {
+                MRCC_LANE1(PlanarView, 4, 0, false, 1024, sc.planar);
+            }
         } else if (stg) {
-            k_lane_edge<PlanarView, 0, 0, true, 256, true><<<1, 256, 0, st>>>(sc.planar, B);
+            MRCC_LANE1(PlanarView, 0, 0, true, 256, sc.planar);
         } else {
-            k_lane_edge<PlanarView, 0, 0, false, 256, true><<<1, 256, 0, st>>>(sc.planar, B);
+            MRCC_LANE1(PlanarView, 0, 0, false, 256, sc.planar);
         }
     }
+#undef MRCC_LANE1
 }
+
+void launch_config1(const SceneRec& sc, const EdgeBatchView& B, hipStream_t st) { launch_lane1<true>(sc, B, st); }
 
 void launch_edge1(const SceneRec& sc, const EdgeBatchView& B, hipStream_t st) {
     switch (sc.kind) {
-        case KIND_DISKS: {
-            const bool stg = lane_stage_fits(sc.disk);
-            if (sc.disk.rd == 2 && sc.disk.nr == 2) {
-                if (stg) k_lane_edge<DiskView, 2, 2, true, 1024><<<1, 1024, 0, st>>>(sc.disk, B);
-                else k_lane_edge<DiskView, 2, 2, false, 1024><<<1, 1024, 0, st>>>(sc.disk, B);
-            } else if (stg) {
-                k_lane_edge<DiskView, 0, 0, true, 256><<<1, 256, 0, st>>>(sc.disk, B);
-            } else {
-                k_lane_edge<DiskView, 0, 0, false, 256><<<1, 256, 0, st>>>(sc.disk, B);
-            }
+        case KIND_DISKS:
+        case KIND_PLANAR:
+            launch_lane1<false>(sc, B, st);
             break;
-        }
-        case KIND_PLANAR: {
-            const bool stg = lane_stage_fits(sc.planar);
-            if (sc.planar.nr == 4) {
-                if (stg) k_lane_edge<PlanarView, 4, 0, true, 1024><<<1, 1024, 0, st>>>(sc.planar, B);
-                else k_lane_edge<PlanarView, 4, 0, false, 1024><<<1, 1024, 0, st>>>(sc.planar, B);
-            } else if (stg) {
-                k_lane_edge<PlanarView, 0, 0, true, 256><<<1, 256, 0, st>>>(sc.planar, B);
-            } else {
-                k_lane_edge<PlanarView, 0, 0, false, 256><<<1, 256, 0, st>>>(sc.planar, B);
-            }
-            break;
-        }
         case KIND_ARMS: {
             const int nw = sc.arm_nw_edge;
             const size_t lds = sc.arm_lds_edge;
